@@ -1,0 +1,454 @@
+"""coincgraph_amd: MI355X incremental graph engine (BFS / SSSP / CC / PageRank).
+
+The compute lives in libcig.so (hand-written HIP kernels for gfx950 behind the
+C-ABI of include/cig.h). This module is a thin ctypes binding plus an
+`Engine` class whose methods mirror the reference engine the samples drive
+(sepgraph::engine::Engine, include/framework/framework.cuh:456-3505):
+LoadGraph, InitGraph, Start, del_edge, add_edge, UpdatePageRank,
+cache maintenance and the Gather* result copies.
+
+There is no CPU fallback: every call goes to the HIP library, and a missing
+library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcig.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cig.h")
+
+# cig_status
+OK, EINVAL, ENOMEM, EDEVICE, ESTATE, EIO, ELIMIT, EINTERNAL = range(8)
+STATUS_NAMES = ["OK", "EINVAL", "ENOMEM", "EDEVICE", "ESTATE", "EIO", "ELIMIT", "EINTERNAL"]
+# cig_algo
+BFS, SSSP, CC, PR = range(4)
+ALGOS = {"bfs": BFS, "sssp": SSSP, "cc": CC, "pr": PR}
+# cig_weight_mode
+WEIGHT_REF128, WEIGHT_HASH64 = 0, 1
+# cig_field
+FIELD_VALUE, FIELD_PARENT, FIELD_RESIDUAL, FIELD_DEGREE, FIELD_TIER, FIELD_HOTNESS = range(6)
+# flags
+CFG_HOST_ROUNDS, CFG_NO_HOTNESS = 1, 2
+BATCH_STAGED = 1
+INF = 0xFFFFFFFF
+ABI_VERSION = 1
+
+
+class CigError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        name = STATUS_NAMES[status] if 0 <= status < len(STATUS_NAMES) else str(status)
+        super().__init__(f"{name}: {message}")
+        self.status = status
+
+
+class Edge(C.Structure):
+    _fields_ = [("u", C.c_uint32), ("v", C.c_uint32), ("w", C.c_uint32)]
+
+
+EDGE_DTYPE = np.dtype([("u", "<u4"), ("v", "<u4"), ("w", "<u4")])
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("algo", C.c_uint32), ("weight_mode", C.c_uint32),
+        ("flags", C.c_uint32), ("weight_seed", C.c_uint64), ("cache_bytes", C.c_uint64),
+        ("device", C.c_int32), ("n_stream", C.c_uint32), ("segments", C.c_uint32),
+        ("block_size", C.c_uint32), ("edges_per_lane", C.c_uint32),
+        ("split_degree", C.c_uint32), ("max_rounds", C.c_uint32),
+        ("pr_max_rounds", C.c_uint32), ("pr_epsilon", C.c_float),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double), ("mutate_ms", C.c_double), ("invalidate_ms", C.c_double),
+        ("repair_ms", C.c_double), ("seed_ms", C.c_double), ("propagate_ms", C.c_double),
+        ("cache_ms", C.c_double),
+        ("edges_scanned", C.c_uint64), ("host_edges_scanned", C.c_uint64),
+        ("vertices_expanded", C.c_uint64), ("affected", C.c_uint64), ("rounds", C.c_uint64),
+        ("deletes_applied", C.c_uint64), ("missing_deletes", C.c_uint64),
+        ("invalid_updates", C.c_uint64), ("adds_applied", C.c_uint64),
+        ("relocated_rows", C.c_uint64), ("pool_compactions", C.c_uint64),
+        ("touched_sources", C.c_uint64), ("converged", C.c_uint32), ("launches", C.c_uint32),
+        ("pr_residual_l1", C.c_double), ("stage_edges", C.c_uint64 * 4),
+        ("stage_vertices", C.c_uint64 * 4), ("stage_host_edges", C.c_uint64 * 4),
+        ("reserved", C.c_uint64 * 4),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            if name == "reserved":
+                continue
+            v = getattr(self, name)
+            out[name] = list(v) if name.startswith("stage_") else v
+        return out
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32), ("algo", C.c_uint32), ("edges", C.c_uint64),
+        ("hbm_pool_slots", C.c_uint64), ("hbm_pool_used", C.c_uint64),
+        ("host_pool_slots", C.c_uint64), ("host_pool_used", C.c_uint64),
+        ("cached_rows", C.c_uint64), ("cached_edges", C.c_uint64),
+        ("device_bytes", C.c_uint64), ("host_pinned_bytes", C.c_uint64),
+        ("grid_blocks", C.c_uint32), ("block_size", C.c_uint32),
+        ("edges_per_lane", C.c_uint32), ("split_degree", C.c_uint32),
+        ("cu_count", C.c_uint32), ("reserved", C.c_uint32 * 7),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+_lib = None
+
+
+def header_functions(path: str = HEADER_PATH) -> list:
+    """Names of the functions include/cig.h declares."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cig_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Loads libcig.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CigError(EDEVICE, f"{LIB_PATH} missing: run python -m coincgraph_amd.build")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    eng = C.c_void_p
+    sig = {
+        "cig_config_default": (C.c_int, [C.POINTER(Config)]),
+        "cig_create": (C.c_int, [C.POINTER(Config), C.POINTER(eng)]),
+        "cig_destroy": (None, [eng]),
+        "cig_last_error": (C.c_char_p, [eng]),
+        "cig_load_csr": (C.c_int, [eng, C.c_uint32, P, P]),
+        "cig_graph_info_get": (C.c_int, [eng, C.POINTER(GraphInfo)]),
+        "cig_gather_topology": (C.c_int, [eng, P, P, C.c_uint64]),
+        "cig_init": (C.c_int, [eng, P, C.c_uint32]),
+        "cig_converge": (C.c_int, [eng, C.POINTER(BatchStats)]),
+        "cig_delete_edges": (C.c_int, [eng, P, C.c_uint64, C.POINTER(BatchStats)]),
+        "cig_insert_edges": (C.c_int, [eng, P, C.c_uint64, C.POINTER(BatchStats)]),
+        "cig_apply_batch": (C.c_int, [eng, P, C.c_uint64, P, C.c_uint64, C.c_uint32,
+                                      C.POINTER(BatchStats)]),
+        "cig_load_updates": (C.c_int, [eng, P, C.c_uint64, P, C.c_uint64, P, C.c_uint32]),
+        "cig_apply_next_batch": (C.c_int, [eng, C.c_uint32, C.POINTER(BatchStats)]),
+        "cig_update_batches_left": (C.c_uint32, [eng]),
+        "cig_cache_refresh": (C.c_int, [eng, C.POINTER(BatchStats)]),
+        "cig_gather": (C.c_int, [eng, C.c_uint32, P, C.c_size_t]),
+        "cig_rmat_csr": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                   C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
+                                   C.POINTER(P), C.POINTER(P)]),
+        "cig_rmat_stream": (C.c_int, [C.c_uint32, P, P, C.c_uint32, C.c_double, C.c_double,
+                                      C.c_double, C.c_uint64, C.c_uint32, C.c_uint64,
+                                      C.c_uint64, C.c_uint32, C.POINTER(P),
+                                      C.POINTER(C.c_uint64), C.POINTER(P),
+                                      C.POINTER(C.c_uint64), C.POINTER(P)]),
+        "cig_read_market": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(P),
+                                      C.POINTER(P)]),
+        "cig_read_updates": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(P),
+                                       C.POINTER(C.c_uint64), C.POINTER(P),
+                                       C.POINTER(C.c_uint64), C.POINTER(P),
+                                       C.POINTER(C.c_uint32)]),
+        "cig_host_free": (None, [P]),
+        "cig_edge_weight": (C.c_uint32, [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
+        "cig_device_count": (C.c_int, []),
+        "cig_copy_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _take(p, count, dtype):
+    """Copies `count` elements from a library-malloc'd buffer and frees it."""
+    if not p.value:
+        return np.zeros(0, dtype=dtype)
+    nbytes = int(count) * np.dtype(dtype).itemsize
+    out = np.empty(count, dtype=dtype)
+    if nbytes:
+        C.memmove(out.ctypes.data, p.value, nbytes)
+    lib().cig_host_free(p)
+    return out
+
+
+def _check(status):
+    if status != OK:
+        raise CigError(status, "host utility failed")
+
+
+def edges_array(u, v, w=None) -> np.ndarray:
+    u = np.asarray(u, dtype=np.uint32)
+    v = np.asarray(v, dtype=np.uint32)
+    a = np.empty(len(u), dtype=EDGE_DTYPE)
+    a["u"] = u
+    a["v"] = v
+    a["w"] = 1 if w is None else np.asarray(w, dtype=np.uint32)
+    return a
+
+
+def device_count() -> int:
+    return int(lib().cig_device_count())
+
+
+def copy_probe(bytes_: int, iters: int = 20, device: int = 0) -> float:
+    """Mean milliseconds of a plain copy kernel moving `bytes_` bytes."""
+    ms = C.c_double()
+    _check(lib().cig_copy_probe(device, bytes_, iters, C.byref(ms)))
+    return ms.value
+
+
+def edge_weight(mode: int, seed: int, u: int, v: int) -> int:
+    return int(lib().cig_edge_weight(mode, seed, u, v))
+
+
+# ---------------------------------------------------------------- host utils
+@dataclass
+class HostGraph:
+    n: int
+    offsets: np.ndarray  # uint64, n+1
+    dst: np.ndarray      # uint32
+
+    @property
+    def m(self) -> int:
+        return int(self.offsets[-1])
+
+
+@dataclass
+class UpdateStream:
+    adds: np.ndarray        # EDGE_DTYPE
+    dels: np.ndarray        # EDGE_DTYPE
+    batch_sizes: np.ndarray  # uint32 (nbatches, 2): adds, dels
+
+    @property
+    def nbatches(self) -> int:
+        return int(self.batch_sizes.shape[0])
+
+    def batch(self, b: int):
+        sizes = self.batch_sizes
+        a0 = int(sizes[:b, 0].sum())
+        d0 = int(sizes[:b, 1].sum())
+        return (self.dels[d0:d0 + int(sizes[b, 1])], self.adds[a0:a0 + int(sizes[b, 0])])
+
+
+RMAT_ABC = (0.57, 0.19, 0.19)
+
+
+def rmat_graph(scale: int, edge_factor: int = 16, seed: int = 1, symmetric: bool = False,
+               abc=RMAT_ABC, threads: int = 0) -> HostGraph:
+    n = C.c_uint32()
+    off = C.c_void_p()
+    dst = C.c_void_p()
+    _check(lib().cig_rmat_csr(scale, edge_factor, abc[0], abc[1], abc[2], seed,
+                              1 if symmetric else 0, threads, C.byref(n), C.byref(off),
+                              C.byref(dst)))
+    offsets = _take(off, n.value + 1, np.uint64)
+    return HostGraph(n.value, offsets, _take(dst, int(offsets[-1]), np.uint32))
+
+
+def rmat_stream(g: HostGraph, scale: int, nbatches: int, n_del: int, n_add: int, seed: int = 2,
+                symmetric: bool = False, abc=RMAT_ABC) -> UpdateStream:
+    adds = C.c_void_p()
+    dels = C.c_void_p()
+    sizes = C.c_void_p()
+    na = C.c_uint64()
+    nd = C.c_uint64()
+    _check(lib().cig_rmat_stream(g.n, _ptr(g.offsets), _ptr(g.dst) if g.m else None, scale,
+                                 abc[0], abc[1], abc[2], seed, nbatches, n_del, n_add,
+                                 1 if symmetric else 0, C.byref(adds), C.byref(na),
+                                 C.byref(dels), C.byref(nd), C.byref(sizes)))
+    return UpdateStream(_take(adds, na.value, EDGE_DTYPE), _take(dels, nd.value, EDGE_DTYPE),
+                        _take(sizes, 2 * nbatches, np.uint32).reshape(nbatches, 2))
+
+
+def read_market(path: str) -> HostGraph:
+    n = C.c_uint32()
+    off = C.c_void_p()
+    dst = C.c_void_p()
+    _check(lib().cig_read_market(path.encode(), C.byref(n), C.byref(off), C.byref(dst)))
+    offsets = _take(off, n.value + 1, np.uint64)
+    return HostGraph(n.value, offsets, _take(dst, int(offsets[-1]), np.uint32))
+
+
+def read_updates(update_path: str, batch_path: str) -> UpdateStream:
+    adds = C.c_void_p()
+    dels = C.c_void_p()
+    sizes = C.c_void_p()
+    na = C.c_uint64()
+    nd = C.c_uint64()
+    nb = C.c_uint32()
+    _check(lib().cig_read_updates(update_path.encode(), batch_path.encode(), C.byref(adds),
+                                  C.byref(na), C.byref(dels), C.byref(nd), C.byref(sizes),
+                                  C.byref(nb)))
+    return UpdateStream(_take(adds, na.value, EDGE_DTYPE), _take(dels, nd.value, EDGE_DTYPE),
+                        _take(sizes, 2 * nb.value, np.uint32).reshape(nb.value, 2))
+
+
+# ---------------------------------------------------------------- engine
+class Engine:
+    """One engine per algorithm instance (reference: Engine<..., TApp, ...>)."""
+
+    def __init__(self, algo="sssp", weight_mode=WEIGHT_REF128, weight_seed=0, cache_bytes=0,
+                 device=0, pr_epsilon=1e-6, pr_max_rounds=100, flags=0, edges_per_lane=0,
+                 split_degree=0, max_rounds=0, n_stream=3, segments=1):
+        L = lib()
+        cfg = Config()
+        _check(L.cig_config_default(C.byref(cfg)))
+        cfg.algo = ALGOS[algo] if isinstance(algo, str) else int(algo)
+        cfg.weight_mode = weight_mode
+        cfg.weight_seed = weight_seed
+        cfg.cache_bytes = cache_bytes
+        cfg.device = device
+        cfg.pr_epsilon = pr_epsilon
+        cfg.pr_max_rounds = pr_max_rounds
+        cfg.flags = flags
+        cfg.edges_per_lane = edges_per_lane
+        cfg.split_degree = split_degree
+        cfg.max_rounds = max_rounds
+        cfg.n_stream = n_stream
+        cfg.segments = segments
+        self.config = cfg
+        self._e = C.c_void_p()
+        st = L.cig_create(C.byref(cfg), C.byref(self._e))
+        if st != OK:
+            raise CigError(st, L.cig_last_error(None).decode())
+        self.algo = cfg.algo
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_e", None) and self._e.value:
+            lib().cig_destroy(self._e)
+            self._e = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _call(self, status):
+        if status != OK:
+            raise CigError(status, lib().cig_last_error(self._e).decode())
+
+    # LoadGraph
+    def load_graph(self, g: HostGraph):
+        off = np.ascontiguousarray(g.offsets, dtype=np.uint64)
+        dst = np.ascontiguousarray(g.dst, dtype=np.uint32)
+        self._call(lib().cig_load_csr(self._e, g.n, _ptr(off), _ptr(dst) if dst.size else None))
+        self.n = g.n
+
+    # InitGraph
+    def init_graph(self, source: int = 0):
+        src = np.array([source], dtype=np.uint32)
+        self._call(lib().cig_init(self._e, _ptr(src), 1))
+
+    # Start / StartPageRank
+    def start(self) -> dict:
+        st = BatchStats()
+        self._call(lib().cig_converge(self._e, C.byref(st)))
+        return st.as_dict()
+
+    def _edges(self, a):
+        a = np.ascontiguousarray(a, dtype=EDGE_DTYPE)
+        return a, (_ptr(a) if a.size else None)
+
+    # del_edge
+    def del_edge(self, dels) -> dict:
+        d, p = self._edges(dels)
+        st = BatchStats()
+        self._call(lib().cig_delete_edges(self._e, p, d.size, C.byref(st)))
+        return st.as_dict()
+
+    # add_edge
+    def add_edge(self, adds) -> dict:
+        a, p = self._edges(adds)
+        st = BatchStats()
+        self._call(lib().cig_insert_edges(self._e, p, a.size, C.byref(st)))
+        return st.as_dict()
+
+    # one batch: deletes then adds (UpdatePageRank for PR)
+    def apply_batch(self, dels, adds, staged: bool = False) -> dict:
+        d, pd = self._edges(dels)
+        a, pa = self._edges(adds)
+        st = BatchStats()
+        self._call(lib().cig_apply_batch(self._e, pd, d.size, pa, a.size,
+                                         BATCH_STAGED if staged else 0, C.byref(st)))
+        return st.as_dict()
+
+    # get_update_file
+    def load_updates(self, stream: UpdateStream):
+        a = np.ascontiguousarray(stream.adds, dtype=EDGE_DTYPE)
+        d = np.ascontiguousarray(stream.dels, dtype=EDGE_DTYPE)
+        s = np.ascontiguousarray(stream.batch_sizes, dtype=np.uint32)
+        self._call(lib().cig_load_updates(self._e, _ptr(a) if a.size else None, a.size,
+                                          _ptr(d) if d.size else None, d.size, _ptr(s),
+                                          stream.nbatches))
+
+    def apply_next_batch(self, staged: bool = False) -> dict:
+        st = BatchStats()
+        self._call(lib().cig_apply_next_batch(self._e, BATCH_STAGED if staged else 0,
+                                              C.byref(st)))
+        return st.as_dict()
+
+    def batches_left(self) -> int:
+        return int(lib().cig_update_batches_left(self._e))
+
+    # compute_hot_vertices + confirm_candidate_batch + evict/compact/LoadCache
+    def cache_refresh(self) -> dict:
+        st = BatchStats()
+        self._call(lib().cig_cache_refresh(self._e, C.byref(st)))
+        return st.as_dict()
+
+    def info(self) -> dict:
+        gi = GraphInfo()
+        self._call(lib().cig_graph_info_get(self._e, C.byref(gi)))
+        return gi.as_dict()
+
+    # GatherValue / GatherParent / GatherBuffer
+    def gather(self, field: int) -> np.ndarray:
+        if field == FIELD_TIER:
+            out = np.empty(self.n, dtype=np.uint8)
+        elif self.algo == PR and field in (FIELD_VALUE, FIELD_RESIDUAL):
+            out = np.empty(self.n, dtype=np.float32)
+        else:
+            out = np.empty(self.n, dtype=np.uint32)
+        self._call(lib().cig_gather(self._e, field, _ptr(out), out.nbytes))
+        return out
+
+    def values(self) -> np.ndarray:
+        return self.gather(FIELD_VALUE)
+
+    def parents(self) -> np.ndarray:
+        return self.gather(FIELD_PARENT)
+
+    def residuals(self) -> np.ndarray:
+        return self.gather(FIELD_RESIDUAL)
+
+    def topology(self) -> HostGraph:
+        info = self.info()
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        cap = max(int(info["edges"]), 1)
+        dst = np.zeros(cap, dtype=np.uint32)
+        self._call(lib().cig_gather_topology(self._e, _ptr(off), _ptr(dst), cap))
+        return HostGraph(self.n, off, dst[: int(off[-1])])

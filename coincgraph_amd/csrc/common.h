@@ -1,0 +1,89 @@
+// Shared host/device declarations of libcig.
+#pragma once
+#include <cstdint>
+
+namespace cig {
+
+uint64_t splitmix64(uint64_t x);
+
+// Per-vertex row descriptor of the two-tier segmented CSR (16 B, one
+// dwordx4 load). `off` is a slot offset into the HBM pool, or into the pinned
+// host pool when bit 63 is set. Rows own `cap` slots; `deg` are live.
+struct Row {
+    uint64_t off;
+    uint32_t deg;
+    uint32_t cap;
+};
+static_assert(sizeof(Row) == 16, "Row must stay 16 bytes");
+
+constexpr uint64_t kHostTier = 1ull << 63;
+constexpr uint32_t kInf = 0xFFFFFFFFu;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kMinRowCap = 4;  // RequiredChunkCapacity minimum (source_local_chunk_store.h:131-144)
+
+inline uint64_t next_pow2_cap(uint64_t deg) {
+    if (deg == 0) return 0;
+    uint64_t c = deg < kMinRowCap ? kMinRowCap : deg;
+    --c;
+    c |= c >> 1; c |= c >> 2; c |= c >> 4; c |= c >> 8; c |= c >> 16; c |= c >> 32;
+    return c + 1;
+}
+
+// One tile of a long row, expanded by one wave.
+struct Tile {
+    uint64_t start;   // pool offset (tier bit included) of the first slot
+    uint32_t v;       // row owner
+    uint32_t len;     // slots in this tile
+    uint32_t payload; // op payload (PR contribution bits); unused by traversal
+    uint32_t pad;
+};
+static_assert(sizeof(Tile) == 24, "Tile layout");
+
+// Copy job for relocating a long row to a new block.
+struct CopyJob {
+    uint64_t from;  // pool offset with tier bit
+    uint64_t to;
+    uint32_t len;
+    uint32_t pad;
+};
+
+// Device control block: counters of the current batch and of the fixed-point
+// rounds. Lives in device memory; zeroed per batch by a fill kernel.
+struct Ctl {
+    // fixed-point rounds (relax / PR); slot = absolute round % 3
+    uint32_t qcount[3];
+    uint32_t tcount[3];
+    uint32_t round;        // next absolute round to run
+    uint32_t converged;
+    // invalidation
+    uint32_t acount;       // affected list length
+    uint32_t icount[3];    // invalidation tile counts
+    uint32_t scount;       // one-shot scan tile count
+    // mutation
+    uint32_t touched;
+    uint32_t small_cnt, big_cnt, add_cnt, copy_cnt, overflow_cnt;
+    uint32_t pr_invalid;
+    uint64_t group_cursor;
+    uint64_t pool_tail[2]; // bump tails of the HBM (0) and host (1) pools
+    uint64_t pool_cap[2];
+    uint64_t garbage[2];   // slots abandoned by relocations
+    // statistics of the current call
+    uint64_t edges_scanned;
+    uint64_t host_edges;
+    uint64_t vexp;
+    uint64_t deletes_applied;
+    uint64_t missing;
+    uint64_t invalid;
+    uint64_t adds_applied;
+    uint64_t relocated;
+    uint64_t rounds;
+    double residual_l1;
+    // per expansion pass: 0 invalidation, 1 repair scan, 2 propagation, 3 PR amend
+    uint64_t st_edges[4];
+    uint64_t st_vexp[4];
+    uint64_t st_host[4];
+};
+
+enum Stage : uint32_t { kStageInvalidate = 0, kStageRepair = 1, kStagePropagate = 2, kStageAmend = 3 };
+
+}  // namespace cig

@@ -1,0 +1,1299 @@
+// libcig: MI355X incremental graph engine behind the C-ABI of include/cig.h.
+//
+// Memory layout in HBM (n vertices):
+//   rows[n]        16 B  {pool offset | tier bit, degree, capacity}
+//   HBM pool            4-B destination slots; each row owns a power-of-two
+//                       block (min 4), rows grow by relocating to the tail
+//   pinned host pool    same layout for rows outside the HBM budget; kernels
+//                       read them zero-copy over the host link
+//   state[n]       8 B  traversal: value<<32 | parent   (SSSP/BFS/CC)
+//   rank/comp/res  4 B each (PR, Kahan-compensated rank)
+//   q[2][n], stamp[n], affected bitmap, affected list, tiles[2], counters.
+// Every batch runs as a chain of kernels on one stream with no host round
+// trip; the host reads a 200-B control block at the end of the call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cig.h"
+#include "common.h"
+#include "kernels.hpp"
+
+using namespace cig;
+using namespace cig::dev;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    cig_status code;
+    HipError(cig_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (call);                                                             \
+        if (_e != hipSuccess)                                                               \
+            throw HipError(_e == hipErrorOutOfMemory ? CIG_ENOMEM : CIG_EDEVICE,            \
+                           std::string(#call) + ": " + hipGetErrorString(_e));              \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t count = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        count = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        const size_t bytes = std::max<size_t>(1, n) * sizeof(T);
+        HIPCHK(hipMalloc(&p, bytes));
+        count = n;
+    }
+    size_t bytes() const { return count * sizeof(T); }
+};
+
+template <class T>
+struct HBuf {  // pinned host memory (hipHostMalloc; never hipHostRegister)
+    T *p = nullptr;
+    size_t count = 0;
+    HBuf() = default;
+    HBuf(const HBuf &) = delete;
+    HBuf &operator=(const HBuf &) = delete;
+    ~HBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        count = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        HIPCHK(hipHostMalloc(&p, std::max<size_t>(1, n) * sizeof(T), hipHostMallocDefault));
+        count = n;
+    }
+    T *device_ptr() const {
+        void *d = nullptr;
+        HIPCHK(hipHostGetDevicePointer(&d, p, 0));
+        return static_cast<T *>(d);
+    }
+};
+
+constexpr uint32_t kTileLenDefault = 1024;
+constexpr uint32_t kSplitDefault = 256;
+constexpr uint32_t kEplDefault = 4;
+
+}  // namespace
+
+struct cig_engine {
+    cig_config cfg{};
+    std::string err;
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+
+    uint32_t n = 0;
+    bool loaded = false, initialized = false;
+    uint32_t source = 0;
+
+    DBuf<Row> rows;
+    DBuf<uint64_t> state;
+    DBuf<float> rank, comp, res, delta;
+    DBuf<uint32_t> q0, q1, stamp, aff, alist, hot, hist, score;
+    DBuf<Tile> t0, t1;
+    uint64_t tile_cap = 0;
+    DBuf<uint64_t> cnt64, fill64, gofs;
+    DBuf<uint32_t> touched, grouped, small_list, big_list, add_list, overflow;
+    DBuf<CopyJob> jobs;
+    uint64_t rec_cap = 0;
+    DBuf<cig_edge> rec_dels, rec_adds;
+    DBuf<Ctl> ctl;
+    HBuf<Ctl> hctl;
+
+    DBuf<uint32_t> hbm_pool;
+    HBuf<uint32_t> host_pool;
+    uint32_t *host_pool_dev = nullptr;
+    uint64_t hbm_slots = 0, host_slots = 0;
+    uint64_t budget_slots = 0;  // 0 = unlimited
+
+    // resident update stream
+    DBuf<cig_edge> s_adds, s_dels;
+    std::vector<uint32_t> batch_sizes;
+    uint32_t next_batch = 0;
+    uint64_t add_off = 0, del_off = 0;
+
+    // launch shape
+    uint32_t split = kSplitDefault, tile_len = kTileLenDefault, epl = kEplDefault;
+    uint32_t grid_small = 0;   // one-shot kernels
+    uint32_t grid_coop = 0;    // persistent cooperative kernels
+    uint32_t grid_delbig = 0;
+    bool coop = true;
+
+    // ------------------------------------------------------------ plumbing
+    Pools pools() const { return Pools{hbm_pool.p, host_pool_dev}; }
+    ExpandCfg ecfg(uint32_t stage) const { return ExpandCfg{pools(), split, tile_len, stage}; }
+    uint32_t weight_kind() const {
+        switch (cfg.algo) {
+        case CIG_ALGO_BFS: return 2;
+        case CIG_ALGO_CC: return 3;
+        default: return cfg.weight_mode == CIG_WEIGHT_HASH64 ? 1u : 0u;
+        }
+    }
+    bool is_pr() const { return cfg.algo == CIG_ALGO_PR; }
+
+    void setup_device() {
+        HIPCHK(hipSetDevice(cfg.device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, cfg.device));
+        cus = prop.multiProcessorCount;
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        ctl.alloc(1);
+        hctl.alloc(1);
+        std::memset(hctl.p, 0, sizeof(Ctl));
+        HIPCHK(hipMemcpyAsync(ctl.p, hctl.p, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+        if (cfg.split_degree) split = std::max<uint32_t>(64, cfg.split_degree);
+        if (cfg.edges_per_lane) epl = cfg.edges_per_lane >= 8 ? 8 : cfg.edges_per_lane >= 4 ? 4
+                                        : cfg.edges_per_lane >= 2 ? 2 : 1;
+        coop = !(cfg.flags & CIG_CFG_HOST_ROUNDS) && prop.cooperativeLaunch;
+        grid_small = uint32_t(cus) * 8;
+        grid_delbig = uint32_t(cus) * 2;
+    }
+
+    void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+
+    template <class K>
+    uint32_t coop_grid_for(K kernel) {
+        int per_cu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kernel), kBlock, 0));
+        per_cu = std::max(1, std::min(per_cu, 4));
+        return uint32_t(per_cu) * uint32_t(cus);
+    }
+
+    template <class K, class A>
+    void launch_coop(K kernel, A &args, uint32_t grid) {
+        void *kargs[] = {&args};
+        HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kernel), dim3(grid), dim3(kBlock),
+                                          kargs, 0, stream));
+    }
+
+    void fill32(uint32_t *p, uint64_t count, uint32_t value) {
+        if (!count) return;
+        k_fill32<<<grid_small, 256, 0, stream>>>(p, count, value);
+        HIPCHK(hipGetLastError());
+    }
+
+    uint32_t round_bound() const { return uint32_t(std::min<uint64_t>(2ull * n + 64, 0x7FFFFFFFull)); }
+
+    // ------------------------------------------------------------ load
+    void load(uint32_t nv, const uint64_t *off, const uint32_t *dst) {
+        if (nv == 0) throw HipError(CIG_EINVAL, "graph must have at least one vertex");
+        for (uint32_t v = 0; v < nv; ++v)
+            if (off[v + 1] < off[v] || off[v + 1] - off[v] > 0xFFFFFFFFull)
+                throw HipError(CIG_EINVAL, "offsets must be non-decreasing, row length < 2^32");
+        const uint64_t m = off[nv];
+        for (uint64_t e = 0; e < m; ++e)
+            if (dst[e] >= nv) throw HipError(CIG_EINVAL, "destination id out of range");
+        n = nv;
+        budget_slots = cfg.cache_bytes / sizeof(uint32_t);
+
+        std::vector<Row> hrows(n);
+        uint64_t total_cap = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            hrows[v].deg = uint32_t(off[v + 1] - off[v]);
+            hrows[v].cap = uint32_t(next_pow2_cap(hrows[v].deg));
+            total_cap += hrows[v].cap;
+        }
+        // tier placement: longest rows first while the budget lasts (90% fill)
+        std::vector<uint8_t> in_hbm(n, 1);
+        uint64_t hbm_need = total_cap, host_need = 0;
+        if (budget_slots && total_cap > budget_slots * 9 / 10) {
+            std::vector<uint32_t> order(n);
+            std::iota(order.begin(), order.end(), 0u);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](uint32_t a, uint32_t b) { return hrows[a].deg > hrows[b].deg; });
+            uint64_t used = 0;
+            const uint64_t limit = budget_slots * 9 / 10;
+            for (uint32_t v : order) {
+                const bool fits = hrows[v].deg != 0 && used + hrows[v].cap <= limit;
+                in_hbm[v] = fits ? 1 : 0;
+                if (fits) used += hrows[v].cap;
+            }
+            hbm_need = used;
+            host_need = total_cap - used;
+        }
+        hbm_slots = budget_slots ? budget_slots : hbm_need + hbm_need / 4 + (uint64_t(1) << 22);
+        host_slots = host_need ? host_need + host_need / 4 + (uint64_t(1) << 20) : (uint64_t(1) << 16);
+        hbm_pool.alloc(hbm_slots);
+        host_pool.alloc(host_slots);
+        host_pool_dev = host_pool.device_ptr();
+        std::vector<uint32_t> stage;
+        stage.reserve(hbm_need);
+        uint64_t htail = 0, ctail = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            Row &r = hrows[v];
+            if (in_hbm[v]) {
+                r.off = htail;
+                stage.insert(stage.end(), dst + off[v], dst + off[v + 1]);
+                stage.resize(htail + r.cap, 0u);
+                htail += r.cap;
+            } else {
+                r.off = ctail | kHostTier;
+                std::memcpy(host_pool.p + ctail, dst + off[v], size_t(r.deg) * sizeof(uint32_t));
+                ctail += r.cap;
+            }
+        }
+        if (!stage.empty())
+            HIPCHK(hipMemcpyAsync(hbm_pool.p, stage.data(), stage.size() * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, stream));
+        rows.alloc(n);
+        HIPCHK(hipMemcpyAsync(rows.p, hrows.data(), n * sizeof(Row), hipMemcpyHostToDevice, stream));
+
+        // per-vertex arrays
+        q0.alloc(n);
+        q1.alloc(n);
+        stamp.alloc(n);
+        hot.alloc(n);
+        hist.alloc(n);
+        score.alloc(n);
+        cnt64.alloc(n);
+        fill64.alloc(n);
+        gofs.alloc(n);
+        touched.alloc(n);
+        small_list.alloc(n);
+        big_list.alloc(n);
+        add_list.alloc(n);
+        overflow.alloc(n);
+        jobs.alloc(n);
+        if (is_pr()) {
+            rank.alloc(n);
+            comp.alloc(n);
+            res.alloc(n);
+            delta.alloc(n);
+        } else {
+            state.alloc(n);
+            aff.alloc((uint64_t(n) + 31) / 32);
+            alist.alloc(n);
+        }
+        ensure_tiles();
+        fill32(stamp.p, n, 0);
+        fill32(hot.p, n, 0);
+        fill32(hist.p, n, 0);
+        fill32(reinterpret_cast<uint32_t *>(cnt64.p), uint64_t(n) * 2, 0);
+        fill32(reinterpret_cast<uint32_t *>(fill64.p), uint64_t(n) * 2, 0);
+        if (!is_pr()) fill32(aff.p, (uint64_t(n) + 31) / 32, 0);
+        // control block: pools
+        std::memset(hctl.p, 0, sizeof(Ctl));
+        hctl.p->round = 1;
+        hctl.p->pool_tail[0] = htail;
+        hctl.p->pool_tail[1] = ctail;
+        hctl.p->pool_cap[0] = hbm_slots;
+        hctl.p->pool_cap[1] = host_slots;
+        HIPCHK(hipMemcpyAsync(ctl.p, hctl.p, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+        sync();
+        loaded = true;
+        initialized = false;
+    }
+
+    // Tile lists hold at most one round's worth of tiles: every row of the
+    // round at most once, so sum(deg)/tile_len + rows >= split.
+    void ensure_tiles() {
+        const uint64_t slots = hbm_slots + host_slots;
+        const uint64_t need = slots / tile_len + slots / split + 1024;
+        if (need <= tile_cap) return;
+        t0.alloc(need);
+        t1.alloc(need);
+        tile_cap = need;
+    }
+
+    void ensure_records(uint64_t nd, uint64_t na) {
+        const uint64_t need = std::max(nd, na);
+        if (rec_dels.count < need) rec_dels.alloc(need);
+        if (rec_adds.count < need) rec_adds.alloc(need);
+        if (grouped.count < nd + na) grouped.alloc(nd + na);
+    }
+
+    // ------------------------------------------------------------ control
+    void read_ctl() {
+        HIPCHK(hipMemcpyAsync(hctl.p, ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+
+    void reset_ctl(bool keep_rounds) {
+        k_reset_ctl<<<1, 64, 0, stream>>>(ctl.p, keep_rounds ? 1u : 0u);
+        HIPCHK(hipGetLastError());
+    }
+
+    LoopArgs loop_args(uint32_t max_rounds) {
+        LoopArgs a{};
+        a.ctl = ctl.p;
+        a.rows = rows.p;
+        a.state = state.p;
+        a.stamp = stamp.p;
+        a.hot = (cfg.flags & CIG_CFG_NO_HOTNESS) ? nullptr : hot.p;
+        a.q0 = q0.p;
+        a.q1 = q1.p;
+        a.t0 = t0.p;
+        a.t1 = t1.p;
+        a.cfg = ecfg(kStagePropagate);
+        a.seed = cfg.weight_seed;
+        a.max_rounds = max_rounds;
+        a.coop = coop ? 1u : 0u;
+        return a;
+    }
+
+    template <int W, bool S, int EPL>
+    void relax_impl(uint32_t launches_cap) {
+        auto kernel = k_relax_loop<W, S, EPL>;
+        if (coop) {
+            static thread_local uint32_t grid = 0;
+            if (!grid) grid = coop_grid_for(kernel);
+            grid_coop = grid;
+            // A launch stops at the round cap; relaunch until the frontier is empty.
+            // Without a configured cap the bound is 2n+64 rounds: a frontier
+            // round finalises every vertex one more hop away (one more for a
+            // deferred tile), so a larger count means a broken invariant.
+            const uint32_t cap = cfg.max_rounds ? cfg.max_rounds : round_bound();
+            LoopArgs a = loop_args(cap);
+            launch_coop(kernel, a, grid);
+            if (cfg.max_rounds) {
+                for (uint32_t i = 0; i < launches_cap; ++i) {
+                    read_ctl();
+                    const uint32_t r = hctl.p->round;
+                    if (!hctl.p->qcount[r % 3] && !hctl.p->tcount[r % 3]) break;
+                    launch_coop(kernel, a, grid);
+                }
+            }
+        } else {
+            LoopArgs a = loop_args(1);
+            for (uint64_t done = 0;; done += 8) {
+                if (done > round_bound()) throw HipError(CIG_EINTERNAL, "fixed point did not converge");
+                for (int k = 0; k < 8; ++k) {
+                    kernel<<<grid_small, kBlock, 0, stream>>>(a);
+                    k_advance_round<<<1, 64, 0, stream>>>(ctl.p);
+                }
+                HIPCHK(hipGetLastError());
+                read_ctl();
+                const uint32_t r = hctl.p->round;
+                if (!hctl.p->qcount[r % 3] && !hctl.p->tcount[r % 3]) break;
+            }
+        }
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int EPL>
+    void relax_dispatch_epl() {
+        switch (weight_kind()) {
+        case 0: relax_impl<0, false, EPL>(1u << 20); break;
+        case 1: relax_impl<1, false, EPL>(1u << 20); break;
+        case 2: relax_impl<2, false, EPL>(1u << 20); break;
+        default: relax_impl<3, true, EPL>(1u << 20); break;
+        }
+    }
+
+    void relax() {
+        switch (epl) {
+        case 1: relax_dispatch_epl<1>(); break;
+        case 2: relax_dispatch_epl<2>(); break;
+        case 8: relax_dispatch_epl<8>(); break;
+        default: relax_dispatch_epl<4>(); break;
+        }
+    }
+
+    PrArgs pr_args(uint32_t max_rounds) {
+        PrArgs a{};
+        a.ctl = ctl.p;
+        a.rows = rows.p;
+        a.rank = rank.p;
+        a.comp = comp.p;
+        a.res = res.p;
+        a.delta = delta.p;
+        a.queued = stamp.p;
+        a.q0 = q0.p;
+        a.q1 = q1.p;
+        a.t0 = t0.p;
+        a.t1 = t1.p;
+        a.cfg = ecfg(kStagePropagate);
+        a.eps = cfg.pr_epsilon;
+        a.max_rounds = max_rounds;
+        return a;
+    }
+
+    void pr_converge() {
+        if (!coop) throw HipError(CIG_EDEVICE, "PageRank rounds need cooperative launch support");
+        auto kernel = k_pr_loop<4>;
+        static thread_local uint32_t grid = 0;
+        if (!grid) grid = coop_grid_for(kernel);
+        grid_coop = grid;
+        PrArgs a = pr_args(cfg.pr_max_rounds);
+        launch_coop(kernel, a, grid);
+        HIPCHK(hipGetLastError());
+    }
+
+    void pr_residual_norm() {
+        k_pr_l1<<<grid_small, 256, 0, stream>>>(res.p, n, ctl.p);
+        HIPCHK(hipGetLastError());
+    }
+
+    // ------------------------------------------------------------ init
+    void init(const uint32_t *sources, uint32_t nsrc) {
+        if (!loaded) throw HipError(CIG_ESTATE, "cig_load_csr must come first");
+        reset_ctl(false);
+        if (is_pr()) {
+            k_init_pr<<<grid_small, 256, 0, stream>>>(rank.p, comp.p, res.p, stamp.p, q1.p, n, ctl.p);
+        } else if (cfg.algo == CIG_ALGO_CC) {
+            k_init_traversal<<<grid_small, 256, 0, stream>>>(state.p, stamp.p, n, 1, 0, q1.p, ctl.p);
+        } else {
+            if (!sources || nsrc != 1) throw HipError(CIG_EINVAL, "SSSP/BFS take exactly one source");
+            if (sources[0] >= n) throw HipError(CIG_EINVAL, "source out of range");
+            source = sources[0];
+            fill32(stamp.p, n, 0);
+            k_init_traversal<<<grid_small, 256, 0, stream>>>(state.p, stamp.p, n, 0, source, q1.p, ctl.p);
+        }
+        HIPCHK(hipGetLastError());
+        sync();
+        initialized = true;
+    }
+
+    // ------------------------------------------------------------ stats
+    void begin_call(cig_batch_stats *st) {
+        if (st) std::memset(st, 0, sizeof(*st));
+        reset_ctl(true);
+        HIPCHK(hipEventRecord(ev[0], stream));
+    }
+
+    float elapsed(int a, int b) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+        return ms;
+    }
+
+    void end_call(cig_batch_stats *st, int marks) {
+        HIPCHK(hipEventRecord(ev[7], stream));
+        read_ctl();
+        if (hctl.p->pr_invalid) throw HipError(CIG_EINTERNAL, "PR non-finite rank/residual");
+        if (!is_pr() && !cfg.max_rounds) {
+            const uint32_t r = hctl.p->round;
+            if (hctl.p->qcount[r % 3] || hctl.p->tcount[r % 3])
+                throw HipError(CIG_EINTERNAL, "fixed point did not converge within the round bound");
+        }
+        if (!st) return;
+        const Ctl &c = *hctl.p;
+        st->total_ms = elapsed(0, 7);
+        if (marks >= 4) {  // ev: 0 start, 1 mutated, 2 invalidated, 3 repaired, 5 seeded, 4 propagated
+            st->mutate_ms = elapsed(0, 1);
+            st->invalidate_ms = elapsed(1, 2);
+            st->repair_ms = elapsed(2, 3);
+            st->seed_ms = elapsed(3, 5);
+            st->propagate_ms = elapsed(5, 4);
+        }
+        for (int i = 0; i < 4; ++i) {
+            st->stage_edges[i] = c.st_edges[i];
+            st->stage_vertices[i] = c.st_vexp[i];
+            st->stage_host_edges[i] = c.st_host[i];
+        }
+        st->edges_scanned = c.edges_scanned;
+        st->host_edges_scanned = c.host_edges;
+        st->vertices_expanded = c.vexp;
+        st->affected = c.acount;
+        st->rounds = c.rounds;
+        st->deletes_applied = c.deletes_applied;
+        st->missing_deletes = c.missing;
+        st->invalid_updates = c.invalid;
+        st->adds_applied = c.adds_applied;
+        st->relocated_rows = c.relocated;
+        st->touched_sources = c.touched;
+        st->converged = is_pr() ? c.converged : 1u;
+        st->pr_residual_l1 = c.residual_l1;
+    }
+
+    // ------------------------------------------------------------ mutation
+    MutArgs mut_args(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na) {
+        MutArgs m{};
+        m.ctl = ctl.p;
+        m.rows = rows.p;
+        m.pools = pools();
+        m.n = n;
+        m.dels = d;
+        m.nd = nd;
+        m.adds = a;
+        m.na = na;
+        m.cnt64 = cnt64.p;
+        m.fill64 = fill64.p;
+        m.gofs = gofs.p;
+        m.touched = touched.p;
+        m.grouped = grouped.p;
+        m.small_list = small_list.p;
+        m.big_list = big_list.p;
+        m.add_list = add_list.p;
+        m.overflow = overflow.p;
+        m.jobs = jobs.p;
+        return m;
+    }
+
+    void mutate_group(const MutArgs &m) {
+        k_count<<<grid_small, 256, 0, stream>>>(m);
+        k_plan<<<grid_small, 256, 0, stream>>>(m);
+        k_scatter<<<grid_small, 256, 0, stream>>>(m);
+        HIPCHK(hipGetLastError());
+    }
+
+    void mutate_apply(const MutArgs &m) {
+        k_del_small<<<grid_small, 256, 0, stream>>>(m);
+        k_del_big<<<grid_delbig, kDelBigThreads, 0, stream>>>(m);
+        k_add<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
+        k_copy_jobs<<<grid_small, 256, 0, stream>>>(m);
+        HIPCHK(hipGetLastError());
+    }
+
+    void mutate_finish(const MutArgs &m) {
+        k_clear_touched<<<grid_small, 256, 0, stream>>>(m);
+        HIPCHK(hipGetLastError());
+    }
+
+    // Rows that found no room: reclaim garbage (compaction), demote to the
+    // host tier if HBM is still short, then append their adds again.
+    bool recover_overflow(const MutArgs &m) {
+        read_ctl();
+        const uint32_t nover = hctl.p->overflow_cnt;
+        if (!nover) return false;
+        std::vector<uint32_t> list(nover);
+        HIPCHK(hipMemcpyAsync(list.data(), overflow.p, nover * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        std::vector<uint64_t> cnts(nover);
+        for (uint32_t i = 0; i < nover; ++i)
+            HIPCHK(hipMemcpyAsync(&cnts[i], cnt64.p + list[i], sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::vector<Row> hrows(n);
+        HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+        uint64_t need[2] = {0, 0};
+        for (uint32_t i = 0; i < nover; ++i) {
+            const Row &r = hrows[list[i]];
+            need[(r.off & kHostTier) ? 1 : 0] += next_pow2_cap(uint64_t(r.deg) + uint32_t(cnts[i] >> 32));
+        }
+        for (int tier = 0; tier < 2; ++tier)
+            if (need[tier]) compact_pool(tier, need[tier]);
+        // reset the overflow list and copy jobs; retry the adds of those rows
+        HIPCHK(hipMemcpyAsync(add_list.p, list.data(), nover * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        uint32_t zero = 0;
+        HIPCHK(hipMemcpyAsync(&ctl.p->add_cnt, &nover, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(&ctl.p->overflow_cnt, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(&ctl.p->copy_cnt, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        MutArgs mm = mut_args(m.dels, m.nd, m.adds, m.na);  // pools/rows may have moved
+        k_add<<<grid_small, 256, 0, stream>>>(mm, add_list.p, &ctl.p->add_cnt);
+        k_copy_jobs<<<grid_small, 256, 0, stream>>>(mm);
+        HIPCHK(hipGetLastError());
+        read_ctl();
+        if (hctl.p->overflow_cnt) throw HipError(CIG_ENOMEM, "edge pool exhausted after compaction");
+        return true;
+    }
+
+    // Compacts one tier's pool (dropping relocation garbage) into a new
+    // allocation with room for `extra` more slots. HBM is bounded by the
+    // cache budget: rows that do not fit are demoted to the host tier.
+    void compact_pool(int tier, uint64_t extra) {
+        sync();
+        std::vector<Row> hrows(n);
+        HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+        auto in_tier = [&](const Row &r) { return ((r.off & kHostTier) != 0) == (tier == 1); };
+        uint64_t live = 0;
+        for (const Row &r : hrows)
+            if (in_tier(r)) live += r.cap;
+        if (tier == 0 && budget_slots && live + extra > budget_slots) {
+            demote_for_space(hrows, live + extra - budget_slots);
+            live = 0;
+            for (const Row &r : hrows)
+                if (in_tier(r)) live += r.cap;
+        }
+        uint64_t cap = live + extra + live / 4 + (uint64_t(1) << 20);
+        if (tier == 0 && budget_slots) cap = std::max<uint64_t>(budget_slots, live + extra);
+        std::vector<Row> nrows = hrows;
+        uint64_t tail = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            if (!in_tier(hrows[v])) continue;
+            nrows[v].off = tail | (tier ? kHostTier : 0ull);
+            tail += hrows[v].cap;
+        }
+        if (tier == 0) {
+            DBuf<uint32_t> np;
+            np.alloc(cap);
+            DBuf<Row> drows;
+            drows.alloc(n);
+            HIPCHK(hipMemcpy(drows.p, nrows.data(), n * sizeof(Row), hipMemcpyHostToDevice));
+            k_relocate_rows<<<grid_small, 256, 0, stream>>>(rows.p, drows.p, n, 0, hbm_pool.p, np.p);
+            HIPCHK(hipGetLastError());
+            sync();
+            std::swap(hbm_pool.p, np.p);
+            std::swap(hbm_pool.count, np.count);
+            std::swap(rows.p, drows.p);
+            hbm_slots = cap;
+        } else {
+            HBuf<uint32_t> np;
+            np.alloc(cap);
+            for (uint32_t v = 0; v < n; ++v) {
+                if (!in_tier(hrows[v]) || !hrows[v].deg) continue;
+                std::memcpy(np.p + (nrows[v].off & ~kHostTier), host_pool.p + (hrows[v].off & ~kHostTier),
+                            size_t(hrows[v].deg) * sizeof(uint32_t));
+            }
+            HIPCHK(hipMemcpy(rows.p, nrows.data(), n * sizeof(Row), hipMemcpyHostToDevice));
+            std::swap(host_pool.p, np.p);
+            std::swap(host_pool.count, np.count);
+            host_pool_dev = host_pool.device_ptr();
+            host_slots = cap;
+        }
+        uint64_t vals[2] = {tail, cap};
+        uint64_t zero = 0;
+        HIPCHK(hipMemcpy(&ctl.p->pool_tail[tier], &vals[0], sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&ctl.p->pool_cap[tier], &vals[1], sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&ctl.p->garbage[tier], &zero, sizeof(uint64_t), hipMemcpyHostToDevice));
+        ensure_tiles();
+        ++compactions;
+    }
+
+    // Moves the coldest HBM rows (lowest activity score, then shortest) to the
+    // host tier until `slots` are freed; updates hrows and the device rows.
+    void demote_for_space(std::vector<Row> &hrows, uint64_t slots) {
+        std::vector<uint32_t> sc(n);
+        HIPCHK(hipMemcpy(sc.data(), score.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> cand;
+        for (uint32_t v = 0; v < n; ++v)
+            if (!(hrows[v].off & kHostTier) && hrows[v].cap) cand.push_back(v);
+        std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
+            return sc[a] != sc[b] ? sc[a] < sc[b] : hrows[a].deg < hrows[b].deg;
+        });
+        std::vector<uint32_t> move;
+        uint64_t freed = 0, host_need = 0;
+        for (uint32_t v : cand) {
+            if (freed >= slots) break;
+            move.push_back(v);
+            freed += hrows[v].cap;
+            host_need += hrows[v].cap;
+        }
+        move_rows(hrows, move, 1, host_need);
+    }
+
+    // Re-homes `vs` into `tier`, growing that tier's pool if needed.
+    void move_rows(std::vector<Row> &hrows, const std::vector<uint32_t> &vs, int tier, uint64_t need) {
+        if (vs.empty()) return;
+        read_ctl();
+        uint64_t tail = hctl.p->pool_tail[tier];
+        uint64_t cap = hctl.p->pool_cap[tier];
+        if (tail + need > cap) {
+            compact_pool(tier, need);
+            HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+            read_ctl();
+            tail = hctl.p->pool_tail[tier];
+            cap = hctl.p->pool_cap[tier];
+            if (tail + need > cap) throw HipError(CIG_ENOMEM, "tier pool cannot hold moved rows");
+        }
+        std::vector<CopyJob> hj;
+        for (uint32_t v : vs) {
+            Row &r = hrows[v];
+            CopyJob j{};
+            j.from = r.off;
+            j.to = tail | (tier ? kHostTier : 0ull);
+            j.len = r.deg;
+            hj.push_back(j);
+            r.off = j.to;
+            tail += r.cap;
+        }
+        DBuf<CopyJob> dj;
+        dj.alloc(hj.size());
+        DBuf<uint32_t> dcount;
+        dcount.alloc(1);
+        const uint32_t nj = uint32_t(hj.size());
+        HIPCHK(hipMemcpy(dj.p, hj.data(), hj.size() * sizeof(CopyJob), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dcount.p, &nj, sizeof(uint32_t), hipMemcpyHostToDevice));
+        k_move_rows<<<grid_small, 256, 0, stream>>>(dj.p, dcount.p, pools());
+        HIPCHK(hipGetLastError());
+        sync();
+        HIPCHK(hipMemcpy(rows.p, hrows.data(), n * sizeof(Row), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&ctl.p->pool_tail[tier], &tail, sizeof(uint64_t), hipMemcpyHostToDevice));
+        uint64_t g = 0;
+        for (uint32_t v : vs) g += hrows[v].cap;
+        // the vacated blocks of the other tier become garbage
+        uint64_t og = 0;
+        HIPCHK(hipMemcpy(&og, &ctl.p->garbage[1 - tier], sizeof(uint64_t), hipMemcpyDeviceToHost));
+        og += g;
+        HIPCHK(hipMemcpy(&ctl.p->garbage[1 - tier], &og, sizeof(uint64_t), hipMemcpyHostToDevice));
+    }
+
+    uint64_t compactions = 0;
+
+    // ------------------------------------------------------------ traversal batch
+    InvalArgs inval_args(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na) {
+        InvalArgs x{};
+        x.ctl = ctl.p;
+        x.rows = rows.p;
+        x.state = state.p;
+        x.stamp = stamp.p;
+        x.aff = aff.p;
+        x.alist = alist.p;
+        x.q0 = q0.p;
+        x.q1 = q1.p;
+        x.t0 = t0.p;
+        x.t1 = t1.p;
+        x.cfg = ecfg(kStageInvalidate);
+        x.dels = d;
+        x.nd = nd;
+        x.adds = a;
+        x.na = na;
+        x.n = n;
+        x.kind = cfg.algo == CIG_ALGO_CC ? 1u : 0u;
+        x.max_rounds = round_bound();  // each round consumes at least one new item
+        x.coop = 1;
+        x.seed = cfg.weight_seed;
+        return x;
+    }
+
+    template <int W, bool S>
+    void repair_scan(const InvalArgs &ia) {
+        RepairOp<W, S> op{rows.p, state.p, aff.p, cfg.weight_seed};
+        Sink none{nullptr, &ctl.p->scount};
+        TileSink ts{t0.p, &ctl.p->scount};
+        k_expand_once<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageRepair), nullptr, nullptr, n,
+                                                            none, ts, ctl.p, &ctl.p->acount);
+        k_expand_tiles<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageRepair), t0.p, &ctl.p->scount,
+                                                             none, ctl.p);
+        HIPCHK(hipGetLastError());
+        (void)ia;
+    }
+
+    template <int W, bool S>
+    void insert_seed(const InvalArgs &ia, bool skip_affected) {
+        k_insert_seed<W, S><<<grid_small, 256, 0, stream>>>(ia, skip_affected ? 1u : 0u);
+        HIPCHK(hipGetLastError());
+    }
+
+    void invalidate_and_reset(const InvalArgs &ia) {
+        k_inval_seed<<<grid_small, 256, 0, stream>>>(ia);
+        HIPCHK(hipGetLastError());
+        auto kernel = k_inval_loop<4>;
+        static thread_local uint32_t grid = 0;
+        if (!grid) grid = coop_grid_for(kernel);
+        InvalArgs a = ia;
+        launch_coop(kernel, a, grid);
+        k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W, bool S>
+    void traversal_batch(const cig_edge *d_dels, uint32_t nd, const cig_edge *d_adds, uint32_t na,
+                         bool staged, cig_batch_stats *st) {
+        begin_call(st);
+        if (staged && nd && na) {
+            // stage 1: deletes to their fixed point, then stage 2: adds
+            delete_stage<W, S>(d_dels, nd);
+            insert_stage<W, S>(d_adds, na);
+            HIPCHK(hipEventRecord(ev[4], stream));
+            end_call(st, 0);
+            return;
+        }
+        MutArgs m = mut_args(d_dels, nd, d_adds, na);
+        mutate_group(m);
+        mutate_apply(m);
+        HIPCHK(hipEventRecord(ev[1], stream));
+        InvalArgs ia = inval_args(d_dels, nd, d_adds, na);
+        if (nd) invalidate_and_reset(ia);
+        HIPCHK(hipEventRecord(ev[2], stream));
+        if (nd) repair_scan<W, S>(ia);
+        HIPCHK(hipEventRecord(ev[3], stream));
+        if (na) insert_seed<W, S>(ia, nd != 0);
+        HIPCHK(hipEventRecord(ev[5], stream));
+        relax();
+        HIPCHK(hipEventRecord(ev[4], stream));
+        if (nd) k_clear_affected<<<grid_small, 256, 0, stream>>>(ia);
+        HIPCHK(hipGetLastError());
+        if (na && recover_overflow(m)) {
+            insert_seed<W, S>(ia, false);
+            relax();
+        }
+        mutate_finish(m);
+        end_call(st, 4);
+    }
+
+    template <int W, bool S>
+    void delete_stage(const cig_edge *d_dels, uint32_t nd) {
+        MutArgs m = mut_args(d_dels, nd, nullptr, 0);
+        mutate_group(m);
+        mutate_apply(m);
+        InvalArgs ia = inval_args(d_dels, nd, nullptr, 0);
+        invalidate_and_reset(ia);
+        repair_scan<W, S>(ia);
+        relax();
+        k_clear_affected<<<grid_small, 256, 0, stream>>>(ia);
+        mutate_finish(m);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W, bool S>
+    void insert_stage(const cig_edge *d_adds, uint32_t na) {
+        // counters of the previous stage stay in the control block; only the
+        // per-mutation lists are cleared
+        k_reset_mut<<<1, 64, 0, stream>>>(ctl.p);
+        MutArgs m = mut_args(nullptr, 0, d_adds, na);
+        mutate_group(m);
+        mutate_apply(m);
+        InvalArgs ia = inval_args(nullptr, 0, d_adds, na);
+        insert_seed<W, S>(ia, false);
+        relax();
+        if (recover_overflow(m)) {
+            insert_seed<W, S>(ia, false);
+            relax();
+        }
+        mutate_finish(m);
+    }
+
+    void traversal_call(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, bool staged,
+                        cig_batch_stats *st) {
+        switch (weight_kind()) {
+        case 0: traversal_batch<0, false>(d, nd, a, na, staged, st); break;
+        case 1: traversal_batch<1, false>(d, nd, a, na, staged, st); break;
+        case 2: traversal_batch<2, false>(d, nd, a, na, staged, st); break;
+        default: traversal_batch<3, true>(d, nd, a, na, staged, st); break;
+        }
+    }
+
+    // ------------------------------------------------------------ PR batch
+    void pr_amend(float sign) {
+        PrScatterOp sc{rows.p, res.p, stamp.p, delta.p, &ctl.p->pr_invalid, cfg.pr_epsilon};
+        PrAmendOp op{rows.p, rank.p, sign, sc};
+        // appends to the pending frontier (queue of the next round)
+        read_ctl();
+        const uint32_t r = hctl.p->round;
+        Sink sink{(r & 1) ? q1.p : q0.p, &ctl.p->qcount[r % 3]};
+        TileSink ts{t0.p, &ctl.p->scount};
+        uint32_t zero = 0;
+        HIPCHK(hipMemcpyAsync(&ctl.p->scount, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        k_expand_once<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), touched.p, &ctl.p->touched,
+                                                            0, sink, ts, ctl.p, nullptr);
+        k_expand_tiles<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), t0.p, &ctl.p->scount, sink,
+                                                             ctl.p);
+        HIPCHK(hipGetLastError());
+    }
+
+    // Engine::UpdatePageRank (framework.cuh:1660-1704):
+    // r' = r + alpha (P_new^T - P_old^T) x for the touched sources, then converge.
+    void pr_batch(const cig_edge *d_dels, uint32_t nd, const cig_edge *d_adds, uint32_t na,
+                  cig_batch_stats *st) {
+        begin_call(st);
+        MutArgs m = mut_args(d_dels, nd, d_adds, na);
+        mutate_group(m);
+        pr_amend(-1.0f);  // old rows, old degrees
+        mutate_apply(m);
+        recover_overflow(m);
+        pr_amend(+1.0f);  // new rows, new degrees
+        HIPCHK(hipEventRecord(ev[1], stream));
+        HIPCHK(hipEventRecord(ev[2], stream));
+        HIPCHK(hipEventRecord(ev[3], stream));
+        HIPCHK(hipEventRecord(ev[5], stream));
+        pr_converge();
+        HIPCHK(hipEventRecord(ev[4], stream));
+        pr_residual_norm();
+        mutate_finish(m);
+        end_call(st, 4);
+    }
+
+    // ------------------------------------------------------------ calls
+    void check_ready() {
+        if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        if (!initialized) throw HipError(CIG_ESTATE, "cig_init must come first");
+    }
+
+    void converge(cig_batch_stats *st) {
+        check_ready();
+        begin_call(st);
+        for (int k : {1, 2, 3, 5}) HIPCHK(hipEventRecord(ev[k], stream));
+        if (is_pr()) pr_converge(); else relax();
+        HIPCHK(hipEventRecord(ev[4], stream));
+        if (is_pr()) pr_residual_norm();
+        end_call(st, 4);
+    }
+
+    void upload_and_apply(const cig_edge *dels, uint64_t nd, const cig_edge *adds, uint64_t na,
+                          uint32_t flags, cig_batch_stats *st) {
+        check_ready();
+        if (nd > 0x7FFFFFFFull || na > 0x7FFFFFFFull) throw HipError(CIG_ELIMIT, "batch too large");
+        if ((nd && !dels) || (na && !adds)) throw HipError(CIG_EINVAL, "null record pointer");
+        ensure_records(nd, na);
+        if (nd) HIPCHK(hipMemcpyAsync(rec_dels.p, dels, nd * sizeof(cig_edge), hipMemcpyHostToDevice, stream));
+        if (na) HIPCHK(hipMemcpyAsync(rec_adds.p, adds, na * sizeof(cig_edge), hipMemcpyHostToDevice, stream));
+        apply_device(rec_dels.p, uint32_t(nd), rec_adds.p, uint32_t(na), flags, st);
+    }
+
+    void apply_device(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, uint32_t flags,
+                      cig_batch_stats *st) {
+        if (grouped.count < uint64_t(nd) + na) grouped.alloc(uint64_t(nd) + na);
+        if (is_pr()) pr_batch(d, nd, a, na, st);
+        else traversal_call(d, nd, a, na, (flags & CIG_BATCH_STAGED) != 0, st);
+        if (st) st->pool_compactions = compactions;
+    }
+
+    void load_updates(const cig_edge *adds, uint64_t na, const cig_edge *dels, uint64_t nd,
+                      const uint32_t *sizes, uint32_t nb) {
+        if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        uint64_t ta = 0, td = 0, max_batch = 0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            ta += sizes[2 * b];
+            td += sizes[2 * b + 1];
+            max_batch = std::max<uint64_t>(max_batch, uint64_t(sizes[2 * b]) + sizes[2 * b + 1]);
+        }
+        if (ta > na || td > nd) throw HipError(CIG_EINVAL, "batch sizes exceed the records");
+        s_adds.alloc(ta);
+        s_dels.alloc(td);
+        if (ta) HIPCHK(hipMemcpy(s_adds.p, adds, ta * sizeof(cig_edge), hipMemcpyHostToDevice));
+        if (td) HIPCHK(hipMemcpy(s_dels.p, dels, td * sizeof(cig_edge), hipMemcpyHostToDevice));
+        batch_sizes.assign(sizes, sizes + 2 * nb);
+        next_batch = 0;
+        add_off = del_off = 0;
+        if (grouped.count < max_batch) grouped.alloc(max_batch);
+    }
+
+    void apply_next(uint32_t flags, cig_batch_stats *st) {
+        check_ready();
+        if (next_batch * 2 >= batch_sizes.size()) throw HipError(CIG_ELIMIT, "no update batch left");
+        const uint32_t na = batch_sizes[2 * next_batch], nd = batch_sizes[2 * next_batch + 1];
+        apply_device(s_dels.p + del_off, nd, s_adds.p + add_off, na, flags, st);
+        add_off += na;
+        del_off += nd;
+        ++next_batch;
+    }
+
+    // ------------------------------------------------------------ cache
+    // Sort-based admission (compute_hot_vertices + confirm_candidate_batch,
+    // framework.cuh:605-729): rows by score descending (stable: ties by id),
+    // admit the longest prefix whose capacity fits 90% of the budget, evict
+    // the rest to the host tier, then load admitted cold rows.
+    void cache_refresh(cig_batch_stats *st) {
+        check_ready();
+        begin_call(st);
+        k_age_hotness<<<grid_small, 256, 0, stream>>>(hot.p, hist.p, score.p, n);
+        HIPCHK(hipGetLastError());
+        if (budget_slots) {
+            sync();
+            std::vector<uint32_t> sc(n);
+            std::vector<Row> hrows(n);
+            HIPCHK(hipMemcpy(sc.data(), score.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+            std::vector<uint32_t> order(n);
+            std::iota(order.begin(), order.end(), 0u);
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sc[a] > sc[b]; });
+            const uint64_t limit = budget_slots * 9 / 10;
+            std::vector<uint8_t> admit(n, 0);
+            uint64_t used = 0;
+            for (uint32_t v : order) {
+                if (!hrows[v].cap) continue;
+                if (used + hrows[v].cap > limit) break;
+                admit[v] = 1;
+                used += hrows[v].cap;
+            }
+            std::vector<uint32_t> evict, load;
+            uint64_t evict_need = 0, load_need = 0;
+            for (uint32_t v = 0; v < n; ++v) {
+                const bool cached = !(hrows[v].off & kHostTier);
+                if (!hrows[v].cap) continue;
+                if (cached && !admit[v]) { evict.push_back(v); evict_need += hrows[v].cap; }
+                if (!cached && admit[v]) { load.push_back(v); load_need += hrows[v].cap; }
+            }
+            move_rows(hrows, evict, 1, evict_need);
+            if (!load.empty()) {
+                read_ctl();
+                if (hctl.p->pool_tail[0] + load_need > hctl.p->pool_cap[0]) {
+                    compact_pool(0, load_need);
+                    HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+                }
+                move_rows(hrows, load, 0, load_need);
+            }
+        }
+        end_call(st, 0);
+    }
+
+    // ------------------------------------------------------------ results
+    void gather(uint32_t field, void *out, size_t bytes) {
+        if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        sync();
+        const size_t need = field == CIG_FIELD_TIER ? n : size_t(n) * 4;
+        if (!out || bytes < need) throw HipError(CIG_EINVAL, "output buffer too small");
+        if (field == CIG_FIELD_DEGREE || field == CIG_FIELD_TIER) {
+            std::vector<Row> hrows(n);
+            HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+            for (uint32_t v = 0; v < n; ++v) {
+                if (field == CIG_FIELD_DEGREE) static_cast<uint32_t *>(out)[v] = hrows[v].deg;
+                else static_cast<uint8_t *>(out)[v] = (hrows[v].off & kHostTier) ? 0 : 1;
+            }
+            return;
+        }
+        if (field == CIG_FIELD_HOTNESS) {
+            HIPCHK(hipMemcpy(out, score.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+            return;
+        }
+        if (!initialized) throw HipError(CIG_ESTATE, "cig_init must come first");
+        if (is_pr()) {
+            if (field == CIG_FIELD_VALUE) HIPCHK(hipMemcpy(out, rank.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+            else if (field == CIG_FIELD_RESIDUAL) HIPCHK(hipMemcpy(out, res.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+            else throw HipError(CIG_EINVAL, "field not available for PR");
+            return;
+        }
+        if (field != CIG_FIELD_VALUE && field != CIG_FIELD_PARENT)
+            throw HipError(CIG_EINVAL, "field not available for traversal");
+        std::vector<uint64_t> hs(n);
+        HIPCHK(hipMemcpy(hs.data(), state.p, size_t(n) * 8, hipMemcpyDeviceToHost));
+        uint32_t *o = static_cast<uint32_t *>(out);
+        for (uint32_t v = 0; v < n; ++v) o[v] = field == CIG_FIELD_VALUE ? uint32_t(hs[v] >> 32) : uint32_t(hs[v]);
+    }
+
+    void gather_topology(uint64_t *offs, uint32_t *dst, uint64_t capacity) {
+        if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        sync();
+        std::vector<Row> hrows(n);
+        HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+        read_ctl();
+        const uint64_t used0 = hctl.p->pool_tail[0];
+        std::vector<uint32_t> hp(std::min<uint64_t>(used0, hbm_slots));
+        if (!hp.empty()) HIPCHK(hipMemcpy(hp.data(), hbm_pool.p, hp.size() * 4, hipMemcpyDeviceToHost));
+        uint64_t o = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            offs[v] = o;
+            const Row &r = hrows[v];
+            if (o + r.deg > capacity) throw HipError(CIG_EINVAL, "dst capacity too small");
+            const uint32_t *src = (r.off & kHostTier) ? host_pool.p + (r.off & ~kHostTier) : hp.data() + r.off;
+            if (r.deg) std::memcpy(dst + o, src, size_t(r.deg) * 4);
+            o += r.deg;
+        }
+        offs[n] = o;
+    }
+
+    void info(cig_graph_info *gi) {
+        std::memset(gi, 0, sizeof(*gi));
+        gi->n = n;
+        gi->algo = cfg.algo;
+        gi->block_size = kBlock;
+        gi->edges_per_lane = epl;
+        gi->split_degree = split;
+        gi->cu_count = uint32_t(cus);
+        gi->grid_blocks = grid_coop;
+        if (!loaded) return;
+        sync();
+        std::vector<Row> hrows(n);
+        HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+        read_ctl();
+        for (const Row &r : hrows) {
+            gi->edges += r.deg;
+            if (!(r.off & kHostTier) && r.cap) {
+                gi->cached_rows++;
+                gi->cached_edges += r.deg;
+            }
+        }
+        gi->hbm_pool_slots = hbm_slots;
+        gi->hbm_pool_used = hctl.p->pool_tail[0];
+        gi->host_pool_slots = host_slots;
+        gi->host_pool_used = hctl.p->pool_tail[1];
+        gi->host_pinned_bytes = host_slots * 4 + sizeof(Ctl);
+        gi->device_bytes = hbm_slots * 4 + rows.bytes() + state.bytes() + rank.bytes() * 4 + q0.bytes() * 2 +
+                           stamp.bytes() + aff.bytes() + alist.bytes() + hot.bytes() * 3 + t0.bytes() * 2 +
+                           cnt64.bytes() * 3 + touched.bytes() * 5 + jobs.bytes() + grouped.bytes() +
+                           rec_dels.bytes() * 2 + s_adds.bytes() + s_dels.bytes();
+    }
+
+    ~cig_engine() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            for (auto &e : ev)
+                if (e) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+namespace {
+
+template <class F>
+cig_status guarded(cig_engine *e, F f) {
+    if (!e) return CIG_EINVAL;
+    try {
+        f();
+        e->err.clear();
+        return CIG_OK;
+    } catch (const HipError &x) {
+        e->err = x.what();
+        return x.code;
+    } catch (const std::bad_alloc &) {
+        e->err = "host allocation failed";
+        return CIG_ENOMEM;
+    } catch (const std::exception &x) {
+        e->err = x.what();
+        return CIG_EINTERNAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+cig_status cig_config_default(cig_config *cfg) {
+    if (!cfg) return CIG_EINVAL;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->abi_version = CIG_ABI_VERSION;
+    cfg->algo = CIG_ALGO_SSSP;
+    cfg->weight_mode = CIG_WEIGHT_REF128;
+    cfg->n_stream = 3;
+    cfg->segments = 1;
+    cfg->pr_max_rounds = 100;
+    cfg->pr_epsilon = 1e-6f;
+    return CIG_OK;
+}
+
+cig_status cig_create(const cig_config *cfg, cig_engine **out) {
+    g_create_error.clear();
+    if (!cfg || !out) { g_create_error = "null argument"; return CIG_EINVAL; }
+    *out = nullptr;
+    if (cfg->abi_version != CIG_ABI_VERSION) { g_create_error = "ABI version mismatch"; return CIG_EINVAL; }
+    if (cfg->algo > CIG_ALGO_PR || cfg->weight_mode > CIG_WEIGHT_HASH64) {
+        g_create_error = "unknown algo or weight mode";
+        return CIG_EINVAL;
+    }
+    if (cfg->algo == CIG_ALGO_PR &&
+        (!(cfg->pr_epsilon > 0.0f) || !std::isfinite(cfg->pr_epsilon) || cfg->pr_max_rounds < 1 ||
+         cfg->pr_max_rounds > 1000000)) {
+        g_create_error = "PR requires finite epsilon > 0 and 1 <= pr_max_rounds <= 1000000";
+        return CIG_EINVAL;
+    }
+    if (cfg->segments > 512) { g_create_error = "segments must be <= 512"; return CIG_EINVAL; }
+    std::unique_ptr<cig_engine> e(new (std::nothrow) cig_engine());
+    if (!e) return CIG_ENOMEM;
+    e->cfg = *cfg;
+    try {
+        e->setup_device();
+    } catch (const HipError &x) {
+        g_create_error = x.what();
+        return x.code;
+    }
+    *out = e.release();
+    return CIG_OK;
+}
+
+void cig_destroy(cig_engine *e) { delete e; }
+
+const char *cig_last_error(const cig_engine *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+cig_status cig_load_csr(cig_engine *e, uint32_t n, const uint64_t *offsets, const uint32_t *dst) {
+    return guarded(e, [&] {
+        if (!offsets || (!dst && n && offsets[n])) throw HipError(CIG_EINVAL, "null CSR pointer");
+        e->load(n, offsets, dst);
+    });
+}
+
+cig_status cig_graph_info_get(cig_engine *e, cig_graph_info *info) {
+    return guarded(e, [&] {
+        if (!info) throw HipError(CIG_EINVAL, "null info");
+        e->info(info);
+    });
+}
+
+cig_status cig_gather_topology(cig_engine *e, uint64_t *offsets, uint32_t *dst, uint64_t cap) {
+    return guarded(e, [&] {
+        if (!offsets || (!dst && cap)) throw HipError(CIG_EINVAL, "null output");
+        e->gather_topology(offsets, dst, cap);
+    });
+}
+
+cig_status cig_init(cig_engine *e, const uint32_t *sources, uint32_t nsrc) {
+    return guarded(e, [&] { e->init(sources, nsrc); });
+}
+
+cig_status cig_converge(cig_engine *e, cig_batch_stats *st) {
+    return guarded(e, [&] { e->converge(st); });
+}
+
+cig_status cig_delete_edges(cig_engine *e, const cig_edge *dels, uint64_t nd, cig_batch_stats *st) {
+    return guarded(e, [&] { e->upload_and_apply(dels, nd, nullptr, 0, 0, st); });
+}
+
+cig_status cig_insert_edges(cig_engine *e, const cig_edge *adds, uint64_t na, cig_batch_stats *st) {
+    return guarded(e, [&] { e->upload_and_apply(nullptr, 0, adds, na, 0, st); });
+}
+
+cig_status cig_apply_batch(cig_engine *e, const cig_edge *dels, uint64_t nd, const cig_edge *adds,
+                           uint64_t na, uint32_t flags, cig_batch_stats *st) {
+    return guarded(e, [&] { e->upload_and_apply(dels, nd, adds, na, flags, st); });
+}
+
+cig_status cig_load_updates(cig_engine *e, const cig_edge *adds, uint64_t na, const cig_edge *dels,
+                            uint64_t nd, const uint32_t *sizes, uint32_t nb) {
+    return guarded(e, [&] {
+        if ((!sizes && nb) || (!adds && na) || (!dels && nd)) throw HipError(CIG_EINVAL, "null argument");
+        e->load_updates(adds, na, dels, nd, sizes, nb);
+    });
+}
+
+cig_status cig_apply_next_batch(cig_engine *e, uint32_t flags, cig_batch_stats *st) {
+    return guarded(e, [&] { e->apply_next(flags, st); });
+}
+
+uint32_t cig_update_batches_left(const cig_engine *e) {
+    if (!e) return 0;
+    const uint32_t total = uint32_t(e->batch_sizes.size() / 2);
+    return total > e->next_batch ? total - e->next_batch : 0;
+}
+
+cig_status cig_cache_refresh(cig_engine *e, cig_batch_stats *st) {
+    return guarded(e, [&] { e->cache_refresh(st); });
+}
+
+cig_status cig_gather(cig_engine *e, uint32_t field, void *out, size_t bytes) {
+    return guarded(e, [&] { e->gather(field, out, bytes); });
+}
+
+cig_status cig_copy_probe(int32_t device, uint64_t bytes, uint32_t iters, double *ms) {
+    if (!ms || bytes < 16 || !iters) return CIG_EINVAL;
+    try {
+        HIPCHK(hipSetDevice(device));
+        const uint64_t n16 = bytes / 16;
+        DBuf<uint4> a, b;
+        a.alloc(n16);
+        b.alloc(n16);
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        const uint32_t grid = uint32_t(prop.multiProcessorCount) * 16;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        k_copy_probe<<<grid, 256>>>(a.p, b.p, n16);  // warm
+        HIPCHK(hipEventRecord(e0, nullptr));
+        for (uint32_t i = 0; i < iters; ++i) k_copy_probe<<<grid, 256>>>(a.p, b.p, n16);
+        HIPCHK(hipEventRecord(e1, nullptr));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = double(t) / iters;
+        return CIG_OK;
+    } catch (const HipError &x) {
+        g_create_error = x.what();
+        return x.code;
+    }
+}
+
+int cig_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+}  // extern "C"

@@ -1,0 +1,285 @@
+/*
+ * cig.h — C-ABI of the MI355X incremental graph engine (libcig.so).
+ *
+ * The reference has no ABI: its engine is a header-only C++ template
+ * (sepgraph::engine::Engine, include/framework/framework.cuh:456-3505)
+ * instantiated per sample with an AppBase policy
+ * (include/framework/variants/api.cuh:22-174). Each entry point below names
+ * the Engine method (file:line) whose behaviour it replaces. The app policy
+ * becomes the `algo` field: the four shipped policies (BFS, SSSP, CC, PR) are
+ * compiled into the library as HIP kernels.
+ *
+ * Conventions
+ *   - Every function returns a cig_status (0 = CIG_OK). No exception or abort
+ *     crosses the ABI; cig_last_error() describes the last failure.
+ *   - The engine owns all device and pinned host memory. Caller buffers are
+ *     borrowed for the duration of the call only (host pointers unless the
+ *     name says _device).
+ *   - One engine is driven by one host thread at a time (the reference's
+ *     engine is not thread-safe either, framework.cuh; see cig_serve_* for
+ *     the concurrent request loop).
+ *   - Vertex ids are dense uint32 in [0, n). Unreachable / unset = 0xFFFFFFFF.
+ */
+#ifndef CIG_H
+#define CIG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CIG_ABI_VERSION 1u
+#define CIG_INF 0xFFFFFFFFu
+#define CIG_MAX_SOURCES 64u
+
+typedef enum cig_status {
+    CIG_OK = 0,
+    CIG_EINVAL = 1,   /* bad argument (reference: std::invalid_argument) */
+    CIG_ENOMEM = 2,   /* device or pinned host allocation failed */
+    CIG_EDEVICE = 3,  /* HIP runtime error (reference: GROUTE_CUDA_CHECK abort) */
+    CIG_ESTATE = 4,   /* call out of order (reference: protocol abort) */
+    CIG_EIO = 5,      /* file read/parse error */
+    CIG_ELIMIT = 6,   /* a documented size limit was exceeded */
+    CIG_EINTERNAL = 7 /* internal invariant violated (reference: std::abort) */
+} cig_status;
+
+/* App policy (reference: the AppBase subclass a sample instantiates). */
+typedef enum cig_algo {
+    CIG_ALGO_BFS = 0,  /* samples/hybrid_bfs/hybrid_bfs.cu:36-192, weight 1 */
+    CIG_ALGO_SSSP = 1, /* samples/hybrid_sssp/hybrid_sssp.cu:36-187 */
+    CIG_ALGO_CC = 2,   /* samples/hybrid_cc/hybrid_cc.cu:36-186, directed min label */
+    CIG_ALGO_PR = 3    /* samples/hybrid_pr/hybrid_pr.cu:33-123, signed residual push */
+} cig_algo;
+
+/* SSSP edge weight. Weights are never stored: they are a function of (u,v)
+ * as in the reference (api.cuh:26-28; framework.cuh:1583 overwrites the
+ * update file's weights). */
+typedef enum cig_weight_mode {
+    CIG_WEIGHT_REF128 = 0, /* (u+v)%128+1, the reference's TraversalEdgeWeight */
+    CIG_WEIGHT_HASH64 = 1  /* 1 + splitmix64(seed ^ (u<<32|v)) % 64, uniform in [1,64] */
+} cig_weight_mode;
+
+/* Update record: the reference's WEdgePair<u32,u32,u32> (edgepair.h:17-30).
+ * w is accepted for layout compatibility and ignored. */
+typedef struct cig_edge {
+    uint32_t u, v, w;
+} cig_edge;
+
+enum {
+    CIG_CFG_HOST_ROUNDS = 1u << 0, /* drive fixed-point rounds from the host
+                                      (one launch per round) instead of the
+                                      persistent cooperative kernel */
+    CIG_CFG_NO_HOTNESS = 1u << 1   /* do not count row activity */
+};
+
+/* Engine configuration. Flags of the reference samples map as:
+ * --cache -> cache_bytes, --n_stream -> n_stream, --SEGMENT -> segments,
+ * --block_size -> block_size, --error -> pr_epsilon,
+ * --pr_max_rounds -> pr_max_rounds (app_skeleton.h:42-90, hybrid_pr.cu:21-24). */
+typedef struct cig_config {
+    uint32_t abi_version;     /* must be CIG_ABI_VERSION */
+    uint32_t algo;            /* cig_algo */
+    uint32_t weight_mode;     /* cig_weight_mode (SSSP only) */
+    uint32_t flags;           /* CIG_CFG_* */
+    uint64_t weight_seed;     /* seed of CIG_WEIGHT_HASH64 */
+    uint64_t cache_bytes;     /* HBM edge-cache budget; 0 = whole graph resident */
+    int32_t device;           /* HIP device ordinal */
+    uint32_t n_stream;        /* accepted for flag compatibility */
+    uint32_t segments;        /* accepted for flag compatibility (1..512) */
+    uint32_t block_size;      /* threads per workgroup; 0 = tuned default */
+    uint32_t edges_per_lane;  /* edges each lane keeps in flight; 0 = tuned default */
+    uint32_t split_degree;    /* rows at least this long are split into tiles; 0 = default */
+    uint32_t max_rounds;      /* traversal round cap per launch; 0 = unlimited (exact) */
+    uint32_t pr_max_rounds;   /* rounds per solve; the reference's flag allows
+                                 1..100 (hybrid_pr.cu:22), the library 1..10^6 */
+    float pr_epsilon;         /* absolute residual threshold (hybrid_pr.cu:23) */
+    uint32_t reserved[7];
+} cig_config;
+
+/* Per-call statistics. Times are device (HIP event) milliseconds of the
+ * stage on the engine's stream; the names follow the reference's
+ * [ATTR]/[DELETE-ATTR] timers (hybrid_sssp.cu:511-524, framework.cuh:2874). */
+typedef struct cig_batch_stats {
+    double total_ms;          /* whole call, submit to results resident */
+    double mutate_ms;         /* topology mutation (deletes + adds) */
+    double invalidate_ms;     /* deletion seeding + invalidation waves + reset */
+    double repair_ms;         /* repair seeding scan */
+    double seed_ms;           /* insertion seeding */
+    double propagate_ms;      /* fixed-point propagation (relax / PR rounds) */
+    double cache_ms;          /* cache admission / eviction / compaction */
+    uint64_t edges_scanned;   /* E_scan: edges read by relax/invalidate/repair */
+    uint64_t host_edges_scanned; /* of which read zero-copy from the host tier */
+    uint64_t vertices_expanded;  /* V_exp: rows expanded (or PR vertices consumed) */
+    uint64_t affected;        /* vertices invalidated by deletions */
+    uint64_t rounds;          /* fixed-point rounds executed */
+    uint64_t deletes_applied; /* delete records that removed an edge */
+    uint64_t missing_deletes; /* delete records with no matching edge */
+    uint64_t invalid_updates; /* records with an endpoint >= n */
+    uint64_t adds_applied;
+    uint64_t relocated_rows;  /* rows moved to a larger block */
+    uint64_t pool_compactions;
+    uint64_t touched_sources;
+    uint32_t converged;       /* PR: 1 if no active residual remains */
+    uint32_t launches;        /* kernel launches issued by the call */
+    double pr_residual_l1;    /* PR: sum |r| computed in FP64 after the call */
+    /* per expansion pass: [0] invalidation waves, [1] repair scan,
+     * [2] propagation (relax or PR rounds), [3] PR amendment */
+    uint64_t stage_edges[4];
+    uint64_t stage_vertices[4];
+    uint64_t stage_host_edges[4];
+    uint64_t reserved[4];
+} cig_batch_stats;
+
+typedef struct cig_graph_info {
+    uint32_t n;
+    uint32_t algo;
+    uint64_t edges;             /* live directed edges */
+    uint64_t hbm_pool_slots;    /* capacity of the HBM edge cache, in 4-B slots */
+    uint64_t hbm_pool_used;     /* bump tail of the HBM pool */
+    uint64_t host_pool_slots;   /* capacity of the pinned host tier */
+    uint64_t host_pool_used;
+    uint64_t cached_rows;       /* rows resident in HBM */
+    uint64_t cached_edges;      /* live edges in HBM rows */
+    uint64_t device_bytes;      /* total device memory held by the engine */
+    uint64_t host_pinned_bytes; /* pinned host memory held by the engine */
+    uint32_t grid_blocks;       /* persistent grid size */
+    uint32_t block_size;
+    uint32_t edges_per_lane;
+    uint32_t split_degree;
+    uint32_t cu_count;
+    uint32_t reserved[7];
+} cig_graph_info;
+
+/* Result fields for cig_gather. */
+typedef enum cig_field {
+    CIG_FIELD_VALUE = 0,    /* u32 distance / depth / label; f32 rank for PR */
+    CIG_FIELD_PARENT = 1,   /* u32 tight in-neighbour (SSSP/BFS/CC), CIG_INF if none */
+    CIG_FIELD_RESIDUAL = 2, /* f32 PR signed residual */
+    CIG_FIELD_DEGREE = 3,   /* u32 out-degree */
+    CIG_FIELD_TIER = 4,     /* u8: 1 if the row is resident in HBM, 0 if host tier */
+    CIG_FIELD_HOTNESS = 5   /* u32 activity score (sum of the 4-window) */
+} cig_field;
+
+typedef struct cig_engine cig_engine;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Fills *cfg with defaults (SSSP, REF128, whole graph resident, eps 1e-6,
+ * 100 PR rounds). */
+cig_status cig_config_default(cig_config *cfg);
+/* Engine ctor (framework.cuh:545-563). */
+cig_status cig_create(const cig_config *cfg, cig_engine **out);
+void cig_destroy(cig_engine *e);
+/* Message of the last failure on e (or of the last cig_create if e==NULL). */
+const char *cig_last_error(const cig_engine *e);
+
+/* ---- graph -------------------------------------------------------------- */
+/* Engine::LoadGraph (framework.cuh:800-986): copies a host CSR (offsets has
+ * n+1 entries, dst has offsets[n] entries, row order kept) into the
+ * two-tier row store. Rows are placed in HBM while the budget lasts
+ * (longest rows first), the rest in the pinned host tier. */
+cig_status cig_load_csr(cig_engine *e, uint32_t n, const uint64_t *offsets,
+                        const uint32_t *dst);
+cig_status cig_graph_info_get(cig_engine *e, cig_graph_info *info);
+/* Copies the live topology back in CSR form (offsets: n+1 entries; dst
+ * capacity dst_capacity). Row order within a row is unspecified when a row
+ * received deletes; the multiset equals the reference's. Used by --check. */
+cig_status cig_gather_topology(cig_engine *e, uint64_t *offsets, uint32_t *dst,
+                               uint64_t dst_capacity);
+
+/* ---- computation -------------------------------------------------------- */
+/* Engine::InitGraph (framework.cuh:991-1041). SSSP/BFS: exactly one source
+ * (clamped to [0,n-1] by the samples, hybrid_sssp.cu:352). CC and PR ignore
+ * sources. */
+cig_status cig_init(cig_engine *e, const uint32_t *sources, uint32_t nsrc);
+/* Engine::Start / StartPageRank (framework.cuh:1151-1182, 1649-1658): runs
+ * the fixed point from the initial state. */
+cig_status cig_converge(cig_engine *e, cig_batch_stats *stats);
+/* Engine::del_edge (framework.cuh:1564-1586 + update_tree_del :2752-2894):
+ * applies deletes and repairs the result to the post-delete fixed point. */
+cig_status cig_delete_edges(cig_engine *e, const cig_edge *dels, uint64_t n_del,
+                            cig_batch_stats *stats);
+/* Engine::add_edge (framework.cuh:1397-1401 + update_tree_add :2139-2348). */
+cig_status cig_insert_edges(cig_engine *e, const cig_edge *adds, uint64_t n_add,
+                            cig_batch_stats *stats);
+enum {
+    CIG_BATCH_STAGED = 1u << 0 /* reach the post-delete fixed point before
+                                  applying adds (the reference's two stages);
+                                  default fuses them into one repair */
+};
+/* One update batch: all deletes, then all adds (README.md:17-37). For PR this
+ * is Engine::UpdatePageRank (framework.cuh:1660-1704). */
+cig_status cig_apply_batch(cig_engine *e, const cig_edge *dels, uint64_t n_del,
+                           const cig_edge *adds, uint64_t n_add, uint32_t flags,
+                           cig_batch_stats *stats);
+
+/* Resident update stream: Engine::get_update_file (framework.cuh:1365-1391)
+ * loads every batch up front; batch_sizes holds nbatches (adds, dels) pairs
+ * as in the batch-size file (Loader.h:28-61). The records are copied to HBM
+ * so that cig_apply_next_batch starts with its inputs resident. */
+cig_status cig_load_updates(cig_engine *e, const cig_edge *adds, uint64_t n_add,
+                            const cig_edge *dels, uint64_t n_del,
+                            const uint32_t *batch_sizes, uint32_t nbatches);
+/* Applies the next resident batch (advances the reference's local_begin,
+ * framework.cuh:2161-2162). Returns CIG_ELIMIT when no batch is left. */
+cig_status cig_apply_next_batch(cig_engine *e, uint32_t flags, cig_batch_stats *stats);
+uint32_t cig_update_batches_left(const cig_engine *e);
+
+/* ---- edge cache (hot rows in HBM, cold rows zero-copy from pinned host) --
+ * compute_hot_vertices_* + confirm_candidate_batch + evication_cache +
+ * compact_cache + LoadCache (framework.cuh:605-769, 1873-1956): ages the
+ * activity window, admits the hottest rows that fit the budget, writes
+ * evicted rows back to the host tier and compacts the HBM pool. */
+cig_status cig_cache_refresh(cig_engine *e, cig_batch_stats *stats);
+
+/* ---- results ------------------------------------------------------------ */
+/* GatherValue / GatherParent / GatherBuffer (framework.cuh GraphDatum
+ * gathers): copies n elements of the field into out (bytes >= n*elem). */
+cig_status cig_gather(cig_engine *e, uint32_t field, void *out, size_t bytes);
+
+/* ---- host-side utilities (no GPU needed) -------------------------------- */
+/* Deterministic R-MAT generator: 2^scale vertices, edge_factor*2^scale
+ * samples, quadrant probabilities a,b,c (d = 1-a-b-c) as in the GAP
+ * generator the reference vendors (include/framework/generator.h:80-113),
+ * ids not permuted, self-loops and duplicates removed; symmetric != 0 stores
+ * each undirected edge both ways. Output arrays are malloc'd by the library
+ * (free with cig_host_free). */
+cig_status cig_rmat_csr(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                        uint64_t seed, uint32_t symmetric, uint32_t threads,
+                        uint32_t *n_out, uint64_t **offsets_out, uint32_t **dst_out);
+/* Synthetic update stream over a CSR graph: per batch, n_del deletes sampled
+ * uniformly without replacement from the current edges, then n_add R-MAT
+ * inserts not present at that point (symmetric: both directions, counts are
+ * pairs). Output: adds, dels (malloc'd) and 2*nbatches batch sizes
+ * (adds, dels per batch). */
+cig_status cig_rmat_stream(uint32_t n, const uint64_t *offsets, const uint32_t *dst,
+                           uint32_t scale, double a, double b, double c, uint64_t seed,
+                           uint32_t nbatches, uint64_t n_del, uint64_t n_add,
+                           uint32_t symmetric,
+                           cig_edge **adds_out, uint64_t *n_add_out,
+                           cig_edge **dels_out, uint64_t *n_del_out,
+                           uint32_t **batch_sizes_out);
+/* Text readers: market_big graph (parser.cpp:292-402: "src dst [w]" per line,
+ * n = max id + 1) and the update + batch-size files (Loader.h:28-99). */
+cig_status cig_read_market(const char *path, uint32_t *n_out, uint64_t **offsets_out,
+                           uint32_t **dst_out);
+cig_status cig_read_updates(const char *update_path, const char *batch_path,
+                            cig_edge **adds_out, uint64_t *n_add_out,
+                            cig_edge **dels_out, uint64_t *n_del_out,
+                            uint32_t **batch_sizes_out, uint32_t *nbatches_out);
+void cig_host_free(void *p);
+/* Edge weight of (u,v) under a mode (the function the kernels use). */
+uint32_t cig_edge_weight(uint32_t weight_mode, uint64_t seed, uint32_t u, uint32_t v);
+/* Number of visible HIP devices (0 when none); never aborts. */
+int cig_device_count(void);
+/* Diagnostics: times a plain device-to-device copy kernel of `bytes` bytes
+ * (16 B per lane) over `iters` launches; *ms_per_launch receives the mean
+ * HIP-event time. The HBM reference point for achievable bandwidth and for
+ * calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE. No reference counterpart. */
+cig_status cig_copy_probe(int32_t device, uint64_t bytes, uint32_t iters, double *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CIG_H */

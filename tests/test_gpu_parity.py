@@ -1,0 +1,284 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the CPU oracle.
+
+BFS depths, SSSP distances and CC labels must be bit-exact after the initial
+fixed point and after every update batch; parents must be valid witnesses;
+the live topology must hold the same per-row multiset as the oracle's
+reference-semantics chunk store. PageRank: normalised L1 (sum|x-x*|/sum|x*|)
+against an FP64 Jacobi solve must stay <= 1e-6 with epsilon = 1e-7, and the
+reference's own PageRankCheck invariant must hold.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import coincgraph_amd as cg
+from oracle import oracle as O
+from tests.helpers import (check_topology, check_traversal, oracle_apply, oracle_graph,
+                           pr_errors)
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+@pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_REF128), ("sssp", cg.WEIGHT_HASH64),
+                                        ("bfs", 0), ("cc", 0)])
+@pytest.mark.parametrize("scale", [8, 14])
+def test_converge_matches_oracle(algo, wmode, scale):
+    _announce(f"converge {algo} w{wmode} s{scale}")
+    g = cg.rmat_graph(scale, 16, seed=scale)
+    og = oracle_graph(g)
+    with cg.Engine(algo, weight_mode=wmode, weight_seed=99) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        st = e.start()
+        assert st["rounds"] > 0 and st["edges_scanned"] > 0
+        check_traversal(e, og, algo, 0, wmode, 99, where="initial")
+
+
+@pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_REF128), ("sssp", cg.WEIGHT_HASH64),
+                                        ("bfs", 0), ("cc", 0)])
+@pytest.mark.parametrize("staged", [False, True])
+def test_incremental_batches(algo, wmode, staged):
+    _announce(f"batches {algo} w{wmode} staged={staged}")
+    scale = 12
+    sym = algo == "cc"
+    g = cg.rmat_graph(scale, 16, seed=5, symmetric=sym)
+    s = cg.rmat_stream(g, scale, 6, 400, 400, seed=6, symmetric=sym)
+    og = oracle_graph(g)
+    with cg.Engine(algo, weight_mode=wmode, weight_seed=7) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            st = e.apply_batch(dels, adds, staged=staged)
+            miss, inval = oracle_apply(og, dels, adds)
+            assert st["missing_deletes"] == miss and st["invalid_updates"] == inval
+            assert st["deletes_applied"] == len(dels) - miss
+            check_traversal(e, og, algo, 0, wmode, 7, where=f"batch {b}")
+        check_topology(e, og, "final")
+
+
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
+def test_delete_then_insert_calls(algo):
+    """del_edge and add_edge as separate calls (the reference's two stages,
+    with the post-delete fixed point checked in between)."""
+    _announce(f"del/add calls {algo}")
+    g = cg.rmat_graph(11, 8, seed=21)
+    s = cg.rmat_stream(g, 11, 4, 300, 300, seed=22)
+    og = oracle_graph(g)
+    with cg.Engine(algo) as e:
+        e.load_graph(g)
+        e.init_graph(3)
+        e.start()
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            e.del_edge(dels)
+            og.apply_deletes(dels["u"], dels["v"])
+            check_traversal(e, og, algo, 3, where=f"delete stage {b}")
+            e.add_edge(adds)
+            og.apply_adds(adds["u"], adds["v"])
+            check_traversal(e, og, algo, 3, where=f"batch {b}")
+
+
+def _multigraph_case():
+    """Hub rows with parallel edges, self-loops, many deletes on one row
+    (the workgroup path), missing and out-of-range records."""
+    rng = np.random.default_rng(1234)
+    n = 3000
+    src = np.concatenate([np.zeros(6000, np.int64), rng.integers(0, n, 20000)])
+    dst = np.concatenate([rng.integers(1, n, 6000), rng.integers(0, n, 20000)])
+    g_src = src.astype(np.uint32)
+    g_dst = dst.astype(np.uint32)
+    order = np.argsort(g_src, kind="stable")
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(np.bincount(g_src, minlength=n), out=off[1:])
+    g = cg.HostGraph(n, off, g_dst[order])
+    batches = []
+    for b in range(4):
+        pick = rng.integers(0, len(src), 2500)
+        du = np.concatenate([src[pick], np.zeros(900, np.int64), rng.integers(0, n, 100),
+                             [n + 5, 1]])
+        dv = np.concatenate([dst[pick], dst[:900], rng.integers(0, n, 100), [1, n + 9]])
+        au = np.concatenate([rng.integers(0, n, 2000), np.zeros(3000, np.int64), [7, n + 1]])
+        av = np.concatenate([rng.integers(0, n, 2000), rng.integers(0, n, 3000), [7, 2]])
+        batches.append((cg.edges_array(du, dv), cg.edges_array(au, av)))
+    return g, batches
+
+
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
+def test_multigraph_semantics(algo):
+    _announce(f"multigraph {algo}")
+    g, batches = _multigraph_case()
+    og = oracle_graph(g)
+    with cg.Engine(algo) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        for b, (dels, adds) in enumerate(batches):
+            st = e.apply_batch(dels, adds)
+            miss, inval = oracle_apply(og, dels, adds)
+            assert (st["missing_deletes"], st["invalid_updates"]) == (miss, inval), b
+            check_traversal(e, og, algo, 0, where=f"multigraph batch {b}")
+            check_topology(e, og, f"multigraph batch {b}")
+
+
+def test_empty_and_noop_batches():
+    _announce("empty batches")
+    g = cg.rmat_graph(10, 8, seed=3)
+    og = oracle_graph(g)
+    with cg.Engine("sssp") as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        empty = cg.edges_array([], [])
+        e.apply_batch(empty, empty)
+        e.apply_batch(cg.edges_array([5], [5]), empty)  # missing delete of a self-loop
+        check_traversal(e, og, "sssp", 0, where="noop")
+
+
+@pytest.mark.parametrize("budget_frac", [0.0, 0.3])
+def test_zero_copy_tier_same_results(budget_frac):
+    """Rows outside the HBM budget are read zero-copy from pinned host memory
+    (and mutated there); results must not depend on placement."""
+    _announce(f"zero-copy budget {budget_frac}")
+    scale = 12
+    g = cg.rmat_graph(scale, 16, seed=8)
+    s = cg.rmat_stream(g, scale, 5, 500, 500, seed=9)
+    og = oracle_graph(g)
+    cap_bytes = max(1, int(budget_frac * g.m * 4 * 2)) if budget_frac else 4096
+    with cg.Engine("sssp", cache_bytes=cap_bytes) as e:
+        e.load_graph(g)
+        info = e.info()
+        assert info["host_pool_used"] > 0
+        e.init_graph(0)
+        st = e.start()
+        assert st["host_edges_scanned"] > 0
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            e.apply_batch(dels, adds)
+            oracle_apply(og, dels, adds)
+            check_traversal(e, og, "sssp", 0, where=f"zc batch {b}")
+            e.cache_refresh()
+            check_traversal(e, og, "sssp", 0, where=f"zc refresh {b}")
+        check_topology(e, og, "zc final")
+
+
+def test_pool_overflow_recovery():
+    """A budget just above the initial footprint forces relocation overflow,
+    compaction and demotion to the host tier; results stay exact."""
+    _announce("overflow recovery")
+    scale = 11
+    g = cg.rmat_graph(scale, 16, seed=10)
+    og = oracle_graph(g)
+    caps = sum(int(cg_cap) for cg_cap in
+               [max(4, 1 << int(np.ceil(np.log2(max(d, 1))))) if d else 0
+                for d in np.diff(g.offsets)])
+    s = cg.rmat_stream(g, scale, 6, 100, 3000, seed=11)
+    with cg.Engine("bfs", cache_bytes=int(caps * 4 * 1.15)) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            e.apply_batch(dels, adds)
+            oracle_apply(og, dels, adds)
+            check_traversal(e, og, "bfs", 0, where=f"overflow batch {b}")
+        check_topology(e, og, "overflow final")
+
+
+@pytest.mark.parametrize("scale", [10, 14])
+def test_pagerank_static_and_batches(scale):
+    _announce(f"pagerank s{scale}")
+    eps = 1e-7
+    g = cg.rmat_graph(scale, 16, seed=12)
+    s = cg.rmat_stream(g, scale, 4, 300, 300, seed=13)
+    og = oracle_graph(g)
+    with cg.Engine("pr", pr_epsilon=eps, pr_max_rounds=2000) as e:
+        e.load_graph(g)
+        e.init_graph()
+        st = e.start()
+        for b in range(-1, s.nbatches):
+            if b >= 0:
+                dels, adds = s.batch(b)
+                st = e.apply_batch(dels, adds)
+                oracle_apply(og, dels, adds)
+            assert st["converged"] == 1, f"batch {b}: not converged in 2000 rounds"
+            ranks, resid = e.values(), e.residuals()
+            ref, rounds = og.pagerank()
+            l1n, maxerr = pr_errors(ranks, ref)
+            assert l1n <= 1e-6, f"batch {b}: normalised L1 {l1n:.3e} (max {maxerr:.3e})"
+            chk = og.pr_check(ranks, resid, eps)
+            assert chk.finite and chk.max_residual <= np.float32(eps)
+            assert chk.invariant_l1 <= 64 * np.finfo(np.float32).eps * max(1.0, chk.reference_mass)
+            assert abs(st["pr_residual_l1"] - float(np.abs(resid.astype(np.float64)).sum())) <= \
+                1e-6 * max(1.0, st["pr_residual_l1"])
+
+
+def test_resident_stream_matches_calls():
+    """get_update_file + per-batch apply (inputs resident in HBM) gives the
+    same results as host-pointer batches."""
+    _announce("resident stream")
+    g = cg.rmat_graph(12, 16, seed=14)
+    s = cg.rmat_stream(g, 12, 5, 500, 500, seed=15)
+    og = oracle_graph(g)
+    with cg.Engine("sssp", weight_mode=cg.WEIGHT_HASH64, weight_seed=5) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        e.load_updates(s)
+        b = 0
+        while e.batches_left():
+            e.apply_next_batch()
+            oracle_apply(og, *s.batch(b))
+            check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 5, where=f"stream {b}")
+            b += 1
+        assert b == s.nbatches
+
+
+def test_host_rounds_mode_matches():
+    """Host-driven rounds (one launch per round) give the same fixed point."""
+    _announce("host rounds")
+    g = cg.rmat_graph(11, 16, seed=16)
+    s = cg.rmat_stream(g, 11, 3, 200, 200, seed=17)
+    og = oracle_graph(g)
+    with cg.Engine("sssp", flags=cg.CFG_HOST_ROUNDS) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        for b in range(s.nbatches):
+            e.apply_batch(*s.batch(b))
+            oracle_apply(og, *s.batch(b))
+            check_traversal(e, og, "sssp", 0, where=f"host rounds {b}")
+
+
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "cc", "pr"])
+def test_golden_readme_example(algo):
+    """README.md:27-35 update example on a hand-checked 5-vertex graph."""
+    _announce(f"golden {algo}")
+    g = cg.read_market(os.path.join(GOLDEN, "readme", "graph.txt"))
+    s = cg.read_updates(os.path.join(GOLDEN, "readme", "updates.txt"),
+                        os.path.join(GOLDEN, "readme", "batches.txt"))
+    exp = np.loadtxt(os.path.join(GOLDEN, "readme", f"expected_{algo}.txt"), ndmin=2)
+    with cg.Engine(algo, pr_epsilon=1e-7, pr_max_rounds=2000) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        e.apply_batch(*s.batch(0))
+        got = e.values()
+        if algo == "pr":
+            np.testing.assert_allclose(got, exp[:, 1], rtol=2e-6, atol=1e-6)
+        else:
+            assert got.tolist() == exp[:, 1].astype(np.int64).tolist()
