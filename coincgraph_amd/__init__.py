@@ -34,7 +34,7 @@ WEIGHT_REF128, WEIGHT_HASH64 = 0, 1
 # cig_field
 FIELD_VALUE, FIELD_PARENT, FIELD_RESIDUAL, FIELD_DEGREE, FIELD_TIER, FIELD_HOTNESS = range(6)
 # flags
-CFG_HOST_ROUNDS, CFG_NO_HOTNESS = 1, 2
+CFG_COOPERATIVE, CFG_NO_HOTNESS = 1, 2
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
@@ -62,7 +62,8 @@ class Config(C.Structure):
         ("block_size", C.c_uint32), ("edges_per_lane", C.c_uint32),
         ("split_degree", C.c_uint32), ("max_rounds", C.c_uint32),
         ("pr_max_rounds", C.c_uint32), ("pr_epsilon", C.c_float),
-        ("reserved", C.c_uint32 * 7),
+        ("tile_len", C.c_uint32), ("round_blocks_per_cu", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
     ]
 
 
@@ -308,7 +309,8 @@ class Engine:
 
     def __init__(self, algo="sssp", weight_mode=WEIGHT_REF128, weight_seed=0, cache_bytes=0,
                  device=0, pr_epsilon=1e-6, pr_max_rounds=100, flags=0, edges_per_lane=0,
-                 split_degree=0, max_rounds=0, n_stream=3, segments=1):
+                 split_degree=0, max_rounds=0, n_stream=3, segments=1, tile_len=0,
+                 round_blocks_per_cu=0):
         L = lib()
         cfg = Config()
         _check(L.cig_config_default(C.byref(cfg)))
@@ -325,6 +327,8 @@ class Engine:
         cfg.max_rounds = max_rounds
         cfg.n_stream = n_stream
         cfg.segments = segments
+        cfg.tile_len = tile_len
+        cfg.round_blocks_per_cu = round_blocks_per_cu
         self.config = cfg
         self._e = C.c_void_p()
         st = L.cig_create(C.byref(cfg), C.byref(self._e))

@@ -55,6 +55,8 @@ struct Ctl {
     uint32_t tcount[3];
     uint32_t round;        // next absolute round to run
     uint32_t converged;
+    uint32_t done;         // workgroups finished in the current round kernel
+    uint32_t ibegin, iend, iround;  // invalidation frontier [ibegin, iend), round
     // invalidation
     uint32_t acount;       // affected list length
     uint32_t icount[3];    // invalidation tile counts

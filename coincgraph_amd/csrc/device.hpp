@@ -135,7 +135,9 @@ __device__ __forceinline__ bool bit_claim(uint32_t *bits, uint32_t v) {
 // ---------------------------------------------------------------- operators
 // source(): decide whether item v (at list position i) expands; fill its row
 // and a 32-bit payload. tile_source(): payload for a tile of v's row.
-// edge(): per-edge work; returns true to append the destination.
+// peek(): the per-edge load the decision needs (issued for all edges a lane
+// holds before any decision, so the loads overlap). edge(): per-edge work on
+// the peeked word; returns true to append the destination.
 
 // Push relax (SyncPushDDB relax body + AccumulateBuffer,
 // sync_push_dd.cuh:148-239, hybrid_sssp.cu:108-128).
@@ -159,9 +161,10 @@ struct RelaxOp {
         pay = uint32_t(state[t.v] >> 32);
         return pay != kInf;
     }
-    __device__ bool edge(uint32_t u, uint32_t val, uint32_t d) const {
+    __device__ uint64_t peek(uint32_t d) const { return state[d]; }
+    __device__ bool edge(uint32_t u, uint32_t val, uint32_t d, uint64_t cur) const {
         const uint64_t c = uint64_t(val) + weight<W>(u, d, seed);
-        if (c >= kInf) return false;
+        if (c >= uint32_t(cur >> 32)) return false;  // also rejects c >= kInf
         if (!relax<kStrict>(state, d, uint32_t(c), u)) return false;
         return claim(stamp, d, tag);
     }
@@ -183,39 +186,10 @@ struct InvalidateOp {
         pay = 0;
         return true;
     }
-    __device__ bool edge(uint32_t x, uint32_t, uint32_t y) const {
-        if (bit_test(aff, y)) return false;
-        if (uint32_t(state[y]) != x) return false;
+    __device__ uint64_t peek(uint32_t y) const { return state[y]; }
+    __device__ bool edge(uint32_t x, uint32_t, uint32_t y, uint64_t cur) const {
+        if (uint32_t(cur) != x) return false;
         return bit_claim(aff, y);
-    }
-};
-
-// Repair seeding: every unaffected, valued vertex offers its value to its
-// affected out-neighbours (the pull of GpuAffectedPullRelax,
-// framework.cuh:137-204, done as a push over the forward rows).
-template <int W, bool kStrict>
-struct RepairOp {
-    const Row *rows;
-    uint64_t *state;
-    const uint32_t *aff;
-    uint64_t seed;
-    __device__ bool source(uint32_t v, uint32_t, Row &row, uint32_t &pay) const {
-        if (bit_test(aff, v)) return false;
-        const uint32_t val = uint32_t(state[v] >> 32);
-        if (val == kInf) return false;
-        row = rows[v];
-        pay = val;
-        return row.deg != 0;
-    }
-    __device__ bool tile_source(const Tile &t, uint32_t &pay) const {
-        pay = uint32_t(state[t.v] >> 32);
-        return pay != kInf;
-    }
-    __device__ bool edge(uint32_t u, uint32_t val, uint32_t y) const {
-        if (!bit_test(aff, y)) return false;
-        const uint64_t c = uint64_t(val) + weight<W>(u, y, seed);
-        if (c < kInf) relax<kStrict>(state, y, uint32_t(c), u);
-        return false;
     }
 };
 
@@ -239,7 +213,8 @@ struct PrScatterOp {
         pay = t.payload;
         return true;
     }
-    __device__ bool edge(uint32_t, uint32_t pay, uint32_t w) const {
+    __device__ uint64_t peek(uint32_t) const { return 0; }
+    __device__ bool edge(uint32_t, uint32_t pay, uint32_t w, uint64_t) const {
         const float c = __uint_as_float(pay);
         const float old = unsafeAtomicAdd(&res[w], c);
         const float now = old + c;
@@ -267,7 +242,10 @@ struct PrAmendOp {
         pay = t.payload;
         return true;
     }
-    __device__ bool edge(uint32_t u, uint32_t pay, uint32_t w) const { return scatter.edge(u, pay, w); }
+    __device__ uint64_t peek(uint32_t) const { return 0; }
+    __device__ bool edge(uint32_t u, uint32_t pay, uint32_t w, uint64_t p) const {
+        return scatter.edge(u, pay, w, p);
+    }
 };
 
 // ---------------------------------------------------------------- expansion
@@ -333,9 +311,15 @@ __device__ void expand_chunk(const Op &op, const ExpandCfg &cfg, const uint32_t 
     ctr.scanned += deg;
     ctr.vexp += deg ? 1 : 0;
     if (off & kHostTier) ctr.host += deg;
-    if (deg >= cfg.split) {
-        const uint32_t nt = (deg + cfg.tile_len - 1) / cfg.tile_len;
-        const uint32_t t0 = atomicAdd(tsink.count, nt);
+    // long rows become tiles; one tile-count atomic per wave
+    const uint32_t nt = deg >= cfg.split ? (deg + cfg.tile_len - 1) / cfg.tile_len : 0u;
+    const uint32_t tincl = wave_incl_scan(nt);
+    const uint32_t ttotal = __shfl(tincl, kWave - 1, kWave);
+    uint32_t tbase = 0;
+    if (ttotal && lane == 0) tbase = atomicAdd(tsink.count, ttotal);
+    tbase = __shfl(tbase, 0, kWave);
+    if (nt) {
+        const uint32_t t0 = tbase + tincl - nt;
         for (uint32_t j = 0; j < nt; ++j) {
             Tile t;
             t.start = off + uint64_t(j) * cfg.tile_len;
@@ -358,26 +342,27 @@ __device__ void expand_chunk(const Op &op, const ExpandCfg &cfg, const uint32_t 
     wave_lds_sync();
     for (uint32_t k0 = 0; k0 < total; k0 += kWave * EPL) {
         uint32_t dst[EPL], own[EPL];
+        uint64_t cur[EPL];
         bool ok[EPL];
+        // owner searches and loads are unconditional (indices clamped to the
+        // last edge) so that all EPL chains are in flight together
 #pragma unroll
         for (int e = 0; e < EPL; ++e) {
             const uint32_t k = k0 + uint32_t(e) * kWave + lane;
             ok[e] = k < total;
-            own[e] = 0;
-            dst[e] = 0;
-            if (ok[e]) {
-                const uint32_t j = owner_of(L.pref, k);
-                own[e] = j;
-                const uint64_t o = (uint64_t(L.offhi[j]) << 32) | L.offlo[j];
-                dst[e] = cfg.pools.row(o)[k - L.pref[j]];
-            }
+            const uint32_t kk = ok[e] ? k : total - 1;
+            own[e] = owner_of(L.pref, kk);
+            const uint64_t o = (uint64_t(L.offhi[own[e]]) << 32) | L.offlo[own[e]];
+            dst[e] = cfg.pools.row(o)[kk - L.pref[own[e]]];
         }
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) {
-            bool push = false;
-            if (ok[e]) push = op.edge(L.v[own[e]], L.pay[own[e]], dst[e]);
-            wave_append(push, dst[e], sink.list, sink.count);
-        }
+        for (int e = 0; e < EPL; ++e) cur[e] = op.peek(dst[e]);
+        bool push[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; ++e)
+            push[e] = ok[e] && op.edge(L.v[own[e]], L.pay[own[e]], dst[e], cur[e]);
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) wave_append(push[e], dst[e], sink.list, sink.count);
     }
     wave_lds_sync();  // LDS is reused by the next chunk of this wave
 }
@@ -390,18 +375,21 @@ __device__ void expand_tile(const Op &op, const ExpandCfg &cfg, const Tile &t, c
     const uint32_t lane = lane_id();
     for (uint32_t k0 = 0; k0 < t.len; k0 += kWave * EPL) {
         uint32_t dst[EPL];
+        uint64_t cur[EPL];
         bool ok[EPL];
 #pragma unroll
         for (int e = 0; e < EPL; ++e) {
             const uint32_t k = k0 + uint32_t(e) * kWave + lane;
             ok[e] = k < t.len;
-            dst[e] = ok[e] ? p[k] : 0;
+            dst[e] = p[ok[e] ? k : t.len - 1];
         }
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) {
-            const bool push = ok[e] && op.edge(t.v, pay, dst[e]);
-            wave_append(push, dst[e], sink.list, sink.count);
-        }
+        for (int e = 0; e < EPL; ++e) cur[e] = op.peek(dst[e]);
+        bool push[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) push[e] = ok[e] && op.edge(t.v, pay, dst[e], cur[e]);
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) wave_append(push[e], dst[e], sink.list, sink.count);
     }
 }
 

@@ -94,8 +94,8 @@ struct HBuf {  // pinned host memory (hipHostMalloc; never hipHostRegister)
     }
 };
 
-constexpr uint32_t kTileLenDefault = 1024;
-constexpr uint32_t kSplitDefault = 256;
+constexpr uint32_t kTileLenDefault = 512;
+constexpr uint32_t kSplitDefault = 64;
 constexpr uint32_t kEplDefault = 4;
 
 }  // namespace
@@ -142,6 +142,7 @@ struct cig_engine {
     uint32_t split = kSplitDefault, tile_len = kTileLenDefault, epl = kEplDefault;
     uint32_t grid_small = 0;   // one-shot kernels
     uint32_t grid_coop = 0;    // persistent cooperative kernels
+    uint32_t grid_round = 0;   // chained round kernels
     uint32_t grid_delbig = 0;
     bool coop = true;
 
@@ -169,10 +170,13 @@ struct cig_engine {
         std::memset(hctl.p, 0, sizeof(Ctl));
         HIPCHK(hipMemcpyAsync(ctl.p, hctl.p, sizeof(Ctl), hipMemcpyHostToDevice, stream));
         if (cfg.split_degree) split = std::max<uint32_t>(64, cfg.split_degree);
+        if (cfg.tile_len) tile_len = std::max<uint32_t>(64, cfg.tile_len);
         if (cfg.edges_per_lane) epl = cfg.edges_per_lane >= 8 ? 8 : cfg.edges_per_lane >= 4 ? 4
                                         : cfg.edges_per_lane >= 2 ? 2 : 1;
-        coop = !(cfg.flags & CIG_CFG_HOST_ROUNDS) && prop.cooperativeLaunch;
+        coop = (cfg.flags & CIG_CFG_COOPERATIVE) != 0;
+        if (coop && !prop.cooperativeLaunch) throw HipError(CIG_EDEVICE, "device lacks cooperative launch");
         grid_small = uint32_t(cus) * 8;
+        grid_round = uint32_t(cus) * (cfg.round_blocks_per_cu ? std::min<uint32_t>(cfg.round_blocks_per_cu, 16) : 1);
         grid_delbig = uint32_t(cus) * 2;
     }
 
@@ -354,18 +358,42 @@ struct cig_engine {
         a.cfg = ecfg(kStagePropagate);
         a.seed = cfg.weight_seed;
         a.max_rounds = max_rounds;
-        a.coop = coop ? 1u : 0u;
         return a;
+    }
+
+    // Enqueues round kernels back to back in growing chunks (8, 16, ... 128)
+    // and reads the control block once per chunk; `more` says whether the
+    // fixed point still has work. Rounds past the bound raise an error.
+    template <class Launch, class More>
+    uint64_t run_chain(Launch launch, More more, uint64_t bound, const char *what) {
+        uint64_t done = 0;
+        uint32_t k = 8;
+        for (;;) {
+            const uint32_t todo = uint32_t(std::min<uint64_t>(k, bound - done));
+            for (uint32_t i = 0; i < todo; ++i) launch();
+            HIPCHK(hipGetLastError());
+            done += todo;
+            read_ctl();
+            if (!more(*hctl.p)) return done;
+            if (done >= bound) {
+                if (cfg.max_rounds || is_pr()) return done;  // a configured cap
+                throw HipError(CIG_EINTERNAL, std::string(what) + " did not converge within the round bound");
+            }
+            k = std::min<uint32_t>(k * 2, 128);
+        }
+    }
+
+    static bool relax_pending(const Ctl &c) {
+        return c.qcount[c.round % 3] || c.tcount[c.round % 3];
     }
 
     template <int W, bool S, int EPL>
     void relax_impl(uint32_t launches_cap) {
-        auto kernel = k_relax_loop<W, S, EPL>;
         if (coop) {
+            auto kernel = k_relax_loop<W, S, EPL>;
             static thread_local uint32_t grid = 0;
             if (!grid) grid = coop_grid_for(kernel);
             grid_coop = grid;
-            // A launch stops at the round cap; relaunch until the frontier is empty.
             // Without a configured cap the bound is 2n+64 rounds: a frontier
             // round finalises every vertex one more hop away (one more for a
             // deferred tile), so a larger count means a broken invariant.
@@ -375,24 +403,15 @@ struct cig_engine {
             if (cfg.max_rounds) {
                 for (uint32_t i = 0; i < launches_cap; ++i) {
                     read_ctl();
-                    const uint32_t r = hctl.p->round;
-                    if (!hctl.p->qcount[r % 3] && !hctl.p->tcount[r % 3]) break;
+                    if (!relax_pending(*hctl.p)) break;
                     launch_coop(kernel, a, grid);
                 }
             }
         } else {
             LoopArgs a = loop_args(1);
-            for (uint64_t done = 0;; done += 8) {
-                if (done > round_bound()) throw HipError(CIG_EINTERNAL, "fixed point did not converge");
-                for (int k = 0; k < 8; ++k) {
-                    kernel<<<grid_small, kBlock, 0, stream>>>(a);
-                    k_advance_round<<<1, 64, 0, stream>>>(ctl.p);
-                }
-                HIPCHK(hipGetLastError());
-                read_ctl();
-                const uint32_t r = hctl.p->round;
-                if (!hctl.p->qcount[r % 3] && !hctl.p->tcount[r % 3]) break;
-            }
+            const uint64_t bound = cfg.max_rounds ? cfg.max_rounds : round_bound();
+            run_chain([&] { k_relax_round<W, S, EPL><<<grid_round, kBlock, 0, stream>>>(a); },
+                      relax_pending, bound, "relax fixed point");
         }
         HIPCHK(hipGetLastError());
     }
@@ -435,14 +454,24 @@ struct cig_engine {
         return a;
     }
 
+    // Runtime::Converge (pr_residual.cuh:171-187): at most pr_max_rounds
+    // consume/scatter rounds; residual left over stays pending.
     void pr_converge() {
-        if (!coop) throw HipError(CIG_EDEVICE, "PageRank rounds need cooperative launch support");
-        auto kernel = k_pr_loop<4>;
-        static thread_local uint32_t grid = 0;
-        if (!grid) grid = coop_grid_for(kernel);
-        grid_coop = grid;
         PrArgs a = pr_args(cfg.pr_max_rounds);
-        launch_coop(kernel, a, grid);
+        if (coop) {
+            auto kernel = k_pr_loop<4>;
+            static thread_local uint32_t grid = 0;
+            if (!grid) grid = coop_grid_for(kernel);
+            grid_coop = grid;
+            launch_coop(kernel, a, grid);
+        } else {
+            run_chain([&] {
+                          k_pr_consume<<<grid_round, kBlock, 0, stream>>>(a);
+                          k_pr_scatter<4><<<grid_round, kBlock, 0, stream>>>(a);
+                      },
+                      [](const Ctl &c) { return c.qcount[c.round % 3] != 0; }, cfg.pr_max_rounds,
+                      "PageRank");
+        }
         HIPCHK(hipGetLastError());
     }
 
@@ -519,7 +548,7 @@ struct cig_engine {
         st->adds_applied = c.adds_applied;
         st->relocated_rows = c.relocated;
         st->touched_sources = c.touched;
-        st->converged = is_pr() ? c.converged : 1u;
+        st->converged = is_pr() ? (c.qcount[c.round % 3] == 0 ? 1u : 0u) : 1u;
         st->pr_residual_l1 = c.residual_l1;
     }
 
@@ -555,7 +584,7 @@ struct cig_engine {
     }
 
     void mutate_apply(const MutArgs &m) {
-        k_del_small<<<grid_small, 256, 0, stream>>>(m);
+        k_del_rows<<<grid_small, kBlock, 0, stream>>>(m);
         k_del_big<<<grid_delbig, kDelBigThreads, 0, stream>>>(m);
         k_add<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
         k_copy_jobs<<<grid_small, 256, 0, stream>>>(m);
@@ -756,22 +785,28 @@ struct cig_engine {
         x.n = n;
         x.kind = cfg.algo == CIG_ALGO_CC ? 1u : 0u;
         x.max_rounds = round_bound();  // each round consumes at least one new item
-        x.coop = 1;
         x.seed = cfg.weight_seed;
         return x;
     }
 
+    template <int W, bool S, int EPL>
+    void repair_scan_epl(const InvalArgs &ia) {
+        InvalArgs a = ia;
+        a.cfg = ecfg(kStageRepair);
+        const uint32_t grid = uint32_t(cus) * 4;  // 4 workgroups of 37 KB LDS per CU
+        k_repair_scan<W, S, EPL><<<grid, kBlock, 0, stream>>>(a);
+        k_repair_tiles<W, S, EPL><<<grid, kBlock, 0, stream>>>(a);
+        HIPCHK(hipGetLastError());
+    }
+
     template <int W, bool S>
     void repair_scan(const InvalArgs &ia) {
-        RepairOp<W, S> op{rows.p, state.p, aff.p, cfg.weight_seed};
-        Sink none{nullptr, &ctl.p->scount};
-        TileSink ts{t0.p, &ctl.p->scount};
-        k_expand_once<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageRepair), nullptr, nullptr, n,
-                                                            none, ts, ctl.p, &ctl.p->acount);
-        k_expand_tiles<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageRepair), t0.p, &ctl.p->scount,
-                                                             none, ctl.p);
-        HIPCHK(hipGetLastError());
-        (void)ia;
+        switch (epl) {
+        case 1: repair_scan_epl<W, S, 1>(ia); break;
+        case 2: repair_scan_epl<W, S, 2>(ia); break;
+        case 8: repair_scan_epl<W, S, 8>(ia); break;
+        default: repair_scan_epl<W, S, 4>(ia); break;
+        }
     }
 
     template <int W, bool S>
@@ -783,11 +818,18 @@ struct cig_engine {
     void invalidate_and_reset(const InvalArgs &ia) {
         k_inval_seed<<<grid_small, 256, 0, stream>>>(ia);
         HIPCHK(hipGetLastError());
-        auto kernel = k_inval_loop<4>;
-        static thread_local uint32_t grid = 0;
-        if (!grid) grid = coop_grid_for(kernel);
         InvalArgs a = ia;
-        launch_coop(kernel, a, grid);
+        if (coop) {
+            auto kernel = k_inval_loop<4>;
+            static thread_local uint32_t grid = 0;
+            if (!grid) grid = coop_grid_for(kernel);
+            launch_coop(kernel, a, grid);
+        } else {
+            k_inval_begin<<<1, 64, 0, stream>>>(ctl.p);
+            run_chain([&] { k_inval_round<4><<<grid_round, kBlock, 0, stream>>>(a); },
+                      [](const Ctl &c) { return c.ibegin != c.iend || c.icount[c.iround % 3] != 0; },
+                      round_bound(), "invalidation");
+        }
         k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
         HIPCHK(hipGetLastError());
     }

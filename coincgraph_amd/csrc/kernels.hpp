@@ -123,11 +123,12 @@ __global__ void k_count(MutArgs a) {
         const uint32_t i = base + threadIdx.x;
         bool first = false;
         uint32_t u = 0;
+        bool bad = false;
         if (i < total) {
             const bool del = i < a.nd;
             const cig_edge e = del ? a.dels[i] : a.adds[i - a.nd];
             if (e.u >= a.n || e.v >= a.n) {
-                atomicAdd((unsigned long long *)&a.ctl->invalid, 1ull);
+                bad = true;
             } else {
                 u = e.u;
                 const uint64_t old = atomicAdd((unsigned long long *)&a.cnt64[u],
@@ -135,6 +136,9 @@ __global__ void k_count(MutArgs a) {
                 first = old == 0;
             }
         }
+        const uint64_t badm = __ballot(bad);
+        if (badm && lane_id() == 0)
+            atomicAdd((unsigned long long *)&a.ctl->invalid, (unsigned long long)__popcll(badm));
         wave_append(first, u, a.touched, &a.ctl->touched);
     }
 }
@@ -145,18 +149,27 @@ __global__ void k_plan(MutArgs a) {
     for (uint32_t base = blockIdx.x * blockDim.x; base < touched; base += gthreads()) {
         const uint32_t t = base + threadIdx.x;
         bool small = false, big = false, add = false;
-        uint32_t u = 0;
+        uint32_t u = 0, len = 0, nd = 0, na = 0;
         if (t < touched) {
             u = a.touched[t];
             const uint64_t c = a.cnt64[u];
-            const uint32_t nd = uint32_t(c), na = uint32_t(c >> 32);
-            a.gofs[u] = atomicAdd((unsigned long long *)&a.ctl->group_cursor,
-                                  (unsigned long long)(nd + na));
-            const uint32_t deg = a.rows[u].deg;
-            small = nd != 0 && deg <= 512 && nd <= 64;
+            nd = uint32_t(c);
+            na = uint32_t(c >> 32);
+            len = nd + na;
+            // one wave scans a row of up to 4096 slots; longer rows (or more than
+            // 64 deletes) get a whole workgroup
+            small = nd != 0 && nd <= 64 && a.rows[u].deg <= 4096;
             big = nd != 0 && !small;
             add = na != 0;
         }
+        // one group-cursor atomic per wave
+        const uint32_t incl = wave_incl_scan(len);
+        const uint32_t total_len = __shfl(incl, kWave - 1, kWave);
+        unsigned long long gbase = 0;
+        if (lane_id() == 0 && total_len)
+            gbase = atomicAdd((unsigned long long *)&a.ctl->group_cursor, (unsigned long long)total_len);
+        gbase = __shfl(gbase, 0, kWave);
+        if (t < touched) a.gofs[u] = gbase + (incl - len);
         wave_append(small, u, a.small_list, &a.ctl->small_cnt);
         wave_append(big, u, a.big_list, &a.ctl->big_cnt);
         wave_append(add, u, a.add_list, &a.ctl->add_cnt);
@@ -176,56 +189,83 @@ __global__ void k_scatter(MutArgs a) {
     }
 }
 
-// Deletes for a short row (deg <= 512, <= 64 deletes), one wave per row: the
-// row sits in registers (8 slots per lane); each delete removes the first
-// live occurrence in row order; survivors are compacted in order
-// (BuildDeletionPlan + CompactSourceInPlace, source_local_chunk_store.h:670-780).
-__global__ void k_del_small(MutArgs a) {
+// Deletes for a row of at most 4096 slots with at most 64 delete records,
+// one wave per row. Lane j holds delete j; the row streams through 64*4 slots at a
+// time and each pending delete claims the first live occurrence in row order
+// (BuildDeletionPlan, source_local_chunk_store.h:670-723); the scan stops once
+// every delete found its edge. The k holes are then refilled from the row's
+// last k slots, so the row is rewritten in O(k) stores instead of the
+// reference's order-preserving compaction (CompactSourceInPlace, :733-780):
+// the multiset of survivors is the same, their order may differ.
+__global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
+    __shared__ uint32_t slot_lds[kWavesPerBlock][kWave];
+    uint32_t *slots = slot_lds[wave_in_block()];
     const uint32_t cnt = ld_acquire(&a.ctl->small_cnt);
     const uint32_t lane = lane_id();
+    uint64_t acc_applied = 0, acc_missing = 0;
+    constexpr int U = 4;
     for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
         const uint32_t u = a.small_list[j];
         const Row r = a.rows[u];
         const uint32_t nd = uint32_t(a.cnt64[u]);
         const uint32_t *dl = a.grouped + a.gofs[u];
         const uint32_t mine = lane < nd ? dl[lane] : kNone;
+        const uint64_t all = nd >= 64 ? ~0ull : ((1ull << nd) - 1);
+        uint64_t pend = all;
+        uint32_t hole = kNone;
         uint32_t *row = a.pools.row(r.off);
-        uint32_t e[8];
-        bool alive[8];
+        for (uint32_t base = 0; base < r.deg && pend; base += kWave * U) {
+            uint32_t e[U];
+            bool live[U];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const uint32_t p = uint32_t(c) * kWave + lane;
-            alive[c] = p < r.deg;
-            e[c] = alive[c] ? row[p] : 0u;
-        }
-        uint32_t matched = 0;
-        for (uint32_t q = 0; q < nd; ++q) {
-            const uint32_t d = __shfl(mine, int(q), kWave);
-            bool found = false;
+            for (int q = 0; q < U; ++q) {
+                const uint32_t p = base + uint32_t(q) * kWave + lane;
+                live[q] = p < r.deg;
+                e[q] = live[q] ? row[p] : kNone;
+            }
+            for (uint64_t todo = pend; todo; todo &= todo - 1) {
+                const int jj = __ffsll((unsigned long long)todo) - 1;
+                const uint32_t d = __shfl(mine, jj, kWave);
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const uint64_t m = __ballot(alive[c] && e[c] == d);
-                if (m && !found) {
-                    if (lane == uint32_t(__ffsll((unsigned long long)m) - 1)) alive[c] = false;
-                    found = true;
+                for (int q = 0; q < U; ++q) {
+                    const uint64_t m = __ballot(live[q] && e[q] == d);
+                    if (m) {
+                        const uint32_t l = uint32_t(__ffsll((unsigned long long)m) - 1);
+                        if (lane == l) live[q] = false;
+                        if (lane == uint32_t(jj)) hole = base + uint32_t(q) * kWave + l;
+                        pend &= ~(1ull << jj);
+                        break;
+                    }
                 }
             }
-            matched += found ? 1u : 0u;
         }
-        if (matched) {
-            uint32_t w = 0;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const uint64_t m = __ballot(alive[c]);
-                if (alive[c]) row[w + uint32_t(__popcll(m & __lanemask_lt()))] = e[c];
-                w += uint32_t(__popcll(m));
+        const uint64_t matched = all & ~pend;
+        const uint32_t k = uint32_t(__popcll(matched));
+        if (k) {
+            const uint32_t newdeg = r.deg - k;
+            // tail slot newdeg+lane (lane < k): dead if it is one of the holes
+            bool tail_dead = false;
+            for (uint64_t mm = matched; mm; mm &= mm - 1) {
+                const uint32_t h = __shfl(hole, __ffsll((unsigned long long)mm) - 1, kWave);
+                tail_dead |= (lane < k) && h == newdeg + lane;
             }
-            if (lane == 0) a.rows[u].deg = w;
+            const bool tail_live = lane < k && !tail_dead;
+            const uint32_t tval = tail_live ? row[newdeg + lane] : 0u;
+            const bool front = ((matched >> lane) & 1ull) && hole < newdeg;
+            const uint64_t fm = __ballot(front);
+            const uint64_t lm = __ballot(tail_live);
+            if (front) slots[__popcll(fm & __lanemask_lt())] = hole;
+            wave_lds_sync();
+            if (tail_live) row[slots[__popcll(lm & __lanemask_lt())]] = tval;
+            wave_lds_sync();
+            if (lane == 0) a.rows[u].deg = newdeg;
         }
-        if (lane == 0) {
-            if (matched) atomicAdd((unsigned long long *)&a.ctl->deletes_applied, (unsigned long long)matched);
-            if (nd - matched) atomicAdd((unsigned long long *)&a.ctl->missing, (unsigned long long)(nd - matched));
-        }
+        acc_applied += k;
+        acc_missing += nd - k;
+    }
+    if (lane == 0) {
+        if (acc_applied) atomicAdd((unsigned long long *)&a.ctl->deletes_applied, (unsigned long long)acc_applied);
+        if (acc_missing) atomicAdd((unsigned long long *)&a.ctl->missing, (unsigned long long)acc_missing);
     }
 }
 
@@ -238,7 +278,9 @@ constexpr int kDelBigThreads = 1024;
 constexpr uint32_t kDelHT = 4096;
 constexpr uint32_t kDelPass = 2048;
 
-__device__ __forceinline__ uint32_t ht_hash(uint32_t x) { return (x * 0x9E3779B1u) >> (32 - 12); }
+__device__ __forceinline__ uint32_t ht_hash(uint32_t x, uint32_t bits) {
+    return (x * 0x9E3779B1u) >> (32 - bits);
+}
 
 __global__ __launch_bounds__(kDelBigThreads) void k_del_big(MutArgs a) {
     __shared__ uint32_t hkey[kDelHT];
@@ -258,17 +300,21 @@ __global__ __launch_bounds__(kDelBigThreads) void k_del_big(MutArgs a) {
         uint32_t deg = r.deg, total = 0;
         for (uint32_t p0 = 0; p0 < nd; p0 += kDelPass) {
             const uint32_t pn = min(kDelPass, nd - p0);
-            for (uint32_t s = tid; s < kDelHT; s += kDelBigThreads) { hkey[s] = kNone; hcnt[s] = 0; }
+            // table of 2*pn slots rounded up to a power of two (64..4096)
+            uint32_t bits = 6;
+            while ((1u << bits) < 2 * pn) ++bits;
+            const uint32_t ht = 1u << bits;
+            for (uint32_t s = tid; s < ht; s += kDelBigThreads) { hkey[s] = kNone; hcnt[s] = 0; }
             for (uint32_t s = tid; s < kDelPass / 32; s += kDelBigThreads) dead[s] = 0;
             if (tid == 0) { s_nh = 0; s_fc = 0; s_lc = 0; }
             __syncthreads();
             for (uint32_t q = tid; q < pn; q += kDelBigThreads) {
                 const uint32_t d = dl[p0 + q];
-                uint32_t h = ht_hash(d);
+                uint32_t h = ht_hash(d, bits);
                 for (;;) {
                     const uint32_t prev = atomicCAS(&hkey[h], kNone, d);
                     if (prev == kNone || prev == d) { atomicAdd(&hcnt[h], 1); break; }
-                    h = (h + 1) & (kDelHT - 1);
+                    h = (h + 1) & (ht - 1);
                 }
             }
             __syncthreads();
@@ -283,8 +329,8 @@ __global__ __launch_bounds__(kDelBigThreads) void k_del_big(MutArgs a) {
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t pos = b + uint32_t(k) * kDelBigThreads + tid;
                     if (pos >= deg) continue;
-                    uint32_t h = ht_hash(x[k]);
-                    while (hkey[h] != kNone && hkey[h] != x[k]) h = (h + 1) & (kDelHT - 1);
+                    uint32_t h = ht_hash(x[k], bits);
+                    while (hkey[h] != kNone && hkey[h] != x[k]) h = (h + 1) & (ht - 1);
                     if (hkey[h] == x[k] && atomicSub(&hcnt[h], 1) > 0)
                         holes[atomicAdd(&s_nh, 1u)] = pos;
                 }
@@ -325,6 +371,7 @@ __global__ __launch_bounds__(kDelBigThreads) void k_del_big(MutArgs a) {
 __global__ void k_add(MutArgs a, const uint32_t *list, const uint32_t *list_count) {
     const uint32_t cnt = ld_acquire(list_count);
     const uint32_t lane = lane_id();
+    uint64_t acc_adds = 0, acc_reloc = 0, acc_garbage[2] = {0, 0};
     for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
         const uint32_t u = list[j];
         const uint64_t c = a.cnt64[u];
@@ -349,8 +396,21 @@ __global__ void k_add(MutArgs a, const uint32_t *list, const uint32_t *list_coun
             const uint64_t noff = uint64_t(o) | (tier ? kHostTier : 0ull);
             const uint32_t *src = a.pools.row(r.off);
             uint32_t *dst = a.pools.row(noff);
-            if (r.deg <= 4096) {
-                for (uint32_t q = lane; q < r.deg; q += kWave) dst[q] = src[q];
+            if (r.deg <= 1024) {
+                constexpr int U = 8;
+                for (uint32_t b = 0; b < r.deg; b += kWave * U) {
+                    uint32_t v[U];
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const uint32_t p = b + uint32_t(q) * kWave + lane;
+                        v[q] = p < r.deg ? src[p] : 0u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        const uint32_t p = b + uint32_t(q) * kWave + lane;
+                        if (p < r.deg) dst[p] = v[q];
+                    }
+                }
             } else if (lane == 0) {
                 CopyJob jb;
                 jb.from = r.off;
@@ -366,21 +426,42 @@ __global__ void k_add(MutArgs a, const uint32_t *list, const uint32_t *list_coun
                 nr.deg = uint32_t(newdeg);
                 nr.cap = uint32_t(newcap);
                 a.rows[u] = nr;
-                atomicAdd((unsigned long long *)&a.ctl->garbage[tier], (unsigned long long)r.cap);
-                atomicAdd((unsigned long long *)&a.ctl->relocated, 1ull);
             }
+            acc_garbage[tier] += r.cap;
+            acc_reloc += 1;
         }
-        if (lane == 0) atomicAdd((unsigned long long *)&a.ctl->adds_applied, (unsigned long long)na);
+        acc_adds += na;
+    }
+    if (lane == 0) {
+        if (acc_adds) atomicAdd((unsigned long long *)&a.ctl->adds_applied, (unsigned long long)acc_adds);
+        if (acc_reloc) atomicAdd((unsigned long long *)&a.ctl->relocated, (unsigned long long)acc_reloc);
+        for (int t = 0; t < 2; ++t)
+            if (acc_garbage[t]) atomicAdd((unsigned long long *)&a.ctl->garbage[t], (unsigned long long)acc_garbage[t]);
     }
 }
 
+// Copies long relocated rows: every workgroup takes 1024-slot tiles of each
+// job, so one long row is spread over many workgroups.
 __global__ void k_copy_jobs(MutArgs a) {
     const uint32_t cnt = ld_acquire(&a.ctl->copy_cnt);
-    for (uint32_t j = blockIdx.x; j < cnt; j += gridDim.x) {
+    constexpr uint32_t kTile = 1024;
+    for (uint32_t j = 0; j < cnt; ++j) {
         const CopyJob jb = a.jobs[j];
         const uint32_t *src = a.pools.row(jb.from);
         uint32_t *dst = a.pools.row(jb.to);
-        for (uint32_t q = threadIdx.x; q < jb.len; q += blockDim.x) dst[q] = src[q];
+        for (uint32_t t = blockIdx.x * kTile; t < jb.len; t += gridDim.x * kTile) {
+            uint32_t v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t p = t + uint32_t(q) * blockDim.x + threadIdx.x;
+                v[q] = p < jb.len ? src[p] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t p = t + uint32_t(q) * blockDim.x + threadIdx.x;
+                if (p < jb.len) dst[p] = v[q];
+            }
+        }
     }
 }
 
@@ -411,7 +492,6 @@ struct InvalArgs {
     uint32_t n;
     uint32_t kind;  // 0 = single source (SSSP/BFS), 1 = vertex seeds (CC)
     uint32_t max_rounds;
-    uint32_t coop;
     uint64_t seed;
 };
 
@@ -457,6 +537,40 @@ __global__ __launch_bounds__(kBlock) void k_inval_loop(InvalArgs a) {
         cg::this_grid().sync();
     }
     flush_counters(ctr, c, a.cfg.stage);
+}
+
+// Opens the invalidation frontier after the seeds: [0, acount).
+__global__ void k_inval_begin(Ctl *c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->ibegin = 0;
+    c->iend = c->acount;
+    c->iround = 0;
+    c->icount[0] = c->icount[1] = c->icount[2] = 0;
+}
+
+// One invalidation wave as an ordinary launch: expands [ibegin, iend) of the
+// affected list; the closing workgroup moves the window to the items this
+// round appended.
+template <int EPL>
+__global__ __launch_bounds__(kBlock) void k_inval_round(InvalArgs a) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    Ctl *c = a.ctl;
+    const uint32_t b = c->ibegin, e = c->iend, k = c->iround;
+    const uint32_t nt = c->icount[k % 3];
+    if (b == e && nt == 0) return;
+    InvalidateOp op{a.rows, a.state, a.aff};
+    Counters ctr;
+    Sink sink{a.alist, &c->acount};
+    TileSink ts{(k & 1) ? a.t0 : a.t1, &c->icount[(k + 1) % 3]};
+    expand_range<EPL>(op, a.cfg, a.alist + b, e - b, sink, ts, lds[wave_in_block()], ctr);
+    expand_tiles<EPL>(op, a.cfg, (k & 1) ? a.t1 : a.t0, nt, sink, ctr);
+    flush_counters(ctr, c, a.cfg.stage);
+    close_round(c, [&] {
+        c->icount[k % 3] = 0;
+        c->ibegin = e;
+        c->iend = ld_acquire(&c->acount);
+        c->iround = k + 1;
+    });
 }
 
 // reset_affected_values (driver.cuh:418-433): affected vertices restart from
@@ -510,6 +624,104 @@ __global__ void k_insert_seed(InvalArgs a, uint32_t skip_affected) {
     }
 }
 
+// -------------------------------------------------------------- repair scan
+// Every workgroup first folds the affected list into an LDS bitmap filter of
+// 2^18 bits (a multiplicative hash of the id); an edge whose destination
+// misses the filter is not affected, so almost every edge of the scan is
+// settled in LDS and only filter hits read the global affected bitmap. The
+// filter is skipped when the affected list is too long to stay selective.
+constexpr uint32_t kFilterBits = 18;
+constexpr uint32_t kFilterWords = (1u << kFilterBits) / 32;
+constexpr uint32_t kFilterMaxItems = 1u << 16;
+
+__device__ __forceinline__ uint32_t filter_hash(uint32_t v) { return (v * 0x9E3779B1u) >> (32 - kFilterBits); }
+
+struct AffFilter {
+    const uint32_t *lds;  // nullptr: filter off
+    const uint32_t *aff;
+    __device__ __forceinline__ bool test(uint32_t v) const {
+        if (lds) {
+            const uint32_t h = filter_hash(v);
+            if (!((lds[h >> 5] >> (h & 31)) & 1u)) return false;
+        }
+        return bit_test(aff, v);
+    }
+};
+
+__device__ bool build_filter(uint32_t *lds, const uint32_t *alist, uint32_t count) {
+    if (count > kFilterMaxItems) return false;
+    for (uint32_t w = threadIdx.x; w < kFilterWords; w += blockDim.x) lds[w] = 0u;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+        const uint32_t h = filter_hash(alist[i]);
+        atomicOr(&lds[h >> 5], 1u << (h & 31));
+    }
+    __syncthreads();
+    return true;
+}
+
+template <int W, bool kStrict>
+struct RepairScanOp {
+    const Row *rows;
+    uint64_t *state;
+    AffFilter filt;
+    uint64_t seed;
+    __device__ bool source(uint32_t v, uint32_t, Row &row, uint32_t &pay) const {
+        if (filt.test(v)) return false;
+        const uint32_t val = uint32_t(state[v] >> 32);
+        if (val == kInf) return false;
+        row = rows[v];
+        pay = val;
+        return row.deg != 0;
+    }
+    __device__ bool tile_source(const Tile &t, uint32_t &pay) const {
+        pay = uint32_t(state[t.v] >> 32);
+        return pay != kInf;
+    }
+    // the LDS filter answers most edges without a global access
+    __device__ uint64_t peek(uint32_t y) const {
+        if (!filt.lds) return 1;
+        const uint32_t h = filter_hash(y);
+        return (filt.lds[h >> 5] >> (h & 31)) & 1u;
+    }
+    __device__ bool edge(uint32_t u, uint32_t val, uint32_t y, uint64_t maybe) const {
+        if (!maybe || !bit_test(filt.aff, y)) return false;
+        const uint64_t c = uint64_t(val) + weight<W>(u, y, seed);
+        if (c < kInf) relax<kStrict>(state, y, uint32_t(c), u);
+        return false;
+    }
+};
+
+template <int W, bool kStrict, int EPL>
+__global__ __launch_bounds__(kBlock) void k_repair_scan(InvalArgs a) {
+    __shared__ uint32_t filt[kFilterWords];
+    __shared__ WaveLds lds[kWavesPerBlock];
+    Ctl *c = a.ctl;
+    const uint32_t na = ld_acquire(&c->acount);
+    if (na == 0) return;
+    const bool on = build_filter(filt, a.alist, na);
+    RepairScanOp<W, kStrict> op{a.rows, a.state, AffFilter{on ? filt : nullptr, a.aff}, a.seed};
+    Counters ctr;
+    Sink none{nullptr, &c->scount};
+    TileSink ts{a.t0, &c->scount};
+    expand_range<EPL>(op, a.cfg, nullptr, a.n, none, ts, lds[wave_in_block()], ctr);
+    flush_counters(ctr, c, a.cfg.stage);
+}
+
+template <int W, bool kStrict, int EPL>
+__global__ __launch_bounds__(kBlock) void k_repair_tiles(InvalArgs a) {
+    __shared__ uint32_t filt[kFilterWords];
+    Ctl *c = a.ctl;
+    const uint32_t na = ld_acquire(&c->acount);
+    const uint32_t nt = ld_acquire(&c->scount);
+    if (na == 0 || nt == 0) return;
+    const bool on = build_filter(filt, a.alist, na);
+    RepairScanOp<W, kStrict> op{a.rows, a.state, AffFilter{on ? filt : nullptr, a.aff}, a.seed};
+    Counters ctr;
+    Sink none{nullptr, &c->scount};
+    expand_tiles<EPL>(op, a.cfg, a.t0, nt, none, ctr);
+}
+
 // ------------------------------------------------- one-shot expansion passes
 template <int EPL, class Op>
 __global__ __launch_bounds__(kBlock) void k_expand_once(Op op, ExpandCfg cfg, const uint32_t *items,
@@ -544,7 +756,6 @@ struct LoopArgs {
     ExpandCfg cfg;
     uint64_t seed;
     uint32_t max_rounds;
-    uint32_t coop;
 };
 
 // One persistent cooperative launch runs rounds until the frontier and the
@@ -553,7 +764,6 @@ struct LoopArgs {
 // queue[r&1] / count slot r%3 and produces into the other queue and slot
 // (r+1)%3; slot (r+2)%3 is cleared for round r+1. Tiles of long rows found in
 // round r are expanded in round r+1 (re-reading the row owner's value).
-// coop == 0 runs exactly one round (host-driven mode; k_advance_round follows).
 template <int W, bool kStrict, int EPL>
 __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
     __shared__ WaveLds lds[kWavesPerBlock];
@@ -566,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
         const uint32_t nq = ld_acquire(&c->qcount[r % 3]);
         const uint32_t nt = ld_acquire(&c->tcount[r % 3]);
         if (nq == 0 && nt == 0) break;
-        if (a.coop && blockIdx.x == 0 && threadIdx.x == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
             c->qcount[(r + 2) % 3] = 0;
             c->tcount[(r + 2) % 3] = 0;
         }
@@ -576,25 +786,56 @@ __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
         TileSink ts{(r & 1) ? a.t0 : a.t1, &c->tcount[(r + 1) % 3]};
         expand_range<EPL>(op, a.cfg, qin, nq, sink, ts, L, ctr);
         expand_tiles<EPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, sink, ctr);
-        if (!a.coop) { ++it; ++r; break; }
         cg::this_grid().sync();
     }
     flush_counters(ctr, c, a.cfg.stage);
-    if (a.coop && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->round = r;
         c->rounds += it;
     }
 }
 
-// Host-driven mode: closes the round just run.
-__global__ void k_advance_round(Ctl *c) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The last workgroup of a round kernel to finish runs `close` on thread 0.
+// Every workgroup publishes its writes (fence) before counting itself done,
+// so `close` sees all appends of the round; the next kernel in the stream
+// sees what `close` wrote. No workgroup ever waits for another.
+template <class F>
+__device__ __forceinline__ void close_round(Ctl *c, F close) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&c->done, 1u) == gridDim.x - 1) {
+            __threadfence();
+            c->done = 0;
+            close();
+        }
+    }
+}
+
+// One relax round as an ordinary launch (the default driver): the host
+// enqueues rounds back to back; a round whose frontier and deferred tiles
+// are empty returns at once.
+template <int W, bool kStrict, int EPL>
+__global__ __launch_bounds__(kBlock) void k_relax_round(LoopArgs a) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    Ctl *c = a.ctl;
     const uint32_t r = c->round;
-    if (c->qcount[r % 3] == 0 && c->tcount[r % 3] == 0) return;
-    c->qcount[r % 3] = 0;
-    c->tcount[r % 3] = 0;
-    c->round = r + 1;
-    c->rounds += 1;
+    const uint32_t nq = c->qcount[r % 3], nt = c->tcount[r % 3];
+    if (nq == 0 && nt == 0) return;
+    Counters ctr;
+    RelaxOp<W, kStrict> op{a.rows, a.state, a.stamp, a.hot, a.seed, r + 1};
+    const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
+    Sink sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]};
+    TileSink ts{(r & 1) ? a.t0 : a.t1, &c->tcount[(r + 1) % 3]};
+    expand_range<EPL>(op, a.cfg, qin, nq, sink, ts, lds[wave_in_block()], ctr);
+    expand_tiles<EPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, sink, ctr);
+    flush_counters(ctr, c, a.cfg.stage);
+    close_round(c, [&] {
+        c->qcount[r % 3] = 0;
+        c->tcount[r % 3] = 0;
+        c->round = r + 1;
+        c->rounds += 1;
+    });
 }
 
 // ---------------------------------------------------------- PageRank rounds
@@ -684,6 +925,84 @@ __global__ __launch_bounds__(kBlock) void k_pr_loop(PrArgs a) {
         c->rounds += it;
         c->converged = ld_acquire(&c->qcount[r % 3]) == 0 ? 1u : 0u;
     }
+}
+
+// PageRank round, consume phase (pr_residual.cuh:54-71) as an ordinary
+// launch; long rows are cut into tiles for this round's scatter.
+__global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a) {
+    Ctl *c = a.ctl;
+    const uint32_t r = c->round;
+    const uint32_t nq = c->qcount[r % 3];
+    if (nq == 0) return;
+    const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
+    Tile *tiles = (r & 1) ? a.t1 : a.t0;
+    Counters ctr;
+    for (uint32_t i = gthread(); i < nq; i += gthreads()) {
+        const uint32_t v = qin[i];
+        const float rv = a.res[v];
+        a.queued[v] = 0u;
+        float d = 0.0f;
+        if (!isfinite(rv) || !isfinite(a.rank[v])) atomicExch(&c->pr_invalid, 1u);
+        if (fabsf(rv) > a.eps) {
+            a.res[v] = 0.0f;
+            const float x = a.rank[v];
+            const float y = rv - a.comp[v];
+            const float t = x + y;
+            a.comp[v] = (t - x) - y;
+            a.rank[v] = t;
+            const Row row = a.rows[v];
+            ctr.vexp += 1;
+            if (row.deg) {
+                d = 0.85f * rv / float(row.deg);
+                if (row.deg >= a.cfg.split) {
+                    const uint32_t nt = (row.deg + a.cfg.tile_len - 1) / a.cfg.tile_len;
+                    const uint32_t t0 = atomicAdd(&c->tcount[r % 3], nt);
+                    for (uint32_t j = 0; j < nt; ++j) {
+                        Tile t;
+                        t.start = row.off + uint64_t(j) * a.cfg.tile_len;
+                        t.v = v;
+                        t.len = min(a.cfg.tile_len, row.deg - j * a.cfg.tile_len);
+                        t.payload = __float_as_uint(d);
+                        t.pad = 0;
+                        tiles[t0 + j] = t;
+                    }
+                    ctr.scanned += row.deg;
+                    if (row.off & kHostTier) ctr.host += row.deg;
+                    d = 0.0f;
+                }
+            }
+        }
+        a.delta[i] = d;
+    }
+    flush_counters(ctr, c, a.cfg.stage);
+}
+
+// PageRank round, scatter phase (pr_residual.cuh:74-100); the closing
+// workgroup advances the round.
+template <int EPL>
+__global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    Ctl *c = a.ctl;
+    const uint32_t r = c->round;
+    const uint32_t nq = c->qcount[r % 3];
+    if (nq == 0) return;
+    const uint32_t nt = c->tcount[r % 3];
+    const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
+    Tile *tiles = (r & 1) ? a.t1 : a.t0;
+    PrScatterOp op{a.rows, a.res, a.queued, a.delta, &c->pr_invalid, a.eps};
+    Sink sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]};
+    TileSink ts{tiles, &c->scount};  // unused: long rows were tiled while consuming
+    Counters ctr;
+    expand_range<EPL>(op, a.cfg, qin, nq, sink, ts, lds[wave_in_block()], ctr);
+    expand_tiles<EPL>(op, a.cfg, tiles, nt, sink, ctr);
+    flush_counters(ctr, c, a.cfg.stage);
+    close_round(c, [&] {
+        c->qcount[r % 3] = 0;
+        c->tcount[r % 3] = 0;
+        c->round = r + 1;
+        c->rounds += 1;
+        c->converged = ld_acquire(&c->qcount[(r + 1) % 3]) == 0 ? 1u : 0u;
+    });
 }
 
 // Plain streaming copy (16 B per lane per step): the HBM reference point for

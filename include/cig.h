@@ -68,9 +68,9 @@ typedef struct cig_edge {
 } cig_edge;
 
 enum {
-    CIG_CFG_HOST_ROUNDS = 1u << 0, /* drive fixed-point rounds from the host
-                                      (one launch per round) instead of the
-                                      persistent cooperative kernel */
+    CIG_CFG_COOPERATIVE = 1u << 0, /* run fixed points as one persistent
+                                      cooperative launch with grid barriers
+                                      instead of chained round kernels */
     CIG_CFG_NO_HOTNESS = 1u << 1   /* do not count row activity */
 };
 
@@ -95,7 +95,9 @@ typedef struct cig_config {
     uint32_t pr_max_rounds;   /* rounds per solve; the reference's flag allows
                                  1..100 (hybrid_pr.cu:22), the library 1..10^6 */
     float pr_epsilon;         /* absolute residual threshold (hybrid_pr.cu:23) */
-    uint32_t reserved[7];
+    uint32_t tile_len;        /* slots per tile of a split row; 0 = default */
+    uint32_t round_blocks_per_cu; /* workgroups per CU of round kernels; 0 = default */
+    uint32_t reserved[5];
 } cig_config;
 
 /* Per-call statistics. Times are device (HIP event) milliseconds of the

@@ -248,20 +248,34 @@ def test_resident_stream_matches_calls():
         assert b == s.nbatches
 
 
-def test_host_rounds_mode_matches():
-    """Host-driven rounds (one launch per round) give the same fixed point."""
-    _announce("host rounds")
-    g = cg.rmat_graph(11, 16, seed=16)
-    s = cg.rmat_stream(g, 11, 3, 200, 200, seed=17)
+@pytest.mark.parametrize("algo", ["sssp", "cc", "pr"])
+def test_cooperative_mode_matches(algo):
+    """The persistent cooperative driver reaches the same fixed point as the
+    chained round kernels."""
+    _announce(f"cooperative {algo}")
+    g = cg.rmat_graph(11, 16, seed=16, symmetric=algo == "cc")
+    s = cg.rmat_stream(g, 11, 3, 200, 200, seed=17, symmetric=algo == "cc")
     og = oracle_graph(g)
-    with cg.Engine("sssp", flags=cg.CFG_HOST_ROUNDS) as e:
+    if algo == "pr":
+        with cg.Engine("pr", flags=cg.CFG_COOPERATIVE, pr_epsilon=1e-7, pr_max_rounds=2000) as e:
+            e.load_graph(g)
+            e.init_graph()
+            e.start()
+            for b in range(s.nbatches):
+                st = e.apply_batch(*s.batch(b))
+                oracle_apply(og, *s.batch(b))
+                assert st["converged"] == 1
+                l1n, _ = pr_errors(e.values(), og.pagerank()[0])
+                assert l1n <= 1e-6, (b, l1n)
+        return
+    with cg.Engine(algo, flags=cg.CFG_COOPERATIVE) as e:
         e.load_graph(g)
         e.init_graph(0)
         e.start()
         for b in range(s.nbatches):
             e.apply_batch(*s.batch(b))
             oracle_apply(og, *s.batch(b))
-            check_traversal(e, og, "sssp", 0, where=f"host rounds {b}")
+            check_traversal(e, og, algo, 0, where=f"cooperative {b}")
 
 
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "cc", "pr"])
