@@ -1002,6 +1002,7 @@ struct cig_engine {
         if (td) HIPCHK(hipMemcpy(s_dels.p, dels, td * sizeof(cig_edge), hipMemcpyHostToDevice));
         batch_sizes.assign(sizes, sizes + 2 * nb);
         next_batch = 0;
+        deletes_done = false;
         add_off = del_off = 0;
         if (grouped.count < max_batch) grouped.alloc(max_batch);
     }
@@ -1010,11 +1011,27 @@ struct cig_engine {
         check_ready();
         if (next_batch * 2 >= batch_sizes.size()) throw HipError(CIG_ELIMIT, "no update batch left");
         const uint32_t na = batch_sizes[2 * next_batch], nd = batch_sizes[2 * next_batch + 1];
-        apply_device(s_dels.p + del_off, nd, s_adds.p + add_off, na, flags, st);
+        const bool del_phase = flags & CIG_BATCH_DELETE_PHASE, add_phase = flags & CIG_BATCH_ADD_PHASE;
+        if (del_phase && add_phase) throw HipError(CIG_EINVAL, "choose one phase");
+        if (del_phase) {
+            if (deletes_done) throw HipError(CIG_ESTATE, "the deletes of this batch were applied already");
+            apply_device(s_dels.p + del_off, nd, nullptr, 0, 0, st);
+            deletes_done = true;
+            return;
+        }
+        if (add_phase) {
+            if (!deletes_done) throw HipError(CIG_ESTATE, "apply the delete phase of this batch first");
+            apply_device(nullptr, 0, s_adds.p + add_off, na, 0, st);
+        } else {
+            if (deletes_done) throw HipError(CIG_ESTATE, "finish the add phase of this batch first");
+            apply_device(s_dels.p + del_off, nd, s_adds.p + add_off, na, flags, st);
+        }
+        deletes_done = false;
         add_off += na;
         del_off += nd;
         ++next_batch;
     }
+    bool deletes_done = false;
 
     // ------------------------------------------------------------ cache
     // Sort-based admission (compute_hot_vertices + confirm_candidate_batch,

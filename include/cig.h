@@ -206,9 +206,14 @@ cig_status cig_delete_edges(cig_engine *e, const cig_edge *dels, uint64_t n_del,
 cig_status cig_insert_edges(cig_engine *e, const cig_edge *adds, uint64_t n_add,
                             cig_batch_stats *stats);
 enum {
-    CIG_BATCH_STAGED = 1u << 0 /* reach the post-delete fixed point before
-                                  applying adds (the reference's two stages);
-                                  default fuses them into one repair */
+    CIG_BATCH_STAGED = 1u << 0,       /* reach the post-delete fixed point before
+                                         applying adds (the reference's two
+                                         stages); default fuses them into one repair */
+    CIG_BATCH_DELETE_PHASE = 1u << 1, /* cig_apply_next_batch: apply only the
+                                         current batch's deletes (del_edge); the
+                                         batch stays current */
+    CIG_BATCH_ADD_PHASE = 1u << 2     /* cig_apply_next_batch: apply only the
+                                         current batch's adds (add_edge) and advance */
 };
 /* One update batch: all deletes, then all adds (README.md:17-37). For PR this
  * is Engine::UpdatePageRank (framework.cuh:1660-1704). */

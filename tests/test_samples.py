@@ -1,0 +1,124 @@
+"""The C++17 samples hybrid_{bfs,sssp,cc,pr}: reference flags, --check
+self-checks, --output format. CPU tests cover the flag parser; GPU tests run
+the binaries end to end and compare --output with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import coincgraph_amd as cg
+from oracle import oracle as O
+from tests.helpers import oracle_apply, oracle_graph, pr_errors
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "coincgraph_amd", "bin")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "readme")
+
+
+def _exe(app):
+    path = os.path.join(BIN, f"hybrid_{app}")
+    if not os.path.exists(path):
+        pytest.skip("samples not built")
+    return path
+
+
+def _run(app, *args, timeout=120):
+    return subprocess.run([_exe(app), *args], capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pr"])
+def test_help_lists_reference_flags(app):
+    out = _run(app, "--help").stdout
+    for flag in ("graphfile", "updatefile", "update_size", "cache", "n_stream", "check",
+                 f"{app}_max_batches", "output", "SEGMENT"):
+        assert f"--{flag} " in out, flag
+
+
+def test_unknown_flag_is_rejected():
+    r = _run("sssp", "--no_such_flag=1")
+    assert r.returncode != 0 and "unknown command line flag" in r.stderr
+
+
+def test_pr_flag_validation_matches_reference():
+    # hybrid_pr.cu:129-132: error > 0 and 1 <= pr_max_rounds <= 100
+    g = os.path.join(GOLDEN, "graph.txt")
+    for bad in (["--pr_max_rounds=0"], ["--pr_max_rounds=101"], ["--error=0"]):
+        r = _run("pr", f"--graphfile={g}", *bad)
+        assert r.returncode == 1 and "PR requires" in r.stderr, bad
+
+
+def _write_graph(path, g):
+    src = np.repeat(np.arange(g.n), np.diff(g.offsets).astype(np.int64))
+    # make sure the largest id appears so the reader infers the same n
+    with open(path, "w") as f:
+        for s, d in zip(src.tolist(), g.dst.tolist()):
+            f.write(f"{s} {d}\n")
+        if g.n - 1 not in set(src.tolist()) | set(g.dst.tolist()):
+            f.write(f"{g.n - 1} {g.n - 1}\n")
+
+
+def _write_updates(upath, bpath, s):
+    with open(upath, "w") as uf, open(bpath, "w") as bf:
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            for e in dels:
+                uf.write(f"d {e['u']} {e['v']} 1\n")
+            for e in adds:
+                uf.write(f"a {e['u']} {e['v']} 1\n")
+            bf.write(f"{len(adds)} {len(dels)}\n")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pr"])
+def test_sample_readme_fixture(app, tmp_path):
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+    out = tmp_path / "out.txt"
+    args = [f"--graphfile={GOLDEN}/graph.txt", "--format=market_big",
+            f"--updatefile={GOLDEN}/updates.txt", f"--update_size={GOLDEN}/batches.txt",
+            "--check=true", f"--output={out}", "--cache=2"]
+    if app == "pr":
+        args += ["--error=0.0000001"]
+    r = _run(app, *args)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert "Overall: Test passed" in r.stdout
+    got = np.loadtxt(out, ndmin=2)
+    exp = np.loadtxt(os.path.join(GOLDEN, f"expected_{app}.txt"), ndmin=2)
+    if app == "pr":
+        np.testing.assert_allclose(got[:, 1], exp[:, 1], rtol=2e-6, atol=1e-6)
+    else:
+        assert got[:, 1].astype(np.int64).tolist() == exp[:, 1].astype(np.int64).tolist()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pr"])
+def test_sample_rmat_stream(app, tmp_path):
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+    g = cg.rmat_graph(11, 8, seed=31, symmetric=app == "cc")
+    s = cg.rmat_stream(g, 11, 4, 150, 150, seed=32, symmetric=app == "cc")
+    gp, up, bp, out = (tmp_path / x for x in ("g.txt", "u.txt", "b.txt", "o.txt"))
+    _write_graph(gp, g)
+    _write_updates(up, bp, s)
+    args = [f"--graphfile={gp}", f"--updatefile={up}", f"--update_size={bp}", "--check=true",
+            f"--output={out}", f"--{app}_max_batches=10", "--source_node=2"]
+    if app == "pr":
+        args += ["--error=0.0000001", "--pr_max_rounds=100"]
+    r = _run(app, *args)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert "clamp" in r.stdout  # 10 requested, 4 available (hybrid_sssp.cu:402-407)
+    og = oracle_graph(cg.read_market(str(gp)))
+    for b in range(s.nbatches):
+        oracle_apply(og, *s.batch(b))
+    got = np.loadtxt(out, ndmin=2)
+    if app == "pr":
+        # 100 rounds may stop short of epsilon (the reference's limit); the
+        # sample then reports iteration_limit and its own check decides
+        l1n, _ = pr_errors(got[:, 1].astype(np.float32), og.pagerank()[0])
+        assert l1n < 1e-4
+        return
+    assert "Overall: Test passed" in r.stdout
+    exp = {"bfs": lambda: og.bfs(2), "sssp": lambda: og.sssp(2), "cc": og.cc}[app]()
+    assert got[:, 1].astype(np.int64).tolist() == exp.astype(np.int64).tolist()
+    assert r.stdout.count("BATCH-CHECK") == 4 and "failed" not in r.stdout
