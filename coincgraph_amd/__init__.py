@@ -80,7 +80,7 @@ class BatchStats(C.Structure):
         ("touched_sources", C.c_uint64), ("converged", C.c_uint32), ("launches", C.c_uint32),
         ("pr_residual_l1", C.c_double), ("stage_edges", C.c_uint64 * 4),
         ("stage_vertices", C.c_uint64 * 4), ("stage_host_edges", C.c_uint64 * 4),
-        ("reserved", C.c_uint64 * 4),
+        ("scan_kernel_ms", C.c_double), ("reserved", C.c_uint64 * 3),
     ]
 
     def as_dict(self):

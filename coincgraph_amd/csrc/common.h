@@ -64,6 +64,7 @@ struct Ctl {
     // mutation
     uint32_t touched;
     uint32_t small_cnt, big_cnt, add_cnt, copy_cnt, overflow_cnt;
+    uint32_t dtile_cnt, ht_cursor;  // long-row delete tiles and table slots
     uint32_t pr_invalid;
     uint64_t group_cursor;
     uint64_t pool_tail[2]; // bump tails of the HBM (0) and host (1) pools
@@ -85,6 +86,20 @@ struct Ctl {
     uint64_t st_vexp[4];
     uint64_t st_host[4];
 };
+
+// Statistics are added into one of kStatShards 256-B shards (picked by the
+// adding wave) so that no single address takes every wave's atomic; a
+// one-workgroup kernel folds the shards into the control block.
+constexpr uint32_t kStatShards = 64;
+struct StatShard {
+    uint64_t edges[4], vexp[4], host[4];  // per expansion pass (Stage)
+    uint64_t deletes_applied, missing, invalid, adds_applied, relocated;
+    uint64_t garbage[2];
+    double l1;             // PR residual norm
+    uint64_t pad[12];
+};
+static_assert(sizeof(StatShard) == 256, "StatShard layout");
+constexpr uint32_t kStatWords = 19;  // leading uint64 fields of a shard
 
 enum Stage : uint32_t { kStageInvalidate = 0, kStageRepair = 1, kStagePropagate = 2, kStageAmend = 3 };
 

@@ -1,10 +1,10 @@
 // Device building blocks of the incremental graph engine (gfx950, wave64).
 //
 // Frontier expansion is wave-cooperative: a wave takes 64 frontier items,
-// scans their row lengths in registers, stages (prefix, owner, payload, row
-// address) in LDS and then walks the concatenated rows 64*EPL edges at a time,
-// so consecutive lanes read consecutive 4-B slots of a row (256-B coalesced
-// wave reads). Rows at least `split` long are cut into tiles of `tile_len`
+// scans their row lengths (in 4-slot quads) in registers, stages (prefix,
+// owner, payload, degree, row address) in LDS and then walks the concatenated
+// rows 64*QPL quads at a time, so consecutive lanes read consecutive 16-B
+// quads of a row (1-KB coalesced wave reads, one owner search per quad). Rows at least `split` long are cut into tiles of `tile_len`
 // slots that any wave of the grid expands. Outputs are appended with one
 // atomic per wave (ballot + popcount ranks). This replaces the reference's
 // CTA/warp degree scheduler (include/groute/device/cta_scheduler_hybrid.cuh:
@@ -249,10 +249,17 @@ struct PrAmendOp {
 };
 
 // ---------------------------------------------------------------- expansion
+// Rows are read in quads: four consecutive slots with one 16-B load. Every
+// row starts on a 4-slot boundary (its capacity, and so every allocation of
+// the bump-allocated pools, is a power of two >= 4) and owns
+// round_up4(deg) <= cap slots, so a quad never leaves its row's block. Slots
+// of the last quad past `deg` are masked; their loaded words are replaced by
+// the quad's first (live) slot before any use as an index.
 struct WaveLds {
-    uint32_t pref[kWave];
+    uint32_t pref[kWave];  // exclusive scan of quad counts
     uint32_t v[kWave];
     uint32_t pay[kWave];
+    uint32_t deg[kWave];
     uint32_t offlo[kWave];
     uint32_t offhi[kWave];
 };
@@ -260,8 +267,9 @@ struct WaveLds {
 struct ExpandCfg {
     Pools pools;
     uint32_t split;     // rows with deg >= split go to tiles
-    uint32_t tile_len;  // slots per tile
+    uint32_t tile_len;  // slots per tile (a multiple of 4)
     uint32_t stage;     // Stage whose counters this pass feeds
+    StatShard *stats;
 };
 
 struct Sink {
@@ -278,6 +286,187 @@ struct Counters {
     uint64_t scanned = 0, host = 0, vexp = 0;
 };
 
+// ------------------------------------------------------------ appends
+// Device-scope atomics on one address serialise where memory resolves them,
+// so every append path reserves space for a whole wave (DirectOut) or for a
+// whole workgroup (StagedOut): waves reserve in an LDS buffer with an LDS
+// atomic and the workgroup moves the buffer to the global list with one
+// global atomic when it finishes (out_flush). A reservation that does not
+// fit in LDS spills its excess to the global list directly.
+
+// A reservation of `total` consecutive elements: element i goes to LDS slot
+// b+i when b+i < cap, else to global slot gb + (b+i) - max(b, cap).
+struct Resv {
+    uint32_t b, cap, gb;
+};
+
+template <class T>
+__device__ __forceinline__ void resv_put(const Resv &r, uint32_t i, T *lds, T *glob, const T &val) {
+    const uint32_t k = r.b + i;
+    if (k < r.cap) lds[k] = val;
+    else glob[r.gb + k - max(r.b, r.cap)] = val;
+}
+
+// Called by the whole wave with a wave-uniform total.
+__device__ __forceinline__ Resv reserve_global(uint32_t *count, uint32_t total) {
+    uint32_t g = 0;
+    if (lane_id() == 0) g = atomicAdd(count, total);
+    return Resv{0u, 0u, __shfl(g, 0, kWave)};
+}
+
+__device__ __forceinline__ Resv reserve_staged(uint32_t *lds_n, uint32_t cap, uint32_t *count,
+                                               uint32_t total) {
+    uint32_t b = 0, g = 0;
+    if (lane_id() == 0) {
+        b = atomicAdd(lds_n, total);
+        const uint32_t lo = max(b, cap);
+        if (b + total > lo) g = atomicAdd(count, b + total - lo);
+    }
+    return Resv{__shfl(b, 0, kWave), cap, __shfl(g, 0, kWave)};
+}
+
+struct DirectOut {
+    Sink sink;
+    TileSink ts;
+};
+
+template <uint32_t QCAP, uint32_t TCAP>
+struct StageLds {
+    uint32_t qn, qbase, tn, tbase;
+    uint32_t q[QCAP];
+    Tile t[TCAP];
+    static_assert(QCAP % 2 == 0, "keeps the tile array 8-B aligned");
+};
+
+template <uint32_t QCAP, uint32_t TCAP>
+struct StagedOut {
+    Sink sink;
+    TileSink ts;
+    StageLds<QCAP, TCAP> *lds;
+};
+
+__device__ __forceinline__ Resv out_reserve_q(const DirectOut &o, uint32_t total) {
+    return reserve_global(o.sink.count, total);
+}
+__device__ __forceinline__ Resv out_reserve_t(const DirectOut &o, uint32_t total) {
+    return reserve_global(o.ts.count, total);
+}
+__device__ __forceinline__ uint32_t *out_qlds(const DirectOut &) { return nullptr; }
+__device__ __forceinline__ Tile *out_tlds(const DirectOut &) { return nullptr; }
+__device__ __forceinline__ void out_init(const DirectOut &) {}
+__device__ __forceinline__ void out_flush(const DirectOut &) {}
+
+template <uint32_t Q, uint32_t T>
+__device__ __forceinline__ Resv out_reserve_q(const StagedOut<Q, T> &o, uint32_t total) {
+    return reserve_staged(&o.lds->qn, Q, o.sink.count, total);
+}
+template <uint32_t Q, uint32_t T>
+__device__ __forceinline__ Resv out_reserve_t(const StagedOut<Q, T> &o, uint32_t total) {
+    return reserve_staged(&o.lds->tn, T, o.ts.count, total);
+}
+template <uint32_t Q, uint32_t T>
+__device__ __forceinline__ uint32_t *out_qlds(const StagedOut<Q, T> &o) { return o.lds->q; }
+template <uint32_t Q, uint32_t T>
+__device__ __forceinline__ Tile *out_tlds(const StagedOut<Q, T> &o) { return o.lds->t; }
+
+// Workgroup-level: every thread must call these (uniform control flow).
+template <uint32_t Q, uint32_t T>
+__device__ __forceinline__ void out_init(const StagedOut<Q, T> &o) {
+    if (threadIdx.x == 0) o.lds->qn = o.lds->tn = 0u;
+    __syncthreads();
+}
+template <uint32_t Q, uint32_t T>
+__device__ void out_flush(const StagedOut<Q, T> &o) {
+    StageLds<Q, T> *s = o.lds;
+    __syncthreads();
+    const uint32_t nq = min(s->qn, Q), nt = min(s->tn, T);
+    if (threadIdx.x == 0) {
+        s->qbase = nq ? atomicAdd(o.sink.count, nq) : 0u;
+        s->tbase = nt ? atomicAdd(o.ts.count, nt) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nq; i += blockDim.x) o.sink.list[s->qbase + i] = s->q[i];
+    for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) o.ts.tiles[s->tbase + i] = s->t[i];
+}
+
+// Appends v[k] for every set f[k] of every lane (wave-uniform call).
+template <int K, class Out>
+__device__ __forceinline__ void out_push(const Out &o, const bool (&f)[K], const uint32_t (&v)[K]) {
+    uint64_t m[K];
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        m[k] = __ballot(f[k]);
+        total += uint32_t(__popcll(m[k]));
+    }
+    if (total == 0) return;
+    const Resv r = out_reserve_q(o, total);
+    uint32_t *lq = out_qlds(o);
+    const uint64_t lt = __lanemask_lt();
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (f[k]) resv_put(r, run + uint32_t(__popcll(m[k] & lt)), lq, o.sink.list, v[k]);
+        run += uint32_t(__popcll(m[k]));
+    }
+}
+
+// ------------------------------------------------- workgroup reservations
+template <class T>
+__device__ __forceinline__ T wave_incl_scan_t(T x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T y = __shfl_up(x, d, kWave);
+        if (lane >= uint32_t(d)) x += y;
+    }
+    return x;
+}
+
+constexpr uint32_t kMaxWavesPerBlock = 16;
+
+// Exclusive prefix of x over the workgroup; the workgroup's total is
+// reserved from *counter with one atomic. Every thread must call it;
+// `scratch` is kMaxWavesPerBlock+1 LDS words of type T.
+template <class T>
+__device__ T block_reserve(T x, T *counter, T *scratch) {
+    const uint32_t lane = lane_id(), w = wave_in_block(), nw = blockDim.x / kWave;
+    const T incl = wave_incl_scan_t<T>(x);
+    if (lane == kWave - 1) scratch[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T run = 0;
+        for (uint32_t i = 0; i < nw; ++i) {
+            const T t = scratch[i];
+            scratch[i] = run;
+            run += t;
+        }
+        scratch[kMaxWavesPerBlock] = run ? atomicAdd(counter, run) : T(0);
+    }
+    __syncthreads();
+    const T r = scratch[kMaxWavesPerBlock] + scratch[w] + incl - x;
+    __syncthreads();  // scratch is reused by the next call
+    return r;
+}
+
+// One-element-per-thread append through block_reserve.
+__device__ __forceinline__ void block_append(bool f, uint32_t v, uint32_t *list, uint32_t *count,
+                                             uint32_t *scratch) {
+    const uint32_t at = block_reserve<uint32_t>(f ? 1u : 0u, count, scratch);
+    if (f) list[at] = v;
+}
+
+// ------------------------------------------------------------ statistics
+__device__ __forceinline__ StatShard *stat_shard(StatShard *s) {
+    const uint32_t w = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
+    return s + (w & (kStatShards - 1));
+}
+
+// Adds one wave-total to a shard field (lane 0; the value must be wave-uniform).
+__device__ __forceinline__ void stat_add(uint64_t *field, uint64_t v) {
+    if (v && lane_id() == 0) atomicAdd((unsigned long long *)field, (unsigned long long)v);
+}
+
 // Largest j with pref[j] <= k (pref is a non-decreasing exclusive scan with
 // pref[0] = 0 <= k); entries of empty rows share their successor's prefix,
 // so j always owns a non-empty row.
@@ -290,12 +479,26 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t *pref, uint32_t k) {
     return j;
 }
 
+// Loads quad `qi` of a row of `deg` slots: the live slot words, with dead
+// slots aliased to slot 0 of the quad; returns the number of live slots.
+__device__ __forceinline__ uint32_t load_quad(const uint32_t *row, uint32_t qi, uint32_t deg,
+                                              uint32_t (&d)[4]) {
+    const uint4 x = *reinterpret_cast<const uint4 *>(row + 4ull * qi);
+    const uint32_t live = min(deg - 4u * qi, 4u);
+    d[0] = x.x;
+    d[1] = live > 1 ? x.y : x.x;
+    d[2] = live > 2 ? x.z : x.x;
+    d[3] = live > 3 ? x.w : x.x;
+    return live;
+}
+
 // Expands the 64 items [base, base+64) of `items` (or the identity range when
-// items == nullptr) that are below `count`.
-template <int EPL, class Op>
+// items == nullptr) that are below `count`. Each lane takes QPL quads of the
+// wave's concatenated rows per step, so a step covers 256*QPL slots.
+template <int QPL, class Op, class Out>
 __device__ void expand_chunk(const Op &op, const ExpandCfg &cfg, const uint32_t *items,
-                             uint32_t base, uint32_t count, WaveLds &L, const Sink &sink,
-                             const TileSink &tsink, Counters &ctr) {
+                             uint32_t base, uint32_t count, WaveLds &L, const Out &out,
+                             Counters &ctr) {
     const uint32_t lane = lane_id();
     const uint32_t i = base + lane;
     uint32_t v = 0, pay = 0, deg = 0;
@@ -303,7 +506,7 @@ __device__ void expand_chunk(const Op &op, const ExpandCfg &cfg, const uint32_t 
     if (i < count) {
         v = items ? items[i] : i;
         Row row;
-        if (op.source(v, i, row, pay)) {
+        if (v != kNone && op.source(v, i, row, pay)) {
             deg = row.deg;
             off = row.off;
         }
@@ -311,15 +514,13 @@ __device__ void expand_chunk(const Op &op, const ExpandCfg &cfg, const uint32_t 
     ctr.scanned += deg;
     ctr.vexp += deg ? 1 : 0;
     if (off & kHostTier) ctr.host += deg;
-    // long rows become tiles; one tile-count atomic per wave
+    // long rows become tiles; one reservation per wave
     const uint32_t nt = deg >= cfg.split ? (deg + cfg.tile_len - 1) / cfg.tile_len : 0u;
     const uint32_t tincl = wave_incl_scan(nt);
     const uint32_t ttotal = __shfl(tincl, kWave - 1, kWave);
-    uint32_t tbase = 0;
-    if (ttotal && lane == 0) tbase = atomicAdd(tsink.count, ttotal);
-    tbase = __shfl(tbase, 0, kWave);
-    if (nt) {
-        const uint32_t t0 = tbase + tincl - nt;
+    if (ttotal) {
+        const Resv r = out_reserve_t(out, ttotal);
+        Tile *lt = out_tlds(out);
         for (uint32_t j = 0; j < nt; ++j) {
             Tile t;
             t.start = off + uint64_t(j) * cfg.tile_len;
@@ -327,114 +528,140 @@ __device__ void expand_chunk(const Op &op, const ExpandCfg &cfg, const uint32_t 
             t.len = min(cfg.tile_len, deg - j * cfg.tile_len);
             t.payload = pay;
             t.pad = 0;
-            tsink.tiles[t0 + j] = t;
+            resv_put(r, tincl - nt + j, lt, out.ts.tiles, t);
         }
-        deg = 0;
+        if (nt) deg = 0;
     }
-    const uint32_t incl = wave_incl_scan(deg);
+    const uint32_t nq = (deg + 3u) >> 2;
+    const uint32_t incl = wave_incl_scan(nq);
     const uint32_t total = __shfl(incl, kWave - 1, kWave);
     if (total == 0) return;
-    L.pref[lane] = incl - deg;
+    L.pref[lane] = incl - nq;
     L.v[lane] = v;
     L.pay[lane] = pay;
+    L.deg[lane] = deg;
     L.offlo[lane] = uint32_t(off);
     L.offhi[lane] = uint32_t(off >> 32);
     wave_lds_sync();
-    for (uint32_t k0 = 0; k0 < total; k0 += kWave * EPL) {
-        uint32_t dst[EPL], own[EPL];
-        uint64_t cur[EPL];
-        bool ok[EPL];
-        // owner searches and loads are unconditional (indices clamped to the
-        // last edge) so that all EPL chains are in flight together
+    for (uint32_t k0 = 0; k0 < total; k0 += kWave * QPL) {
+        uint32_t d[QPL][4], own[QPL], live[QPL];
+        // owner searches and loads are unconditional (quad indices clamped to
+        // the last quad) so that all QPL loads are in flight together
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) {
-            const uint32_t k = k0 + uint32_t(e) * kWave + lane;
-            ok[e] = k < total;
-            const uint32_t kk = ok[e] ? k : total - 1;
-            own[e] = owner_of(L.pref, kk);
-            const uint64_t o = (uint64_t(L.offhi[own[e]]) << 32) | L.offlo[own[e]];
-            dst[e] = cfg.pools.row(o)[kk - L.pref[own[e]]];
+        for (int q = 0; q < QPL; ++q) {
+            const uint32_t k = k0 + uint32_t(q) * kWave + lane;
+            const uint32_t kk = k < total ? k : total - 1;
+            own[q] = owner_of(L.pref, kk);
+            const uint64_t o = (uint64_t(L.offhi[own[q]]) << 32) | L.offlo[own[q]];
+            live[q] = load_quad(cfg.pools.row(o), kk - L.pref[own[q]], L.deg[own[q]], d[q]);
+            if (k >= total) live[q] = 0;
         }
+        uint64_t cur[QPL][4];
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) cur[e] = op.peek(dst[e]);
-        bool push[EPL];
+        for (int q = 0; q < QPL; ++q)
 #pragma unroll
-        for (int e = 0; e < EPL; ++e)
-            push[e] = ok[e] && op.edge(L.v[own[e]], L.pay[own[e]], dst[e], cur[e]);
+            for (int s = 0; s < 4; ++s) cur[q][s] = op.peek(d[q][s]);
+        bool push[QPL * 4];
+        uint32_t dst[QPL * 4];
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) wave_append(push[e], dst[e], sink.list, sink.count);
+        for (int q = 0; q < QPL; ++q) {
+            const uint32_t u = L.v[own[q]], p = L.pay[own[q]];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                push[q * 4 + s] = uint32_t(s) < live[q] && op.edge(u, p, d[q][s], cur[q][s]);
+                dst[q * 4 + s] = d[q][s];
+            }
+        }
+        out_push<QPL * 4>(out, push, dst);
     }
     wave_lds_sync();  // LDS is reused by the next chunk of this wave
 }
 
-template <int EPL, class Op>
-__device__ void expand_tile(const Op &op, const ExpandCfg &cfg, const Tile &t, const Sink &sink) {
+template <int QPL, class Op, class Out>
+__device__ void expand_tile(const Op &op, const ExpandCfg &cfg, const Tile &t, const Out &out) {
     uint32_t pay;
-    if (!op.tile_source(t, pay)) return;
-    const uint32_t *p = cfg.pools.row(t.start);
+    if (t.len == 0 || !op.tile_source(t, pay)) return;
+    const uint32_t *p = cfg.pools.row(t.start);  // t.start is a multiple of 4
     const uint32_t lane = lane_id();
-    for (uint32_t k0 = 0; k0 < t.len; k0 += kWave * EPL) {
-        uint32_t dst[EPL];
-        uint64_t cur[EPL];
-        bool ok[EPL];
+    const uint32_t nq = (t.len + 3u) >> 2;
+    for (uint32_t k0 = 0; k0 < nq; k0 += kWave * QPL) {
+        uint32_t d[QPL][4], live[QPL];
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) {
-            const uint32_t k = k0 + uint32_t(e) * kWave + lane;
-            ok[e] = k < t.len;
-            dst[e] = p[ok[e] ? k : t.len - 1];
+        for (int q = 0; q < QPL; ++q) {
+            const uint32_t k = k0 + uint32_t(q) * kWave + lane;
+            live[q] = load_quad(p, k < nq ? k : nq - 1, t.len, d[q]);
+            if (k >= nq) live[q] = 0;
         }
+        uint64_t cur[QPL][4];
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) cur[e] = op.peek(dst[e]);
-        bool push[EPL];
+        for (int q = 0; q < QPL; ++q)
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) push[e] = ok[e] && op.edge(t.v, pay, dst[e], cur[e]);
+            for (int s = 0; s < 4; ++s) cur[q][s] = op.peek(d[q][s]);
+        bool push[QPL * 4];
+        uint32_t dst[QPL * 4];
 #pragma unroll
-        for (int e = 0; e < EPL; ++e) wave_append(push[e], dst[e], sink.list, sink.count);
+        for (int q = 0; q < QPL; ++q)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                push[q * 4 + s] = uint32_t(s) < live[q] && op.edge(t.v, pay, d[q][s], cur[q][s]);
+                dst[q * 4 + s] = d[q][s];
+            }
+        out_push<QPL * 4>(out, push, dst);
     }
 }
 
-// Grid-strided driver over a list (or identity range) and then a tile list.
-template <int EPL, class Op>
+// Grid-strided driver over a list (or identity range) and then a tile list;
+// any block size that is a multiple of 64.
+// A wave takes `per` consecutive items at a time: 64 when the list is long,
+// fewer (a power of two) when the list is shorter than 64 items per wave of
+// the grid, so that small frontiers still spread over every wave.
+template <int QPL, class Op, class Out>
 __device__ void expand_range(const Op &op, const ExpandCfg &cfg, const uint32_t *items,
-                             uint32_t count, const Sink &sink, const TileSink &tsink,
-                             WaveLds &L, Counters &ctr) {
-    const uint32_t gwave = blockIdx.x * kWavesPerBlock + wave_in_block();
-    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-    for (uint64_t base = uint64_t(gwave) * kWave; base < count; base += uint64_t(nwaves) * kWave)
-        expand_chunk<EPL>(op, cfg, items, uint32_t(base), count, L, sink, tsink, ctr);
+                             uint32_t count, const Out &out, WaveLds &L, Counters &ctr) {
+    const uint32_t wpb = blockDim.x / kWave;
+    const uint32_t gwave = blockIdx.x * wpb + wave_in_block();
+    const uint32_t nwaves = gridDim.x * wpb;
+    uint32_t per = kWave;
+    while (per > 1 && uint64_t(per / 2) * nwaves >= count) per >>= 1;
+    for (uint64_t base = uint64_t(gwave) * per; base < count; base += uint64_t(nwaves) * per)
+        expand_chunk<QPL>(op, cfg, items, uint32_t(base), uint32_t(min(uint64_t(count), base + per)), L,
+                          out, ctr);
 }
 
-template <int EPL, class Op>
+template <int QPL, class Op, class Out>
 __device__ void expand_tiles(const Op &op, const ExpandCfg &cfg, const Tile *tiles,
-                             uint32_t ntiles, const Sink &sink, Counters &ctr) {
-    const uint32_t gwave = blockIdx.x * kWavesPerBlock + wave_in_block();
-    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+                             uint32_t ntiles, const Out &out) {
+    const uint32_t wpb = blockDim.x / kWave;
+    const uint32_t gwave = blockIdx.x * wpb + wave_in_block();
+    const uint32_t nwaves = gridDim.x * wpb;
     for (uint32_t t = gwave; t < ntiles; t += nwaves) {
         const Tile tile = tiles[t];
-        expand_tile<EPL>(op, cfg, tile, sink);
+        expand_tile<QPL>(op, cfg, tile, out);
     }
-    (void)ctr;
 }
 
-// Adds this wave's counters into the control block (one atomic per wave and
-// counter), both to the call totals and to the pass's stage.
-__device__ __forceinline__ void flush_counters(Counters &ctr, Ctl *ctl, uint32_t stage) {
+// Adds this wave's counters to its statistics shard (one atomic per wave and
+// non-zero counter).
+__device__ __forceinline__ void flush_counters(Counters &ctr, StatShard *stats, uint32_t stage) {
     const uint64_t s = wave_sum64(ctr.scanned), h = wave_sum64(ctr.host), x = wave_sum64(ctr.vexp);
-    if (lane_id() == 0) {
-        if (s) {
-            atomicAdd((unsigned long long *)&ctl->edges_scanned, (unsigned long long)s);
-            atomicAdd((unsigned long long *)&ctl->st_edges[stage], (unsigned long long)s);
-        }
-        if (h) {
-            atomicAdd((unsigned long long *)&ctl->host_edges, (unsigned long long)h);
-            atomicAdd((unsigned long long *)&ctl->st_host[stage], (unsigned long long)h);
-        }
-        if (x) {
-            atomicAdd((unsigned long long *)&ctl->vexp, (unsigned long long)x);
-            atomicAdd((unsigned long long *)&ctl->st_vexp[stage], (unsigned long long)x);
-        }
-    }
+    StatShard *sh = stat_shard(stats);
+    stat_add(&sh->edges[stage], s);
+    stat_add(&sh->host[stage], h);
+    stat_add(&sh->vexp[stage], x);
     ctr = Counters{};
+}
+
+// Round progress for the host driver: every round launch posts exactly one
+// word to pinned host memory, (tag << 1) | more, where tag = chain epoch and
+// launch sequence number and `more` says whether the following round has
+// work. The closing workgroup posts it, or workgroup 0 when the round found
+// nothing to do.
+__device__ __forceinline__ void post_progress(uint64_t *prog, uint64_t tag, bool more) {
+    __hip_atomic_store(prog, (tag << 1) | (more ? 1ull : 0ull), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ void post_idle(uint64_t *prog, uint64_t tag) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) post_progress(prog, tag, false);
 }
 
 }  // namespace dev

@@ -82,9 +82,9 @@ struct HBuf {  // pinned host memory (hipHostMalloc; never hipHostRegister)
         p = nullptr;
         count = 0;
     }
-    void alloc(size_t n) {
+    void alloc(size_t n, unsigned flags = hipHostMallocDefault) {
         release();
-        HIPCHK(hipHostMalloc(&p, std::max<size_t>(1, n) * sizeof(T), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc(&p, std::max<size_t>(1, n) * sizeof(T), flags));
         count = n;
     }
     T *device_ptr() const {
@@ -96,7 +96,7 @@ struct HBuf {  // pinned host memory (hipHostMalloc; never hipHostRegister)
 
 constexpr uint32_t kTileLenDefault = 512;
 constexpr uint32_t kSplitDefault = 64;
-constexpr uint32_t kEplDefault = 4;
+constexpr uint32_t kQplDefault = 2;  // quads (4 slots) per lane and step
 
 }  // namespace
 
@@ -106,7 +106,8 @@ struct cig_engine {
     int device = 0;
     int cus = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
+    bool scan_timed = false;  // ev[8]/ev[9] bracket this call's repair scan kernel
 
     uint32_t n = 0;
     bool loaded = false, initialized = false;
@@ -115,16 +116,22 @@ struct cig_engine {
     DBuf<Row> rows;
     DBuf<uint64_t> state;
     DBuf<float> rank, comp, res, delta;
-    DBuf<uint32_t> q0, q1, stamp, aff, alist, hot, hist, score;
+    DBuf<uint32_t> q0, q1, stamp, aff, alist, filt, hot, hist, score;
     DBuf<Tile> t0, t1;
     uint64_t tile_cap = 0;
     DBuf<uint64_t> cnt64, fill64, gofs;
     DBuf<uint32_t> touched, grouped, small_list, big_list, add_list, overflow;
-    DBuf<CopyJob> jobs;
+    DBuf<AddPiece> pieces;              // insert-phase copy pieces
+    DBuf<StatShard> stats;              // sharded statistics (see k_reduce_stats)
+    DBuf<uint32_t> htoff, hkeys, hcnt;  // long-row delete tables
+    DBuf<uint2> hcnts, dtiles;
     uint64_t rec_cap = 0;
     DBuf<cig_edge> rec_dels, rec_adds;
     DBuf<Ctl> ctl;
     HBuf<Ctl> hctl;
+    HBuf<uint64_t> prog;  // round progress word written by the GPU (coherent, mapped)
+    uint64_t *prog_dev = nullptr;
+    uint64_t chain_epoch = 0;
 
     DBuf<uint32_t> hbm_pool;
     HBuf<uint32_t> host_pool;
@@ -139,16 +146,15 @@ struct cig_engine {
     uint64_t add_off = 0, del_off = 0;
 
     // launch shape
-    uint32_t split = kSplitDefault, tile_len = kTileLenDefault, epl = kEplDefault;
+    uint32_t split = kSplitDefault, tile_len = kTileLenDefault, qpl = kQplDefault;
     uint32_t grid_small = 0;   // one-shot kernels
     uint32_t grid_coop = 0;    // persistent cooperative kernels
     uint32_t grid_round = 0;   // chained round kernels
-    uint32_t grid_delbig = 0;
     bool coop = true;
 
     // ------------------------------------------------------------ plumbing
     Pools pools() const { return Pools{hbm_pool.p, host_pool_dev}; }
-    ExpandCfg ecfg(uint32_t stage) const { return ExpandCfg{pools(), split, tile_len, stage}; }
+    ExpandCfg ecfg(uint32_t stage) const { return ExpandCfg{pools(), split, tile_len, stage, stats.p}; }
     uint32_t weight_kind() const {
         switch (cfg.algo) {
         case CIG_ALGO_BFS: return 2;
@@ -167,17 +173,21 @@ struct cig_engine {
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         ctl.alloc(1);
         hctl.alloc(1);
+        prog.alloc(1, hipHostMallocCoherent | hipHostMallocMapped);
+        *prog.p = 0;
+        prog_dev = prog.device_ptr();
+        stats.alloc(kStatShards);
+        HIPCHK(hipMemsetAsync(stats.p, 0, stats.bytes(), stream));
         std::memset(hctl.p, 0, sizeof(Ctl));
         HIPCHK(hipMemcpyAsync(ctl.p, hctl.p, sizeof(Ctl), hipMemcpyHostToDevice, stream));
         if (cfg.split_degree) split = std::max<uint32_t>(64, cfg.split_degree);
-        if (cfg.tile_len) tile_len = std::max<uint32_t>(64, cfg.tile_len);
-        if (cfg.edges_per_lane) epl = cfg.edges_per_lane >= 8 ? 8 : cfg.edges_per_lane >= 4 ? 4
-                                        : cfg.edges_per_lane >= 2 ? 2 : 1;
+        if (cfg.tile_len) tile_len = (std::max<uint32_t>(64, cfg.tile_len) + 3u) & ~3u;  // whole quads
+        // edges per lane come in whole quads: 4, 8 or 16
+        if (cfg.edges_per_lane) qpl = cfg.edges_per_lane >= 16 ? 4 : cfg.edges_per_lane >= 8 ? 2 : 1;
         coop = (cfg.flags & CIG_CFG_COOPERATIVE) != 0;
         if (coop && !prop.cooperativeLaunch) throw HipError(CIG_EDEVICE, "device lacks cooperative launch");
         grid_small = uint32_t(cus) * 8;
         grid_round = uint32_t(cus) * (cfg.round_blocks_per_cu ? std::min<uint32_t>(cfg.round_blocks_per_cu, 16) : 1);
-        grid_delbig = uint32_t(cus) * 2;
     }
 
     void sync() { HIPCHK(hipStreamSynchronize(stream)); }
@@ -284,7 +294,8 @@ struct cig_engine {
         big_list.alloc(n);
         add_list.alloc(n);
         overflow.alloc(n);
-        jobs.alloc(n);
+        htoff.alloc(n);
+        hcnt.alloc(n);
         if (is_pr()) {
             rank.alloc(n);
             comp.alloc(n);
@@ -294,11 +305,13 @@ struct cig_engine {
             state.alloc(n);
             aff.alloc((uint64_t(n) + 31) / 32);
             alist.alloc(n);
+            filt.alloc(kFilterWords);
         }
         ensure_tiles();
         fill32(stamp.p, n, 0);
         fill32(hot.p, n, 0);
         fill32(hist.p, n, 0);
+        fill32(hcnt.p, n, 0);
         fill32(reinterpret_cast<uint32_t *>(cnt64.p), uint64_t(n) * 2, 0);
         fill32(reinterpret_cast<uint32_t *>(fill64.p), uint64_t(n) * 2, 0);
         if (!is_pr()) fill32(aff.p, (uint64_t(n) + 31) / 32, 0);
@@ -364,22 +377,43 @@ struct cig_engine {
     // Enqueues round kernels back to back in growing chunks (8, 16, ... 128)
     // and reads the control block once per chunk; `more` says whether the
     // fixed point still has work. Rounds past the bound raise an error.
-    template <class Launch, class More>
-    uint64_t run_chain(Launch launch, More more, uint64_t bound, const char *what) {
-        uint64_t done = 0;
-        uint32_t k = 8;
+    template <class Launch>
+    uint64_t run_chain(Launch launch, uint64_t bound, const char *what) {
+        // Rounds are launched kLookahead ahead of the last one the GPU has
+        // closed; every launch posts one progress word (see post_progress),
+        // so the host never blocks on the stream and at most kLookahead
+        // no-op rounds follow the fixed point.
+        constexpr uint64_t kLookahead = 3;
+        const uint64_t epoch = ++chain_epoch & 0xFFFFFFFFull;
+        uint64_t issued = 0, seen = 0;
         for (;;) {
-            const uint32_t todo = uint32_t(std::min<uint64_t>(k, bound - done));
-            for (uint32_t i = 0; i < todo; ++i) launch();
+            while (issued < bound && issued < seen + kLookahead) {
+                ++issued;
+                launch((epoch << 31) | issued);
+            }
             HIPCHK(hipGetLastError());
-            done += todo;
-            read_ctl();
-            if (!more(*hctl.p)) return done;
-            if (done >= bound) {
-                if (cfg.max_rounds || is_pr()) return done;  // a configured cap
+            uint64_t w = 0;
+            for (uint32_t spin = 0;; ++spin) {
+                w = __atomic_load_n(prog.p, __ATOMIC_ACQUIRE);
+                if ((w >> 32) == epoch && ((w >> 1) & 0x7FFFFFFFull) > seen) break;
+                if ((spin & 1023) == 1023) {
+                    const hipError_t q = hipStreamQuery(stream);
+                    if (q != hipSuccess && q != hipErrorNotReady)
+                        throw HipError(CIG_EDEVICE, std::string(what) + ": " + hipGetErrorString(q));
+                    if (q == hipSuccess) {  // stream idle: the word must be final now
+                        w = __atomic_load_n(prog.p, __ATOMIC_ACQUIRE);
+                        if ((w >> 32) == epoch && ((w >> 1) & 0x7FFFFFFFull) > seen) break;
+                        throw HipError(CIG_EINTERNAL, std::string(what) + ": round progress lost");
+                    }
+                }
+                __builtin_ia32_pause();
+            }
+            seen = (w >> 1) & 0x7FFFFFFFull;
+            if (!(w & 1ull)) return seen;
+            if (seen >= bound) {
+                if (cfg.max_rounds || is_pr()) return seen;  // a configured cap
                 throw HipError(CIG_EINTERNAL, std::string(what) + " did not converge within the round bound");
             }
-            k = std::min<uint32_t>(k * 2, 128);
         }
     }
 
@@ -387,10 +421,10 @@ struct cig_engine {
         return c.qcount[c.round % 3] || c.tcount[c.round % 3];
     }
 
-    template <int W, bool S, int EPL>
+    template <int W, bool S, int QPL>
     void relax_impl(uint32_t launches_cap) {
         if (coop) {
-            auto kernel = k_relax_loop<W, S, EPL>;
+            auto kernel = k_relax_loop<W, S, QPL>;
             static thread_local uint32_t grid = 0;
             if (!grid) grid = coop_grid_for(kernel);
             grid_coop = grid;
@@ -410,28 +444,27 @@ struct cig_engine {
         } else {
             LoopArgs a = loop_args(1);
             const uint64_t bound = cfg.max_rounds ? cfg.max_rounds : round_bound();
-            run_chain([&] { k_relax_round<W, S, EPL><<<grid_round, kBlock, 0, stream>>>(a); },
-                      relax_pending, bound, "relax fixed point");
+            run_chain([&](uint64_t tag) { k_relax_round<W, S, QPL><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag); },
+                      bound, "relax fixed point");
         }
         HIPCHK(hipGetLastError());
     }
 
-    template <int EPL>
-    void relax_dispatch_epl() {
+    template <int QPL>
+    void relax_dispatch_qpl() {
         switch (weight_kind()) {
-        case 0: relax_impl<0, false, EPL>(1u << 20); break;
-        case 1: relax_impl<1, false, EPL>(1u << 20); break;
-        case 2: relax_impl<2, false, EPL>(1u << 20); break;
-        default: relax_impl<3, true, EPL>(1u << 20); break;
+        case 0: relax_impl<0, false, QPL>(1u << 20); break;
+        case 1: relax_impl<1, false, QPL>(1u << 20); break;
+        case 2: relax_impl<2, false, QPL>(1u << 20); break;
+        default: relax_impl<3, true, QPL>(1u << 20); break;
         }
     }
 
     void relax() {
-        switch (epl) {
-        case 1: relax_dispatch_epl<1>(); break;
-        case 2: relax_dispatch_epl<2>(); break;
-        case 8: relax_dispatch_epl<8>(); break;
-        default: relax_dispatch_epl<4>(); break;
+        switch (qpl) {
+        case 1: relax_dispatch_qpl<1>(); break;
+        case 4: relax_dispatch_qpl<4>(); break;
+        default: relax_dispatch_qpl<2>(); break;
         }
     }
 
@@ -459,24 +492,23 @@ struct cig_engine {
     void pr_converge() {
         PrArgs a = pr_args(cfg.pr_max_rounds);
         if (coop) {
-            auto kernel = k_pr_loop<4>;
+            auto kernel = k_pr_loop<1>;
             static thread_local uint32_t grid = 0;
             if (!grid) grid = coop_grid_for(kernel);
             grid_coop = grid;
             launch_coop(kernel, a, grid);
         } else {
-            run_chain([&] {
+            run_chain([&](uint64_t tag) {
                           k_pr_consume<<<grid_round, kBlock, 0, stream>>>(a);
-                          k_pr_scatter<4><<<grid_round, kBlock, 0, stream>>>(a);
+                          k_pr_scatter<1><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag);
                       },
-                      [](const Ctl &c) { return c.qcount[c.round % 3] != 0; }, cfg.pr_max_rounds,
-                      "PageRank");
+                      cfg.pr_max_rounds, "PageRank");
         }
         HIPCHK(hipGetLastError());
     }
 
     void pr_residual_norm() {
-        k_pr_l1<<<grid_small, 256, 0, stream>>>(res.p, n, ctl.p);
+        k_pr_l1<<<grid_small, 256, 0, stream>>>(res.p, n, stats.p);
         HIPCHK(hipGetLastError());
     }
 
@@ -503,6 +535,7 @@ struct cig_engine {
     // ------------------------------------------------------------ stats
     void begin_call(cig_batch_stats *st) {
         if (st) std::memset(st, 0, sizeof(*st));
+        scan_timed = false;
         reset_ctl(true);
         HIPCHK(hipEventRecord(ev[0], stream));
     }
@@ -513,8 +546,14 @@ struct cig_engine {
         return ms;
     }
 
+    void reduce_stats() {
+        k_reduce_stats<<<1, kWave, 0, stream>>>(stats.p, ctl.p);
+        HIPCHK(hipGetLastError());
+    }
+
     void end_call(cig_batch_stats *st, int marks) {
         HIPCHK(hipEventRecord(ev[7], stream));
+        reduce_stats();
         read_ctl();
         if (hctl.p->pr_invalid) throw HipError(CIG_EINTERNAL, "PR non-finite rank/residual");
         if (!is_pr() && !cfg.max_rounds) {
@@ -532,6 +571,7 @@ struct cig_engine {
             st->seed_ms = elapsed(3, 5);
             st->propagate_ms = elapsed(5, 4);
         }
+        if (scan_timed) st->scan_kernel_ms = elapsed(8, 9);
         for (int i = 0; i < 4; ++i) {
             st->stage_edges[i] = c.st_edges[i];
             st->stage_vertices[i] = c.st_vexp[i];
@@ -572,7 +612,13 @@ struct cig_engine {
         m.big_list = big_list.p;
         m.add_list = add_list.p;
         m.overflow = overflow.p;
-        m.jobs = jobs.p;
+        m.pieces = pieces.p;
+        m.stats = stats.p;
+        m.htoff = htoff.p;
+        m.hkeys = hkeys.p;
+        m.hcnts = hcnts.p;
+        m.hcnt = hcnt.p;
+        m.dtiles = dtiles.p;
         return m;
     }
 
@@ -585,9 +631,10 @@ struct cig_engine {
 
     void mutate_apply(const MutArgs &m) {
         k_del_rows<<<grid_small, kBlock, 0, stream>>>(m);
-        k_del_big<<<grid_delbig, kDelBigThreads, 0, stream>>>(m);
-        k_add<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
-        k_copy_jobs<<<grid_small, 256, 0, stream>>>(m);
+        k_del_scan<<<grid_small, 256, 0, stream>>>(m);
+        k_del_fix<<<grid_small, 256, 0, stream>>>(m);
+        k_add_plan<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
+        k_add_copy<<<grid_small, kBlock, 0, stream>>>(m);
         HIPCHK(hipGetLastError());
     }
 
@@ -624,8 +671,10 @@ struct cig_engine {
         HIPCHK(hipMemcpyAsync(&ctl.p->overflow_cnt, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(&ctl.p->copy_cnt, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         MutArgs mm = mut_args(m.dels, m.nd, m.adds, m.na);  // pools/rows may have moved
-        k_add<<<grid_small, 256, 0, stream>>>(mm, add_list.p, &ctl.p->add_cnt);
-        k_copy_jobs<<<grid_small, 256, 0, stream>>>(mm);
+        ensure_pieces(m.nd, m.na);
+        mm.pieces = pieces.p;
+        k_add_plan<<<grid_small, 256, 0, stream>>>(mm, add_list.p, &ctl.p->add_cnt);
+        k_add_copy<<<grid_small, kBlock, 0, stream>>>(mm);
         HIPCHK(hipGetLastError());
         read_ctl();
         if (hctl.p->overflow_cnt) throw HipError(CIG_ENOMEM, "edge pool exhausted after compaction");
@@ -636,6 +685,7 @@ struct cig_engine {
     // allocation with room for `extra` more slots. HBM is bounded by the
     // cache budget: rows that do not fit are demoted to the host tier.
     void compact_pool(int tier, uint64_t extra) {
+        reduce_stats();  // garbage counted so far lands in the control block first
         sync();
         std::vector<Row> hrows(n);
         HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
@@ -773,6 +823,7 @@ struct cig_engine {
         x.stamp = stamp.p;
         x.aff = aff.p;
         x.alist = alist.p;
+        x.filt = filt.p;
         x.q0 = q0.p;
         x.q1 = q1.p;
         x.t0 = t0.p;
@@ -789,23 +840,28 @@ struct cig_engine {
         return x;
     }
 
-    template <int W, bool S, int EPL>
-    void repair_scan_epl(const InvalArgs &ia) {
+    template <int W, bool S, int QPL>
+    void repair_scan_qpl(const InvalArgs &ia) {
         InvalArgs a = ia;
         a.cfg = ecfg(kStageRepair);
-        const uint32_t grid = uint32_t(cus) * 4;  // 4 workgroups of 37 KB LDS per CU
-        k_repair_scan<W, S, EPL><<<grid, kBlock, 0, stream>>>(a);
-        k_repair_tiles<W, S, EPL><<<grid, kBlock, 0, stream>>>(a);
+        fill32(filt.p, kFilterWords, 0);
+        k_build_filter<<<grid_small, 256, 0, stream>>>(a);
+        // 32 KB filter + 8 wave slots of LDS per workgroup: 3 workgroups per CU
+        const uint32_t grid = uint32_t(cus) * 3;
+        HIPCHK(hipEventRecord(ev[8], stream));
+        k_repair_scan<W, S, QPL><<<grid, kRepairBlock, 0, stream>>>(a);
+        HIPCHK(hipEventRecord(ev[9], stream));
+        scan_timed = true;
+        k_repair_tiles<W, S, QPL><<<grid, kRepairBlock, 0, stream>>>(a);
         HIPCHK(hipGetLastError());
     }
 
     template <int W, bool S>
     void repair_scan(const InvalArgs &ia) {
-        switch (epl) {
-        case 1: repair_scan_epl<W, S, 1>(ia); break;
-        case 2: repair_scan_epl<W, S, 2>(ia); break;
-        case 8: repair_scan_epl<W, S, 8>(ia); break;
-        default: repair_scan_epl<W, S, 4>(ia); break;
+        switch (qpl) {
+        case 1: repair_scan_qpl<W, S, 1>(ia); break;
+        case 4: repair_scan_qpl<W, S, 4>(ia); break;
+        default: repair_scan_qpl<W, S, 2>(ia); break;
         }
     }
 
@@ -820,14 +876,13 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
         InvalArgs a = ia;
         if (coop) {
-            auto kernel = k_inval_loop<4>;
+            auto kernel = k_inval_loop<1>;
             static thread_local uint32_t grid = 0;
             if (!grid) grid = coop_grid_for(kernel);
             launch_coop(kernel, a, grid);
         } else {
             k_inval_begin<<<1, 64, 0, stream>>>(ctl.p);
-            run_chain([&] { k_inval_round<4><<<grid_round, kBlock, 0, stream>>>(a); },
-                      [](const Ctl &c) { return c.ibegin != c.iend || c.icount[c.iround % 3] != 0; },
+            run_chain([&](uint64_t tag) { k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag); },
                       round_bound(), "invalidation");
         }
         k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
@@ -922,10 +977,9 @@ struct cig_engine {
         TileSink ts{t0.p, &ctl.p->scount};
         uint32_t zero = 0;
         HIPCHK(hipMemcpyAsync(&ctl.p->scount, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-        k_expand_once<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), touched.p, &ctl.p->touched,
-                                                            0, sink, ts, ctl.p, nullptr);
-        k_expand_tiles<4><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), t0.p, &ctl.p->scount, sink,
-                                                             ctl.p);
+        k_expand_once<1><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), touched.p, &ctl.p->touched,
+                                                            0, sink, ts, nullptr);
+        k_expand_tiles<1><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), t0.p, &ctl.p->scount, sink);
         HIPCHK(hipGetLastError());
     }
 
@@ -978,9 +1032,33 @@ struct cig_engine {
         apply_device(rec_dels.p, uint32_t(nd), rec_adds.p, uint32_t(na), flags, st);
     }
 
+    // Delete tables hold at most sum(max(16, 4*nd_u)) <= 20*nd slots; scan
+    // tiles are at most (live slots)/kDelTile + (rows with deletes).
+    void ensure_delete_space(uint64_t nd) {
+        const uint64_t tab = 20 * nd + 64;
+        if (hkeys.count < tab) {
+            hkeys.alloc(tab);
+            hcnts.alloc(tab);
+            fill32(hkeys.p, tab, 0);
+            fill32(reinterpret_cast<uint32_t *>(hcnts.p), 2 * tab, 0);
+        }
+        const uint64_t tiles = (hbm_slots + host_slots) / kDelTile + nd + 64;
+        if (dtiles.count < tiles) dtiles.alloc(tiles);
+        if (tab > 0xFFFFFFFFull) throw HipError(CIG_ELIMIT, "batch too large for the delete tables");
+    }
+
+    // Insert pieces: a relocated row gives ceil(deg/kPieceLen) copy pieces and
+    // every touched source ceil(adds/kPieceLen) append pieces.
+    void ensure_pieces(uint64_t nd, uint64_t na) {
+        const uint64_t need = (hbm_slots + host_slots) / kPieceLen + 2 * (nd + na) + na / kPieceLen + 64;
+        if (pieces.count < need) pieces.alloc(need);
+    }
+
     void apply_device(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, uint32_t flags,
                       cig_batch_stats *st) {
         if (grouped.count < uint64_t(nd) + na) grouped.alloc(uint64_t(nd) + na);
+        ensure_delete_space(nd);
+        ensure_pieces(nd, na);
         if (is_pr()) pr_batch(d, nd, a, na, st);
         else traversal_call(d, nd, a, na, (flags & CIG_BATCH_STAGED) != 0, st);
         if (st) st->pool_compactions = compactions;
@@ -1142,7 +1220,7 @@ struct cig_engine {
         gi->n = n;
         gi->algo = cfg.algo;
         gi->block_size = kBlock;
-        gi->edges_per_lane = epl;
+        gi->edges_per_lane = 4 * qpl;
         gi->split_degree = split;
         gi->cu_count = uint32_t(cus);
         gi->grid_blocks = grid_coop;
@@ -1165,8 +1243,10 @@ struct cig_engine {
         gi->host_pinned_bytes = host_slots * 4 + sizeof(Ctl);
         gi->device_bytes = hbm_slots * 4 + rows.bytes() + state.bytes() + rank.bytes() * 4 + q0.bytes() * 2 +
                            stamp.bytes() + aff.bytes() + alist.bytes() + hot.bytes() * 3 + t0.bytes() * 2 +
-                           cnt64.bytes() * 3 + touched.bytes() * 5 + jobs.bytes() + grouped.bytes() +
-                           rec_dels.bytes() * 2 + s_adds.bytes() + s_dels.bytes();
+                           cnt64.bytes() * 3 + touched.bytes() * 5 + pieces.bytes() + grouped.bytes() +
+                           rec_dels.bytes() * 2 + s_adds.bytes() + s_dels.bytes() + filt.bytes() +
+                           htoff.bytes() + hcnt.bytes() + hkeys.bytes() + hcnts.bytes() + dtiles.bytes() +
+                           stats.bytes();
     }
 
     ~cig_engine() {

@@ -36,6 +36,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
     c->scount = 0;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
+    c->dtile_cnt = c->ht_cursor = 0;
     c->pr_invalid = 0;
     c->group_cursor = 0;
     c->edges_scanned = c->host_edges = c->vexp = 0;
@@ -55,6 +56,7 @@ __global__ void k_reset_mut(Ctl *c) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
+    c->dtile_cnt = c->ht_cursor = 0;
     c->group_cursor = 0;
 }
 
@@ -112,13 +114,41 @@ struct MutArgs {
     uint32_t *big_list;
     uint32_t *add_list;
     uint32_t *overflow;
-    CopyJob *jobs;
+    // long-row deletes
+    uint32_t *htoff;   // per vertex: offset of its delete table (kNone: none)
+    uint32_t *hkeys;   // delete tables: destination + 1 (0 = empty)
+    uint2 *hcnts;      // delete tables: (requested, claimed)
+    uint32_t *hcnt;    // per vertex: holes found (zero between calls)
+    uint2 *dtiles;     // (row, first slot) of each delete scan tile
+    struct AddPiece *pieces;  // copy pieces of the insert phase (AddPiece below)
+    StatShard *stats;
 };
 
+// Rows with more than 64 deletes, or longer than 4096 slots, are deleted
+// from in tiles of kDelTile slots spread over the whole grid. Their delete
+// multisets live in open-addressing tables of 2^bits >= 2*nd slots.
+constexpr uint32_t kDelTile = 4096;
+constexpr uint32_t kDelLdsKeys = 8192;
+
+__device__ __forceinline__ bool del_is_small(uint32_t nd, uint32_t deg) { return nd <= 64 && deg <= 4096; }
+
+__device__ __forceinline__ uint32_t del_table_bits(uint32_t nd) {
+    uint32_t bits = 4;
+    while ((1ull << bits) < 2ull * nd) ++bits;
+    return bits;
+}
+
+__device__ __forceinline__ uint32_t ht_hash(uint32_t x, uint32_t bits) {
+    return (x * 0x9E3779B1u) >> (32 - bits);
+}
+
 // GroupedUpdateBatch (effective_update_batch.h:35-87) on the device: count
-// records per source and collect the touched sources.
+// records per source and collect the touched sources (one list reservation
+// per workgroup step).
 __global__ void k_count(MutArgs a) {
+    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     const uint32_t total = a.nd + a.na;
+    uint64_t bad_total = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gthreads()) {
         const uint32_t i = base + threadIdx.x;
         bool first = false;
@@ -136,46 +166,52 @@ __global__ void k_count(MutArgs a) {
                 first = old == 0;
             }
         }
-        const uint64_t badm = __ballot(bad);
-        if (badm && lane_id() == 0)
-            atomicAdd((unsigned long long *)&a.ctl->invalid, (unsigned long long)__popcll(badm));
-        wave_append(first, u, a.touched, &a.ctl->touched);
+        bad_total += uint32_t(__popcll(__ballot(bad)));
+        block_append(first, u, a.touched, &a.ctl->touched, scratch);
     }
+    stat_add(&stat_shard(a.stats)->invalid, bad_total);
 }
 
-// Reserves each touched source's group and classifies its work.
+// Reserves each touched source's group and classifies its work; long-row
+// deletes also reserve their table and scan tiles. Every cursor and list
+// takes one atomic per workgroup step.
 __global__ void k_plan(MutArgs a) {
+    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
+    __shared__ unsigned long long scratch64[kMaxWavesPerBlock + 1];
     const uint32_t touched = ld_acquire(&a.ctl->touched);
     for (uint32_t base = blockIdx.x * blockDim.x; base < touched; base += gthreads()) {
         const uint32_t t = base + threadIdx.x;
         bool small = false, big = false, add = false;
-        uint32_t u = 0, len = 0, nd = 0, na = 0;
+        uint32_t u = 0, len = 0, nd = 0, na = 0, deg = 0;
         if (t < touched) {
             u = a.touched[t];
             const uint64_t c = a.cnt64[u];
             nd = uint32_t(c);
             na = uint32_t(c >> 32);
             len = nd + na;
-            // one wave scans a row of up to 4096 slots; longer rows (or more than
-            // 64 deletes) get a whole workgroup
-            small = nd != 0 && nd <= 64 && a.rows[u].deg <= 4096;
+            deg = a.rows[u].deg;
+            // one wave scans a row of up to 4096 slots for up to 64 deletes;
+            // anything larger is cut into grid-wide tiles
+            small = nd != 0 && del_is_small(nd, deg);
             big = nd != 0 && !small;
             add = na != 0;
         }
-        // one group-cursor atomic per wave
-        const uint32_t incl = wave_incl_scan(len);
-        const uint32_t total_len = __shfl(incl, kWave - 1, kWave);
-        unsigned long long gbase = 0;
-        if (lane_id() == 0 && total_len)
-            gbase = atomicAdd((unsigned long long *)&a.ctl->group_cursor, (unsigned long long)total_len);
-        gbase = __shfl(gbase, 0, kWave);
-        if (t < touched) a.gofs[u] = gbase + (incl - len);
-        wave_append(small, u, a.small_list, &a.ctl->small_cnt);
-        wave_append(big, u, a.big_list, &a.ctl->big_cnt);
-        wave_append(add, u, a.add_list, &a.ctl->add_cnt);
+        const uint64_t g = block_reserve<unsigned long long>(len, (unsigned long long *)&a.ctl->group_cursor, scratch64);
+        if (t < touched) a.gofs[u] = g;
+        const uint32_t tsz = big ? (1u << del_table_bits(nd)) : 0u;
+        const uint32_t ntl = big ? (deg + kDelTile - 1) / kDelTile : 0u;
+        const uint32_t hb = block_reserve<uint32_t>(tsz, &a.ctl->ht_cursor, scratch);
+        const uint32_t tb = block_reserve<uint32_t>(ntl, &a.ctl->dtile_cnt, scratch);
+        if (t < touched) a.htoff[u] = big ? hb : kNone;
+        for (uint32_t j = 0; j < ntl; ++j) a.dtiles[tb + j] = make_uint2(u, j * kDelTile);
+        block_append(small, u, a.small_list, &a.ctl->small_cnt, scratch);
+        block_append(big, u, a.big_list, &a.ctl->big_cnt, scratch);
+        block_append(add, u, a.add_list, &a.ctl->add_cnt, scratch);
     }
 }
 
+// Groups the records by source; deletes of long rows also enter their
+// row's table (destination -> requested count).
 __global__ void k_scatter(MutArgs a) {
     const uint32_t total = a.nd + a.na;
     for (uint32_t i = gthread(); i < total; i += gthreads()) {
@@ -184,8 +220,21 @@ __global__ void k_scatter(MutArgs a) {
         if (e.u >= a.n || e.v >= a.n) continue;
         const uint64_t pos = atomicAdd((unsigned long long *)&a.fill64[e.u],
                                        del ? 1ull : (1ull << 32));
-        const uint32_t slot = del ? uint32_t(pos) : uint32_t(a.cnt64[e.u]) + uint32_t(pos >> 32);
+        const uint32_t nd = uint32_t(a.cnt64[e.u]);
+        const uint32_t slot = del ? uint32_t(pos) : nd + uint32_t(pos >> 32);
         a.grouped[a.gofs[e.u] + slot] = e.v;
+        const uint32_t to = del ? a.htoff[e.u] : kNone;
+        if (to != kNone) {
+            // at most nd distinct keys in 2^bits >= 2*nd slots: the probe ends
+            const uint32_t bits = del_table_bits(nd), mask = (1u << bits) - 1u;
+            uint32_t h = ht_hash(e.v, bits);
+            for (;;) {
+                const uint32_t prev = atomicCAS(&a.hkeys[to + h], 0u, e.v + 1u);
+                if (prev == 0u || prev == e.v + 1u) break;
+                h = (h + 1u) & mask;
+            }
+            atomicAdd(&a.hcnts[to + h].x, 1u);
+        }
     }
 }
 
@@ -263,205 +312,270 @@ __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
         acc_applied += k;
         acc_missing += nd - k;
     }
-    if (lane == 0) {
-        if (acc_applied) atomicAdd((unsigned long long *)&a.ctl->deletes_applied, (unsigned long long)acc_applied);
-        if (acc_missing) atomicAdd((unsigned long long *)&a.ctl->missing, (unsigned long long)acc_missing);
+    StatShard *sh = stat_shard(a.stats);
+    stat_add(&sh->deletes_applied, acc_applied);
+    stat_add(&sh->missing, acc_missing);
+}
+
+// Exclusive scan of a 0/1 flag over a 256-thread workgroup; *total gets the
+// sum. Contains barriers: every thread of the workgroup must call it.
+__device__ __forceinline__ uint32_t block_flag_scan(bool f, uint32_t *wsum, uint32_t &total) {
+    const uint64_t m = __ballot(f);
+    const uint32_t w = wave_in_block();
+    if (lane_id() == 0) wsum[w] = uint32_t(__popcll(m));
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (uint32_t i = 0; i < blockDim.x / kWave; ++i) {
+        if (i < w) before += wsum[i];
+        total += wsum[i];
+    }
+    __syncthreads();
+    return before + uint32_t(__popcll(m & __lanemask_lt()));
+}
+
+// Long-row deletes, scan phase: one workgroup per kDelTile-slot tile. A slot
+// whose destination is in the row's table claims one requested occurrence
+// (first claims win; equal destinations are interchangeable, so the survivor
+// multiset is the reference's, BuildDeletionPlan,
+// source_local_chunk_store.h:670-723); a claimed slot is overwritten with
+// kNone and its position is recorded in the row's group (whose delete
+// records are no longer needed once the table is built).
+__global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
+    __shared__ uint32_t keys[kDelLdsKeys];
+    const uint32_t nt = ld_acquire(&a.ctl->dtile_cnt);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const uint2 dt = a.dtiles[t];
+        const uint32_t u = dt.x, start = dt.y;
+        const Row r = a.rows[u];
+        const uint32_t nd = uint32_t(a.cnt64[u]);
+        const uint32_t bits = del_table_bits(nd), mask = (1u << bits) - 1u;
+        const uint32_t to = a.htoff[u];
+        const bool in_lds = (1u << bits) <= kDelLdsKeys;
+        __syncthreads();  // the previous tile is done with `keys`
+        if (in_lds)
+            for (uint32_t k = tid; k <= mask; k += blockDim.x) keys[k] = a.hkeys[to + k];
+        __syncthreads();
+        const uint32_t *kt = in_lds ? keys : a.hkeys + to;
+        uint32_t *row = a.pools.row(r.off) + start;  // start is a multiple of 4
+        const uint32_t len = min(kDelTile, r.deg - start);
+        const uint32_t nq = (len + 3u) >> 2;
+        uint32_t *holes = a.grouped + a.gofs[u];
+        for (uint32_t q0 = 0; q0 < nq; q0 += blockDim.x * 4) {
+            uint32_t d[4][4], live[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t q = q0 + uint32_t(i) * blockDim.x + tid;
+                live[i] = q < nq ? load_quad(row, q, len, d[i]) : 0u;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t q = q0 + uint32_t(i) * blockDim.x + tid;
+                for (uint32_t sl = 0; sl < live[i]; ++sl) {
+                    const uint32_t x = d[i][sl];
+                    uint32_t h = ht_hash(x, bits);
+                    for (uint32_t probe = 0; probe <= mask; ++probe) {
+                        const uint32_t key = kt[h];
+                        if (key == 0u) break;
+                        if (key == x + 1u) {
+                            if (atomicAdd(&a.hcnts[to + h].y, 1u) < a.hcnts[to + h].x) {
+                                row[4u * q + sl] = kNone;
+                                holes[atomicAdd(&a.hcnt[u], 1u)] = start + 4u * q + sl;
+                            }
+                            break;
+                        }
+                        h = (h + 1u) & mask;
+                    }
+                }
+            }
+        }
     }
 }
 
-// Deletes for long rows or many deletes, one 1024-thread workgroup per row:
-// the delete multiset goes to an LDS hash table (up to kPass deletes per
-// pass); the row is scanned in parallel, each match claims one requested
-// occurrence; holes below the new length are refilled from the live tail.
-// The multiset of survivors equals the reference's; their order may differ.
-constexpr int kDelBigThreads = 1024;
-constexpr uint32_t kDelHT = 4096;
-constexpr uint32_t kDelPass = 2048;
-
-__device__ __forceinline__ uint32_t ht_hash(uint32_t x, uint32_t bits) {
-    return (x * 0x9E3779B1u) >> (32 - bits);
-}
-
-__global__ __launch_bounds__(kDelBigThreads) void k_del_big(MutArgs a) {
-    __shared__ uint32_t hkey[kDelHT];
-    __shared__ int32_t hcnt[kDelHT];
-    __shared__ uint32_t holes[kDelPass];
-    __shared__ uint32_t fh[kDelPass];
-    __shared__ uint32_t dead[kDelPass / 32];
-    __shared__ uint32_t s_nh, s_fc, s_lc;
+// Long-row deletes, fix-up phase: one workgroup per row. With k holes the
+// row shrinks to deg-k; the live slots of the last k positions move into the
+// holes below deg-k (O(k) stores; survivors keep their multiset, not their
+// order, unlike CompactSourceInPlace, source_local_chunk_store.h:733-780).
+// The row's table and hole counter are cleared for the next call.
+__global__ __launch_bounds__(256) void k_del_fix(MutArgs a) {
+    __shared__ uint32_t wsum[256 / kWave];
     const uint32_t cnt = ld_acquire(&a.ctl->big_cnt);
     const uint32_t tid = threadIdx.x;
+    uint64_t applied = 0, missing = 0;
     for (uint32_t j = blockIdx.x; j < cnt; j += gridDim.x) {
         const uint32_t u = a.big_list[j];
         const Row r = a.rows[u];
         const uint32_t nd = uint32_t(a.cnt64[u]);
-        const uint32_t *dl = a.grouped + a.gofs[u];
+        const uint32_t k = a.hcnt[u];
+        uint32_t *hl = a.grouped + a.gofs[u];
         uint32_t *row = a.pools.row(r.off);
-        uint32_t deg = r.deg, total = 0;
-        for (uint32_t p0 = 0; p0 < nd; p0 += kDelPass) {
-            const uint32_t pn = min(kDelPass, nd - p0);
-            // table of 2*pn slots rounded up to a power of two (64..4096)
-            uint32_t bits = 6;
-            while ((1u << bits) < 2 * pn) ++bits;
-            const uint32_t ht = 1u << bits;
-            for (uint32_t s = tid; s < ht; s += kDelBigThreads) { hkey[s] = kNone; hcnt[s] = 0; }
-            for (uint32_t s = tid; s < kDelPass / 32; s += kDelBigThreads) dead[s] = 0;
-            if (tid == 0) { s_nh = 0; s_fc = 0; s_lc = 0; }
-            __syncthreads();
-            for (uint32_t q = tid; q < pn; q += kDelBigThreads) {
-                const uint32_t d = dl[p0 + q];
-                uint32_t h = ht_hash(d, bits);
-                for (;;) {
-                    const uint32_t prev = atomicCAS(&hkey[h], kNone, d);
-                    if (prev == kNone || prev == d) { atomicAdd(&hcnt[h], 1); break; }
-                    h = (h + 1) & (ht - 1);
-                }
-            }
-            __syncthreads();
-            for (uint32_t b = 0; b < deg; b += kDelBigThreads * 4) {
-                uint32_t x[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t pos = b + uint32_t(k) * kDelBigThreads + tid;
-                    x[k] = pos < deg ? row[pos] : kNone;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t pos = b + uint32_t(k) * kDelBigThreads + tid;
-                    if (pos >= deg) continue;
-                    uint32_t h = ht_hash(x[k], bits);
-                    while (hkey[h] != kNone && hkey[h] != x[k]) h = (h + 1) & (ht - 1);
-                    if (hkey[h] == x[k] && atomicSub(&hcnt[h], 1) > 0)
-                        holes[atomicAdd(&s_nh, 1u)] = pos;
-                }
-            }
-            __syncthreads();
-            const uint32_t k = s_nh;
-            const uint32_t nd2 = deg - k;
-            for (uint32_t q = tid; q < k; q += kDelBigThreads) {
-                const uint32_t p = holes[q];
-                if (p < nd2) fh[atomicAdd(&s_fc, 1u)] = p;
-                else atomicOr(&dead[(p - nd2) >> 5], 1u << ((p - nd2) & 31));
-            }
-            __syncthreads();
-            for (uint32_t q = tid; q < k; q += kDelBigThreads) {
-                if ((dead[q >> 5] >> (q & 31)) & 1u) continue;
-                const uint32_t val = row[nd2 + q];
-                row[fh[atomicAdd(&s_lc, 1u)]] = val;
-            }
-            __threadfence();
-            __syncthreads();
-            deg = nd2;
-            total += k;
-        }
-        if (tid == 0) {
-            a.rows[u].deg = deg;
-            if (total) atomicAdd((unsigned long long *)&a.ctl->deletes_applied, (unsigned long long)total);
-            if (nd - total) atomicAdd((unsigned long long *)&a.ctl->missing, (unsigned long long)(nd - total));
+        const uint32_t newdeg = r.deg - k;
+        // 1. compact the holes below newdeg to the front of the hole list
+        //    (write index <= read index, and the scan's barrier orders them)
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < k; b += blockDim.x) {
+            const uint32_t i = b + tid;
+            const uint32_t h = i < k ? hl[i] : 0u;
+            const bool front = i < k && h < newdeg;
+            uint32_t tot;
+            const uint32_t rk = block_flag_scan(front, wsum, tot);
+            if (front) hl[run + rk] = h;
+            run += tot;
         }
         __syncthreads();
+        // 2. live slots of [newdeg, deg) fill those holes in order
+        run = 0;
+        for (uint32_t b = 0; b < k; b += blockDim.x) {
+            const uint32_t i = b + tid;
+            const uint32_t v = i < k ? row[newdeg + i] : kNone;
+            const bool live = v != kNone;
+            uint32_t tot;
+            const uint32_t rk = block_flag_scan(live, wsum, tot);
+            if (live) row[hl[run + rk]] = v;
+            run += tot;
+        }
+        // 3. clear the table for the next call
+        const uint32_t to = a.htoff[u], tsz = 1u << del_table_bits(nd);
+        for (uint32_t s = tid; s < tsz; s += blockDim.x) {
+            a.hkeys[to + s] = 0u;
+            a.hcnts[to + s] = make_uint2(0u, 0u);
+        }
+        if (tid == 0) {
+            a.rows[u].deg = newdeg;
+            a.hcnt[u] = 0u;
+        }
+        applied += k;
+        missing += nd - k;
+        __syncthreads();
+    }
+    if (wave_in_block() == 0) {
+        StatShard *sh = stat_shard(a.stats);
+        stat_add(&sh->deletes_applied, applied);
+        stat_add(&sh->missing, missing);
     }
 }
 
-// Appends the adds of each touched source (one wave per row); a full row
-// moves to a block of twice the capacity at the tail of its tier
-// (PatchOrInvalidateCachedAdjacency's grow path, csr_graph.cuh:1793-1821, and
-// RewriteSourceToNewBlock, source_local_chunk_store.h:782-848). A row that
-// cannot be placed is listed in `overflow` and left for the host to retry.
-__global__ void k_add(MutArgs a, const uint32_t *list, const uint32_t *list_count) {
+// Insert phase (PatchOrInvalidateCachedAdjacency's grow path,
+// csr_graph.cuh:1793-1821, and RewriteSourceToNewBlock,
+// source_local_chunk_store.h:782-848): each touched source appends its adds;
+// a row that outgrows its block moves to a block of the next power of two at
+// the tail of its tier. k_add_plan decides per row (one thread each),
+// reserves pool blocks and copy pieces with one atomic per workgroup and
+// tier, and rewrites the descriptors; k_add_copy then moves every piece with
+// whole waves. A row that cannot be placed is listed in `overflow` and left
+// for the host to retry after compaction.
+struct AddPiece {
+    const uint32_t *src;
+    uint32_t *dst;
+    uint32_t len;
+    uint32_t pad;
+};
+constexpr uint32_t kPieceLen = 1024;
+
+__device__ __forceinline__ uint32_t pieces_of(uint64_t len) { return uint32_t((len + kPieceLen - 1) / kPieceLen); }
+
+__global__ void k_add_plan(MutArgs a, const uint32_t *list, const uint32_t *list_count) {
+    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
+    __shared__ unsigned long long scratch64[kMaxWavesPerBlock + 1];
     const uint32_t cnt = ld_acquire(list_count);
-    const uint32_t lane = lane_id();
     uint64_t acc_adds = 0, acc_reloc = 0, acc_garbage[2] = {0, 0};
-    for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
-        const uint32_t u = list[j];
-        const uint64_t c = a.cnt64[u];
-        const uint32_t nd = uint32_t(c), na = uint32_t(c >> 32);
-        const uint32_t *al = a.grouped + a.gofs[u] + nd;
-        const Row r = a.rows[u];
-        const uint64_t newdeg = uint64_t(r.deg) + na;
-        if (newdeg <= r.cap) {
-            uint32_t *row = a.pools.row(r.off);
-            for (uint32_t q = lane; q < na; q += kWave) row[r.deg + q] = al[q];
-            if (lane == 0) a.rows[u].deg = uint32_t(newdeg);
-        } else {
-            const uint32_t tier = (r.off & kHostTier) ? 1u : 0u;
-            const uint64_t newcap = next_pow2_dev(newdeg);
-            unsigned long long o = 0;
-            if (lane == 0) o = atomicAdd((unsigned long long *)&a.ctl->pool_tail[tier], (unsigned long long)newcap);
-            o = __shfl(o, 0, kWave);
-            if (o + newcap > a.ctl->pool_cap[tier] || newcap > 0xFFFFFFFFull) {
-                if (lane == 0) a.overflow[atomicAdd(&a.ctl->overflow_cnt, 1u)] = u;
-                continue;
-            }
-            const uint64_t noff = uint64_t(o) | (tier ? kHostTier : 0ull);
-            const uint32_t *src = a.pools.row(r.off);
-            uint32_t *dst = a.pools.row(noff);
-            if (r.deg <= 1024) {
-                constexpr int U = 8;
-                for (uint32_t b = 0; b < r.deg; b += kWave * U) {
-                    uint32_t v[U];
-#pragma unroll
-                    for (int q = 0; q < U; ++q) {
-                        const uint32_t p = b + uint32_t(q) * kWave + lane;
-                        v[q] = p < r.deg ? src[p] : 0u;
-                    }
-#pragma unroll
-                    for (int q = 0; q < U; ++q) {
-                        const uint32_t p = b + uint32_t(q) * kWave + lane;
-                        if (p < r.deg) dst[p] = v[q];
-                    }
-                }
-            } else if (lane == 0) {
-                CopyJob jb;
-                jb.from = r.off;
-                jb.to = noff;
-                jb.len = r.deg;
-                jb.pad = 0;
-                a.jobs[atomicAdd(&a.ctl->copy_cnt, 1u)] = jb;
-            }
-            for (uint32_t q = lane; q < na; q += kWave) dst[r.deg + q] = al[q];
-            if (lane == 0) {
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += gthreads()) {
+        const uint32_t j = base + threadIdx.x;
+        const bool on = j < cnt;
+        uint32_t u = 0, nd = 0, na = 0, tier = 0;
+        Row r{};
+        uint64_t newdeg = 0, newcap = 0;
+        bool grow = false;
+        if (on) {
+            u = list[j];
+            const uint64_t c = a.cnt64[u];
+            nd = uint32_t(c);
+            na = uint32_t(c >> 32);
+            r = a.rows[u];
+            newdeg = uint64_t(r.deg) + na;
+            tier = (r.off & kHostTier) ? 1u : 0u;
+            grow = newdeg > r.cap;
+            if (grow) newcap = next_pow2_dev(newdeg);
+        }
+        const uint64_t o0 = block_reserve<unsigned long long>(grow && tier == 0 ? newcap : 0ull,
+                                                              (unsigned long long *)&a.ctl->pool_tail[0], scratch64);
+        const uint64_t o1 = block_reserve<unsigned long long>(grow && tier == 1 ? newcap : 0ull,
+                                                              (unsigned long long *)&a.ctl->pool_tail[1], scratch64);
+        const uint64_t o = tier ? o1 : o0;
+        const bool over = grow && (o + newcap > a.ctl->pool_cap[tier] || newcap > 0xFFFFFFFFull);
+        block_append(over, u, a.overflow, &a.ctl->overflow_cnt, scratch);
+        const bool go = on && !over && na != 0;
+        const uint64_t noff = grow ? (o | (tier ? kHostTier : 0ull)) : r.off;
+        const uint32_t np = go ? (grow ? pieces_of(r.deg) : 0u) + pieces_of(na) : 0u;
+        const uint32_t pb = block_reserve<uint32_t>(np, &a.ctl->copy_cnt, scratch);
+        if (go) {
+            const uint32_t *old_row = a.pools.row(r.off);
+            uint32_t *row = a.pools.row(noff);
+            uint32_t k = pb;
+            if (grow)
+                for (uint32_t p = 0; p < r.deg; p += kPieceLen)
+                    a.pieces[k++] = AddPiece{old_row + p, row + p, min(kPieceLen, r.deg - p), 0u};
+            const uint32_t *al = a.grouped + a.gofs[u] + nd;
+            for (uint32_t p = 0; p < na; p += kPieceLen)
+                a.pieces[k++] = AddPiece{al + p, row + r.deg + p, min(kPieceLen, na - p), 0u};
+            if (grow) {
                 Row nr;
                 nr.off = noff;
                 nr.deg = uint32_t(newdeg);
                 nr.cap = uint32_t(newcap);
                 a.rows[u] = nr;
+                acc_garbage[tier] += r.cap;
+                acc_reloc += 1;
+            } else {
+                a.rows[u].deg = uint32_t(newdeg);
             }
-            acc_garbage[tier] += r.cap;
-            acc_reloc += 1;
+            acc_adds += na;
         }
-        acc_adds += na;
     }
-    if (lane == 0) {
-        if (acc_adds) atomicAdd((unsigned long long *)&a.ctl->adds_applied, (unsigned long long)acc_adds);
-        if (acc_reloc) atomicAdd((unsigned long long *)&a.ctl->relocated, (unsigned long long)acc_reloc);
-        for (int t = 0; t < 2; ++t)
-            if (acc_garbage[t]) atomicAdd((unsigned long long *)&a.ctl->garbage[t], (unsigned long long)acc_garbage[t]);
-    }
+    StatShard *sh = stat_shard(a.stats);
+    stat_add(&sh->adds_applied, wave_sum64(acc_adds));
+    stat_add(&sh->relocated, wave_sum64(acc_reloc));
+    stat_add(&sh->garbage[0], wave_sum64(acc_garbage[0]));
+    stat_add(&sh->garbage[1], wave_sum64(acc_garbage[1]));
 }
 
-// Copies long relocated rows: every workgroup takes 1024-slot tiles of each
-// job, so one long row is spread over many workgroups.
-__global__ void k_copy_jobs(MutArgs a) {
+// Moves the pieces of the insert phase: a wave takes 64 pieces, scans their
+// lengths and copies the concatenation 256 slots per step, so short pieces
+// (a few adds) and long ones (a relocated row) keep every lane busy.
+__global__ __launch_bounds__(kBlock) void k_add_copy(MutArgs a) {
+    __shared__ uint32_t pref_lds[kWavesPerBlock][kWave];
+    uint32_t *pref = pref_lds[wave_in_block()];
     const uint32_t cnt = ld_acquire(&a.ctl->copy_cnt);
-    constexpr uint32_t kTile = 1024;
-    for (uint32_t j = 0; j < cnt; ++j) {
-        const CopyJob jb = a.jobs[j];
-        const uint32_t *src = a.pools.row(jb.from);
-        uint32_t *dst = a.pools.row(jb.to);
-        for (uint32_t t = blockIdx.x * kTile; t < jb.len; t += gridDim.x * kTile) {
-            uint32_t v[4];
+    const uint32_t lane = lane_id();
+    for (uint64_t base = uint64_t(gwave()) * kWave; base < cnt; base += uint64_t(gwaves()) * kWave) {
+        const uint32_t i = uint32_t(base) + lane;
+        AddPiece pc{nullptr, nullptr, 0u, 0u};
+        if (i < cnt) pc = a.pieces[i];
+        const uint32_t incl = wave_incl_scan(pc.len);
+        const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        pref[lane] = incl - pc.len;
+        wave_lds_sync();
+        const unsigned long long src = (unsigned long long)pc.src, dst = (unsigned long long)pc.dst;
+        for (uint32_t k0 = 0; k0 < total; k0 += kWave * 4) {
+            uint32_t val[4], *dp[4];
+            bool ok[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t p = t + uint32_t(q) * blockDim.x + threadIdx.x;
-                v[q] = p < jb.len ? src[p] : 0u;
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t k = k0 + uint32_t(e) * kWave + lane;
+                ok[e] = k < total;
+                const uint32_t kk = ok[e] ? k : total - 1;
+                const uint32_t own = owner_of(pref, kk);
+                const uint32_t off = kk - pref[own];
+                const uint32_t *sp = reinterpret_cast<const uint32_t *>(__shfl(src, int(own), kWave));
+                dp[e] = reinterpret_cast<uint32_t *>(__shfl(dst, int(own), kWave)) + off;
+                val[e] = sp[off];
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t p = t + uint32_t(q) * blockDim.x + threadIdx.x;
-                if (p < jb.len) dst[p] = v[q];
-            }
+            for (int e = 0; e < 4; ++e)
+                if (ok[e]) *dp[e] = val[e];
         }
+        wave_lds_sync();
     }
 }
 
@@ -482,6 +596,7 @@ struct InvalArgs {
     uint32_t *stamp;
     uint32_t *aff;
     uint32_t *alist;
+    uint32_t *filt;  // global copy of the repair filter (kFilterWords words)
     uint32_t *q0, *q1;
     Tile *t0, *t1;
     ExpandCfg cfg;
@@ -498,6 +613,7 @@ struct InvalArgs {
 // reset_del_edges (driver.cuh:371-411): a deleted (u,v) whose v has parent u
 // seeds the affected set. Parallel copies of (u,v) are treated conservatively.
 __global__ void k_inval_seed(InvalArgs a) {
+    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     for (uint32_t base = blockIdx.x * blockDim.x; base < a.nd; base += gthreads()) {
         const uint32_t i = base + threadIdx.x;
         bool hit = false;
@@ -509,14 +625,14 @@ __global__ void k_inval_seed(InvalArgs a) {
                 hit = bit_claim(a.aff, v);
             }
         }
-        wave_append(hit, v, a.alist, &a.ctl->acount);
+        block_append(hit, v, a.alist, &a.ctl->acount, scratch);
     }
 }
 
 // Invalidation waves (RunSyncPushDDB_del rounds, framework.cuh:2815-2853) as
 // one persistent launch: the affected list grows in place; round k expands
 // the items appended by round k-1.
-template <int EPL>
+template <int QPL>
 __global__ __launch_bounds__(kBlock) void k_inval_loop(InvalArgs a) {
     __shared__ WaveLds lds[kWavesPerBlock];
     WaveLds &L = lds[wave_in_block()];
@@ -529,14 +645,13 @@ __global__ __launch_bounds__(kBlock) void k_inval_loop(InvalArgs a) {
         const uint32_t nt = ld_acquire(&c->icount[k % 3]);
         if (b == e && nt == 0) break;
         if (blockIdx.x == 0 && threadIdx.x == 0) c->icount[(k + 2) % 3] = 0;
-        Sink sink{a.alist, &c->acount};
-        TileSink ts{(k & 1) ? a.t0 : a.t1, &c->icount[(k + 1) % 3]};
-        expand_range<EPL>(op, a.cfg, a.alist + b, e - b, sink, ts, L, ctr);
-        expand_tiles<EPL>(op, a.cfg, (k & 1) ? a.t1 : a.t0, nt, sink, ctr);
+        const DirectOut out{Sink{a.alist, &c->acount}, TileSink{(k & 1) ? a.t0 : a.t1, &c->icount[(k + 1) % 3]}};
+        expand_range<QPL>(op, a.cfg, a.alist + b, e - b, out, L, ctr);
+        expand_tiles<QPL>(op, a.cfg, (k & 1) ? a.t1 : a.t0, nt, out);
         b = e;
         cg::this_grid().sync();
     }
-    flush_counters(ctr, c, a.cfg.stage);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
 }
 
 // Opens the invalidation frontier after the seeds: [0, acount).
@@ -551,31 +666,38 @@ __global__ void k_inval_begin(Ctl *c) {
 // One invalidation wave as an ordinary launch: expands [ibegin, iend) of the
 // affected list; the closing workgroup moves the window to the items this
 // round appended.
-template <int EPL>
-__global__ __launch_bounds__(kBlock) void k_inval_round(InvalArgs a) {
+using RoundStage = StageLds<2048, 128>;
+
+template <int QPL>
+__global__ __launch_bounds__(kBlock) void k_inval_round(InvalArgs a, uint64_t *prog, uint64_t tag) {
     __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ RoundStage stage;
     Ctl *c = a.ctl;
     const uint32_t b = c->ibegin, e = c->iend, k = c->iround;
     const uint32_t nt = c->icount[k % 3];
-    if (b == e && nt == 0) return;
+    if (b == e && nt == 0) return post_idle(prog, tag);
     InvalidateOp op{a.rows, a.state, a.aff};
     Counters ctr;
-    Sink sink{a.alist, &c->acount};
-    TileSink ts{(k & 1) ? a.t0 : a.t1, &c->icount[(k + 1) % 3]};
-    expand_range<EPL>(op, a.cfg, a.alist + b, e - b, sink, ts, lds[wave_in_block()], ctr);
-    expand_tiles<EPL>(op, a.cfg, (k & 1) ? a.t1 : a.t0, nt, sink, ctr);
-    flush_counters(ctr, c, a.cfg.stage);
+    const StagedOut<2048, 128> out{Sink{a.alist, &c->acount},
+                                   TileSink{(k & 1) ? a.t0 : a.t1, &c->icount[(k + 1) % 3]}, &stage};
+    out_init(out);
+    expand_range<QPL>(op, a.cfg, a.alist + b, e - b, out, lds[wave_in_block()], ctr);
+    expand_tiles<QPL>(op, a.cfg, (k & 1) ? a.t1 : a.t0, nt, out);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
+    out_flush(out);
     close_round(c, [&] {
         c->icount[k % 3] = 0;
         c->ibegin = e;
         c->iend = ld_acquire(&c->acount);
         c->iround = k + 1;
+        post_progress(prog, tag, c->iend != e || c->icount[(k + 1) % 3] != 0);
     });
 }
 
 // reset_affected_values (driver.cuh:418-433): affected vertices restart from
 // the initial value (CC: own label), and all of them form the next frontier.
 __global__ void k_reset_affected(InvalArgs a) {
+    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     const uint32_t cnt = ld_acquire(&a.ctl->acount);
     const uint32_t r = a.ctl->round;
     uint32_t *q = (r & 1) ? a.q1 : a.q0;
@@ -588,7 +710,7 @@ __global__ void k_reset_affected(InvalArgs a) {
             a.state[v] = a.kind == 1 ? ((uint64_t(v) << 32) | kNone) : ~0ull;
             push = claim(a.stamp, v, r);
         }
-        wave_append(push, v, q, &a.ctl->qcount[r % 3]);
+        block_append(push, v, q, &a.ctl->qcount[r % 3], scratch);
     }
 }
 
@@ -601,6 +723,7 @@ __global__ void k_clear_affected(InvalArgs a) {
 // value(u)+w to v; improved destinations join the pending frontier.
 template <int W, bool kStrict>
 __global__ void k_insert_seed(InvalArgs a, uint32_t skip_affected) {
+    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     const uint32_t r = a.ctl->round;
     uint32_t *q = (r & 1) ? a.q1 : a.q0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < a.na; base += gthreads()) {
@@ -620,19 +743,21 @@ __global__ void k_insert_seed(InvalArgs a, uint32_t skip_affected) {
                 }
             }
         }
-        wave_append(push, v, q, &a.ctl->qcount[r % 3]);
+        block_append(push, v, q, &a.ctl->qcount[r % 3], scratch);
     }
 }
 
 // -------------------------------------------------------------- repair scan
-// Every workgroup first folds the affected list into an LDS bitmap filter of
-// 2^18 bits (a multiplicative hash of the id); an edge whose destination
-// misses the filter is not affected, so almost every edge of the scan is
-// settled in LDS and only filter hits read the global affected bitmap. The
-// filter is skipped when the affected list is too long to stay selective.
+// The affected list is folded once into a bitmap filter of 2^18 bits (a
+// multiplicative hash of the id) in global memory; every repair workgroup
+// copies it into LDS. An edge whose destination misses the filter is not
+// affected, so almost every edge of the scan is settled in LDS and only
+// filter hits read the global affected bitmap. The filter is off when the
+// affected list is too long to stay selective.
 constexpr uint32_t kFilterBits = 18;
 constexpr uint32_t kFilterWords = (1u << kFilterBits) / 32;
 constexpr uint32_t kFilterMaxItems = 1u << 16;
+constexpr uint32_t kRepairBlock = 512;
 
 __device__ __forceinline__ uint32_t filter_hash(uint32_t v) { return (v * 0x9E3779B1u) >> (32 - kFilterBits); }
 
@@ -648,14 +773,21 @@ struct AffFilter {
     }
 };
 
-__device__ bool build_filter(uint32_t *lds, const uint32_t *alist, uint32_t count) {
-    if (count > kFilterMaxItems) return false;
-    for (uint32_t w = threadIdx.x; w < kFilterWords; w += blockDim.x) lds[w] = 0u;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
-        const uint32_t h = filter_hash(alist[i]);
-        atomicOr(&lds[h >> 5], 1u << (h & 31));
+// a.filt must be zero on entry (k_fill32 before this launch).
+__global__ void k_build_filter(InvalArgs a) {
+    const uint32_t na = ld_acquire(&a.ctl->acount);
+    if (na > kFilterMaxItems) return;
+    for (uint32_t i = gthread(); i < na; i += gthreads()) {
+        const uint32_t h = filter_hash(a.alist[i]);
+        atomicOr(&a.filt[h >> 5], 1u << (h & 31));
     }
+}
+
+__device__ __forceinline__ bool load_filter(uint32_t *lds, const uint32_t *g, uint32_t na) {
+    if (na > kFilterMaxItems) return false;
+    const uint4 *src = reinterpret_cast<const uint4 *>(g);
+    uint4 *dst = reinterpret_cast<uint4 *>(lds);
+    for (uint32_t w = threadIdx.x; w < kFilterWords / 4; w += blockDim.x) dst[w] = src[w];
     __syncthreads();
     return true;
 }
@@ -692,56 +824,61 @@ struct RepairScanOp {
     }
 };
 
-template <int W, bool kStrict, int EPL>
-__global__ __launch_bounds__(kBlock) void k_repair_scan(InvalArgs a) {
-    __shared__ uint32_t filt[kFilterWords];
-    __shared__ WaveLds lds[kWavesPerBlock];
+template <int W, bool kStrict, int QPL>
+__global__ __launch_bounds__(kRepairBlock) void k_repair_scan(InvalArgs a) {
+    __shared__ __align__(16) uint32_t filt[kFilterWords];
+    __shared__ WaveLds lds[kRepairBlock / kWave];
+    __shared__ StageLds<2, 160> stage;
     Ctl *c = a.ctl;
     const uint32_t na = ld_acquire(&c->acount);
     if (na == 0) return;
-    const bool on = build_filter(filt, a.alist, na);
+    const bool on = load_filter(filt, a.filt, na);
     RepairScanOp<W, kStrict> op{a.rows, a.state, AffFilter{on ? filt : nullptr, a.aff}, a.seed};
     Counters ctr;
-    Sink none{nullptr, &c->scount};
-    TileSink ts{a.t0, &c->scount};
-    expand_range<EPL>(op, a.cfg, nullptr, a.n, none, ts, lds[wave_in_block()], ctr);
-    flush_counters(ctr, c, a.cfg.stage);
+    const StagedOut<2, 160> out{Sink{nullptr, &c->scount}, TileSink{a.t0, &c->scount}, &stage};
+    out_init(out);
+    expand_range<QPL>(op, a.cfg, nullptr, a.n, out, lds[wave_in_block()], ctr);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
+    out_flush(out);
 }
 
-template <int W, bool kStrict, int EPL>
-__global__ __launch_bounds__(kBlock) void k_repair_tiles(InvalArgs a) {
-    __shared__ uint32_t filt[kFilterWords];
+template <int W, bool kStrict, int QPL>
+__global__ __launch_bounds__(kRepairBlock) void k_repair_tiles(InvalArgs a) {
+    __shared__ __align__(16) uint32_t filt[kFilterWords];
     Ctl *c = a.ctl;
     const uint32_t na = ld_acquire(&c->acount);
     const uint32_t nt = ld_acquire(&c->scount);
     if (na == 0 || nt == 0) return;
-    const bool on = build_filter(filt, a.alist, na);
+    const bool on = load_filter(filt, a.filt, na);
     RepairScanOp<W, kStrict> op{a.rows, a.state, AffFilter{on ? filt : nullptr, a.aff}, a.seed};
-    Counters ctr;
-    Sink none{nullptr, &c->scount};
-    expand_tiles<EPL>(op, a.cfg, a.t0, nt, none, ctr);
+    expand_tiles<QPL>(op, a.cfg, a.t0, nt, DirectOut{Sink{nullptr, &c->scount}, TileSink{nullptr, &c->scount}});
 }
 
 // ------------------------------------------------- one-shot expansion passes
-template <int EPL, class Op>
+template <int QPL, class Op>
 __global__ __launch_bounds__(kBlock) void k_expand_once(Op op, ExpandCfg cfg, const uint32_t *items,
                                                          const uint32_t *count_ptr, uint32_t count,
-                                                         Sink sink, TileSink ts, Ctl *ctl,
-                                                         const uint32_t *gate) {
+                                                         Sink sink, TileSink ts, const uint32_t *gate) {
     __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ RoundStage stage;
     if (gate && ld_acquire(gate) == 0) return;
     Counters ctr;
     const uint32_t n = count_ptr ? ld_acquire(count_ptr) : count;
-    expand_range<EPL>(op, cfg, items, n, sink, ts, lds[wave_in_block()], ctr);
-    flush_counters(ctr, ctl, cfg.stage);
+    const StagedOut<2048, 128> out{sink, ts, &stage};
+    out_init(out);
+    expand_range<QPL>(op, cfg, items, n, out, lds[wave_in_block()], ctr);
+    flush_counters(ctr, cfg.stats, cfg.stage);
+    out_flush(out);
 }
 
-template <int EPL, class Op>
+template <int QPL, class Op>
 __global__ __launch_bounds__(kBlock) void k_expand_tiles(Op op, ExpandCfg cfg, const Tile *tiles,
-                                                          const uint32_t *ntiles, Sink sink, Ctl *ctl) {
-    Counters ctr;
-    expand_tiles<EPL>(op, cfg, tiles, ld_acquire(ntiles), sink, ctr);
-    flush_counters(ctr, ctl, cfg.stage);
+                                                          const uint32_t *ntiles, Sink sink) {
+    __shared__ RoundStage stage;
+    const StagedOut<2048, 128> out{sink, TileSink{nullptr, sink.count}, &stage};
+    out_init(out);
+    expand_tiles<QPL>(op, cfg, tiles, ld_acquire(ntiles), out);
+    out_flush(out);
 }
 
 // ------------------------------------------------------ traversal fixed point
@@ -764,7 +901,7 @@ struct LoopArgs {
 // queue[r&1] / count slot r%3 and produces into the other queue and slot
 // (r+1)%3; slot (r+2)%3 is cleared for round r+1. Tiles of long rows found in
 // round r are expanded in round r+1 (re-reading the row owner's value).
-template <int W, bool kStrict, int EPL>
+template <int W, bool kStrict, int QPL>
 __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
     __shared__ WaveLds lds[kWavesPerBlock];
     WaveLds &L = lds[wave_in_block()];
@@ -782,13 +919,13 @@ __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
         }
         RelaxOp<W, kStrict> op{a.rows, a.state, a.stamp, a.hot, a.seed, r + 1};
         const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
-        Sink sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]};
-        TileSink ts{(r & 1) ? a.t0 : a.t1, &c->tcount[(r + 1) % 3]};
-        expand_range<EPL>(op, a.cfg, qin, nq, sink, ts, L, ctr);
-        expand_tiles<EPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, sink, ctr);
+        const DirectOut out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
+                            TileSink{(r & 1) ? a.t0 : a.t1, &c->tcount[(r + 1) % 3]}};
+        expand_range<QPL>(op, a.cfg, qin, nq, out, L, ctr);
+        expand_tiles<QPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, out);
         cg::this_grid().sync();
     }
-    flush_counters(ctr, c, a.cfg.stage);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->round = r;
         c->rounds += it;
@@ -801,6 +938,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
 // sees what `close` wrote. No workgroup ever waits for another.
 template <class F>
 __device__ __forceinline__ void close_round(Ctl *c, F close) {
+    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
@@ -813,28 +951,32 @@ __device__ __forceinline__ void close_round(Ctl *c, F close) {
 }
 
 // One relax round as an ordinary launch (the default driver): the host
-// enqueues rounds back to back; a round whose frontier and deferred tiles
-// are empty returns at once.
-template <int W, bool kStrict, int EPL>
-__global__ __launch_bounds__(kBlock) void k_relax_round(LoopArgs a) {
+// keeps a few rounds enqueued ahead; a round whose frontier and deferred
+// tiles are empty returns at once.
+template <int W, bool kStrict, int QPL>
+__global__ __launch_bounds__(kBlock) void k_relax_round(LoopArgs a, uint64_t *prog, uint64_t tag) {
     __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ RoundStage stage;
     Ctl *c = a.ctl;
     const uint32_t r = c->round;
     const uint32_t nq = c->qcount[r % 3], nt = c->tcount[r % 3];
-    if (nq == 0 && nt == 0) return;
+    if (nq == 0 && nt == 0) return post_idle(prog, tag);
     Counters ctr;
     RelaxOp<W, kStrict> op{a.rows, a.state, a.stamp, a.hot, a.seed, r + 1};
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
-    Sink sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]};
-    TileSink ts{(r & 1) ? a.t0 : a.t1, &c->tcount[(r + 1) % 3]};
-    expand_range<EPL>(op, a.cfg, qin, nq, sink, ts, lds[wave_in_block()], ctr);
-    expand_tiles<EPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, sink, ctr);
-    flush_counters(ctr, c, a.cfg.stage);
+    const StagedOut<2048, 128> out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
+                                   TileSink{(r & 1) ? a.t0 : a.t1, &c->tcount[(r + 1) % 3]}, &stage};
+    out_init(out);
+    expand_range<QPL>(op, a.cfg, qin, nq, out, lds[wave_in_block()], ctr);
+    expand_tiles<QPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, out);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
+    out_flush(out);
     close_round(c, [&] {
         c->qcount[r % 3] = 0;
         c->tcount[r % 3] = 0;
         c->round = r + 1;
         c->rounds += 1;
+        post_progress(prog, tag, c->qcount[(r + 1) % 3] != 0 || c->tcount[(r + 1) % 3] != 0);
     });
 }
 
@@ -854,7 +996,7 @@ struct PrArgs {
 // Runtime::Converge (pr_residual.cuh:171-187) as one persistent launch:
 // consume (rank += r with Kahan compensation, r = 0, contribution
 // alpha*r/deg), grid barrier, scatter, grid barrier.
-template <int EPL>
+template <int QPL>
 __global__ __launch_bounds__(kBlock) void k_pr_loop(PrArgs a) {
     __shared__ WaveLds lds[kWavesPerBlock];
     WaveLds &L = lds[wave_in_block()];
@@ -910,16 +1052,16 @@ __global__ __launch_bounds__(kBlock) void k_pr_loop(PrArgs a) {
         }
         cg::this_grid().sync();
         PrScatterOp op{a.rows, a.res, a.queued, a.delta, &c->pr_invalid, a.eps};
-        Sink sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]};
-        TileSink ts{tiles, &c->scount};  // unused: long rows were tiled while consuming
+        // tiles of long rows were made while consuming; the scatter makes none
+        const DirectOut out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]}, TileSink{tiles, &c->scount}};
         Counters sctr;
-        expand_range<EPL>(op, a.cfg, qin, nq, sink, ts, L, sctr);
+        expand_range<QPL>(op, a.cfg, qin, nq, out, L, sctr);
         ctr.scanned += sctr.scanned;
         ctr.host += sctr.host;
-        expand_tiles<EPL>(op, a.cfg, tiles, ld_acquire(&c->tcount[r % 3]), sink, ctr);
+        expand_tiles<QPL>(op, a.cfg, tiles, ld_acquire(&c->tcount[r % 3]), out);
         cg::this_grid().sync();
     }
-    flush_counters(ctr, c, a.cfg.stage);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->round = r;
         c->rounds += it;
@@ -928,8 +1070,10 @@ __global__ __launch_bounds__(kBlock) void k_pr_loop(PrArgs a) {
 }
 
 // PageRank round, consume phase (pr_residual.cuh:54-71) as an ordinary
-// launch; long rows are cut into tiles for this round's scatter.
+// launch; long rows are cut into tiles for this round's scatter (one tile
+// reservation per workgroup step).
 __global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a) {
+    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     Ctl *c = a.ctl;
     const uint32_t r = c->round;
     const uint32_t nq = c->qcount[r % 3];
@@ -937,71 +1081,81 @@ __global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a) {
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
     Tile *tiles = (r & 1) ? a.t1 : a.t0;
     Counters ctr;
-    for (uint32_t i = gthread(); i < nq; i += gthreads()) {
-        const uint32_t v = qin[i];
-        const float rv = a.res[v];
-        a.queued[v] = 0u;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nq; base += gthreads()) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t v = 0, nt = 0;
+        Row row{};
         float d = 0.0f;
-        if (!isfinite(rv) || !isfinite(a.rank[v])) atomicExch(&c->pr_invalid, 1u);
-        if (fabsf(rv) > a.eps) {
-            a.res[v] = 0.0f;
-            const float x = a.rank[v];
-            const float y = rv - a.comp[v];
-            const float t = x + y;
-            a.comp[v] = (t - x) - y;
-            a.rank[v] = t;
-            const Row row = a.rows[v];
-            ctr.vexp += 1;
-            if (row.deg) {
-                d = 0.85f * rv / float(row.deg);
-                if (row.deg >= a.cfg.split) {
-                    const uint32_t nt = (row.deg + a.cfg.tile_len - 1) / a.cfg.tile_len;
-                    const uint32_t t0 = atomicAdd(&c->tcount[r % 3], nt);
-                    for (uint32_t j = 0; j < nt; ++j) {
-                        Tile t;
-                        t.start = row.off + uint64_t(j) * a.cfg.tile_len;
-                        t.v = v;
-                        t.len = min(a.cfg.tile_len, row.deg - j * a.cfg.tile_len);
-                        t.payload = __float_as_uint(d);
-                        t.pad = 0;
-                        tiles[t0 + j] = t;
-                    }
-                    ctr.scanned += row.deg;
-                    if (row.off & kHostTier) ctr.host += row.deg;
-                    d = 0.0f;
+        if (i < nq) {
+            v = qin[i];
+            const float rv = a.res[v];
+            a.queued[v] = 0u;
+            if (!isfinite(rv) || !isfinite(a.rank[v])) atomicExch(&c->pr_invalid, 1u);
+            if (fabsf(rv) > a.eps) {
+                a.res[v] = 0.0f;
+                const float x = a.rank[v];
+                const float y = rv - a.comp[v];
+                const float t = x + y;
+                a.comp[v] = (t - x) - y;
+                a.rank[v] = t;
+                row = a.rows[v];
+                ctr.vexp += 1;
+                if (row.deg) {
+                    d = 0.85f * rv / float(row.deg);
+                    if (row.deg >= a.cfg.split) nt = (row.deg + a.cfg.tile_len - 1) / a.cfg.tile_len;
                 }
             }
         }
-        a.delta[i] = d;
+        const uint32_t t0 = block_reserve<uint32_t>(nt, &c->tcount[r % 3], scratch);
+        for (uint32_t j = 0; j < nt; ++j) {
+            Tile t;
+            t.start = row.off + uint64_t(j) * a.cfg.tile_len;
+            t.v = v;
+            t.len = min(a.cfg.tile_len, row.deg - j * a.cfg.tile_len);
+            t.payload = __float_as_uint(d);
+            t.pad = 0;
+            tiles[t0 + j] = t;
+        }
+        if (nt) {
+            ctr.scanned += row.deg;
+            if (row.off & kHostTier) ctr.host += row.deg;
+            d = 0.0f;
+        }
+        if (i < nq) a.delta[i] = d;
     }
-    flush_counters(ctr, c, a.cfg.stage);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
 }
 
 // PageRank round, scatter phase (pr_residual.cuh:74-100); the closing
 // workgroup advances the round.
-template <int EPL>
-__global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a) {
+template <int QPL>
+__global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a, uint64_t *prog, uint64_t tag) {
     __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ RoundStage stage;
     Ctl *c = a.ctl;
     const uint32_t r = c->round;
     const uint32_t nq = c->qcount[r % 3];
-    if (nq == 0) return;
+    if (nq == 0) return post_idle(prog, tag);
     const uint32_t nt = c->tcount[r % 3];
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
     Tile *tiles = (r & 1) ? a.t1 : a.t0;
     PrScatterOp op{a.rows, a.res, a.queued, a.delta, &c->pr_invalid, a.eps};
-    Sink sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]};
-    TileSink ts{tiles, &c->scount};  // unused: long rows were tiled while consuming
+    // tiles of long rows were made while consuming; the scatter makes none
+    const StagedOut<2048, 128> out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
+                                   TileSink{tiles, &c->scount}, &stage};
+    out_init(out);
     Counters ctr;
-    expand_range<EPL>(op, a.cfg, qin, nq, sink, ts, lds[wave_in_block()], ctr);
-    expand_tiles<EPL>(op, a.cfg, tiles, nt, sink, ctr);
-    flush_counters(ctr, c, a.cfg.stage);
+    expand_range<QPL>(op, a.cfg, qin, nq, out, lds[wave_in_block()], ctr);
+    expand_tiles<QPL>(op, a.cfg, tiles, nt, out);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
+    out_flush(out);
     close_round(c, [&] {
         c->qcount[r % 3] = 0;
         c->tcount[r % 3] = 0;
         c->round = r + 1;
         c->rounds += 1;
         c->converged = ld_acquire(&c->qcount[(r + 1) % 3]) == 0 ? 1u : 0u;
+        post_progress(prog, tag, c->converged == 0u);
     });
 }
 
@@ -1012,11 +1166,46 @@ __global__ void k_copy_probe(const uint4 *src, uint4 *dst, uint64_t n16) {
 }
 
 // FP64 residual norm sum |r| (the only FP64 arithmetic of the PR path).
-__global__ void k_pr_l1(const float *res, uint32_t n, Ctl *c) {
+__global__ void k_pr_l1(const float *res, uint32_t n, StatShard *stats) {
     double s = 0;
     for (uint32_t v = gthread(); v < n; v += gthreads()) s += fabs(double(res[v]));
     for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kWave);
-    if (lane_id() == 0 && s != 0) atomicAdd(&c->residual_l1, s);
+    if (lane_id() == 0 && s != 0) atomicAdd(&stat_shard(stats)->l1, s);
+}
+
+// Folds the statistics shards into the control block (adding to its
+// totals) and clears them. One workgroup of 64 threads: lane j reads shard j.
+__global__ void k_reduce_stats(StatShard *stats, Ctl *c) {
+    static_assert(kStatShards == kWave, "one lane per shard");
+    const uint32_t j = threadIdx.x;
+    uint64_t *w = reinterpret_cast<uint64_t *>(stats + j);
+    uint64_t v[kStatWords];
+#pragma unroll
+    for (uint32_t f = 0; f < kStatWords; ++f) {
+        v[f] = wave_sum64(w[f]);
+        w[f] = 0;
+    }
+    double l1 = stats[j].l1;
+    stats[j].l1 = 0;
+    for (int d = kWave / 2; d > 0; d >>= 1) l1 += __shfl_xor(l1, d, kWave);
+    if (j == 0) {
+        for (int i = 0; i < 4; ++i) {
+            c->st_edges[i] += v[i];
+            c->st_vexp[i] += v[4 + i];
+            c->st_host[i] += v[8 + i];
+            c->edges_scanned += v[i];
+            c->vexp += v[4 + i];
+            c->host_edges += v[8 + i];
+        }
+        c->deletes_applied += v[12];
+        c->missing += v[13];
+        c->invalid += v[14];
+        c->adds_applied += v[15];
+        c->relocated += v[16];
+        c->garbage[0] += v[17];
+        c->garbage[1] += v[18];
+        c->residual_l1 += l1;
+    }
 }
 
 // Pool compaction: copies every row of `tier` to its new offset.

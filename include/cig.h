@@ -89,7 +89,7 @@ typedef struct cig_config {
     uint32_t n_stream;        /* accepted for flag compatibility */
     uint32_t segments;        /* accepted for flag compatibility (1..512) */
     uint32_t block_size;      /* threads per workgroup; 0 = tuned default */
-    uint32_t edges_per_lane;  /* edges each lane keeps in flight; 0 = tuned default */
+    uint32_t edges_per_lane;  /* edges each lane keeps in flight: 4, 8 or 16 (whole 16-B quads); 0 = tuned default */
     uint32_t split_degree;    /* rows at least this long are split into tiles; 0 = default */
     uint32_t max_rounds;      /* traversal round cap per launch; 0 = unlimited (exact) */
     uint32_t pr_max_rounds;   /* rounds per solve; the reference's flag allows
@@ -131,7 +131,8 @@ typedef struct cig_batch_stats {
     uint64_t stage_edges[4];
     uint64_t stage_vertices[4];
     uint64_t stage_host_edges[4];
-    uint64_t reserved[4];
+    double scan_kernel_ms;    /* the repair scan kernel alone (k_repair_scan), 0 if not run */
+    uint64_t reserved[3];
 } cig_batch_stats;
 
 typedef struct cig_graph_info {

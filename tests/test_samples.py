@@ -104,7 +104,9 @@ def test_sample_rmat_stream(app, tmp_path):
     args = [f"--graphfile={gp}", f"--updatefile={up}", f"--update_size={bp}", "--check=true",
             f"--output={out}", f"--{app}_max_batches=10", "--source_node=2"]
     if app == "pr":
-        args += ["--error=0.0000001", "--pr_max_rounds=100"]
+        # the sample keeps the reference's 1..100 round limit (hybrid_pr.cu:129-132);
+        # 1e-6 converges within it on this graph
+        args += ["--error=0.000001", "--pr_max_rounds=100"]
     r = _run(app, *args)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert "clamp" in r.stdout  # 10 requested, 4 available (hybrid_sssp.cu:402-407)
@@ -113,8 +115,7 @@ def test_sample_rmat_stream(app, tmp_path):
         oracle_apply(og, *s.batch(b))
     got = np.loadtxt(out, ndmin=2)
     if app == "pr":
-        # 100 rounds may stop short of epsilon (the reference's limit); the
-        # sample then reports iteration_limit and its own check decides
+        assert "Overall: Test passed" in r.stdout
         l1n, _ = pr_errors(got[:, 1].astype(np.float32), og.pagerank()[0])
         assert l1n < 1e-4
         return
