@@ -35,6 +35,7 @@ WEIGHT_REF128, WEIGHT_HASH64 = 0, 1
 FIELD_VALUE, FIELD_PARENT, FIELD_RESIDUAL, FIELD_DEGREE, FIELD_TIER, FIELD_HOTNESS = range(6)
 # flags
 CFG_COOPERATIVE, CFG_NO_HOTNESS = 1, 2
+CFG_SCAN_REPAIR = 1 << 2
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
@@ -137,6 +138,7 @@ def lib():
         "cig_load_csr": (C.c_int, [eng, C.c_uint32, P, P]),
         "cig_graph_info_get": (C.c_int, [eng, C.POINTER(GraphInfo)]),
         "cig_gather_topology": (C.c_int, [eng, P, P, C.c_uint64]),
+        "cig_gather_incoming": (C.c_int, [eng, P, P, C.c_uint64]),
         "cig_init": (C.c_int, [eng, P, C.c_uint32]),
         "cig_converge": (C.c_int, [eng, C.POINTER(BatchStats)]),
         "cig_delete_edges": (C.c_int, [eng, P, C.c_uint64, C.POINTER(BatchStats)]),
@@ -456,3 +458,12 @@ class Engine:
         dst = np.zeros(cap, dtype=np.uint32)
         self._call(lib().cig_gather_topology(self._e, _ptr(off), _ptr(dst), cap))
         return HostGraph(self.n, off, dst[: int(off[-1])])
+
+    def incoming(self) -> HostGraph:
+        """The device incoming-row index as CSR (row v = sources of v's in-edges)."""
+        info = self.info()
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        cap = max(int(info["edges"]), 1)
+        src = np.zeros(cap, dtype=np.uint32)
+        self._call(lib().cig_gather_incoming(self._e, _ptr(off), _ptr(src), cap))
+        return HostGraph(self.n, off, src[: int(off[-1])])

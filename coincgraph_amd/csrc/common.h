@@ -56,7 +56,9 @@ struct Ctl {
     uint32_t round;        // next absolute round to run
     uint32_t converged;
     uint32_t done;         // workgroups finished in the current round kernel
-    uint32_t ibegin, iend, iround;  // invalidation frontier [ibegin, iend), round
+    uint32_t iround;       // invalidation: first round of the running chain
+    uint32_t istart[3];    // invalidation: where round slot k's frontier starts in the affected list
+    uint32_t ifront[3];    // invalidation: frontier length of round slot k
     // invalidation
     uint32_t acount;       // affected list length
     uint32_t icount[3];    // invalidation tile counts
@@ -66,7 +68,8 @@ struct Ctl {
     uint32_t small_cnt, big_cnt, add_cnt, copy_cnt, overflow_cnt;
     uint32_t dtile_cnt, ht_cursor;  // long-row delete tiles and table slots
     uint32_t pr_invalid;
-    uint64_t group_cursor;
+    uint32_t group_cursor;  // records grouped so far (< 2^32 per call)
+    uint32_t pad_gc;
     uint64_t pool_tail[2]; // bump tails of the HBM (0) and host (1) pools
     uint64_t pool_cap[2];
     uint64_t garbage[2];   // slots abandoned by relocations
@@ -79,6 +82,7 @@ struct Ctl {
     uint64_t invalid;
     uint64_t adds_applied;
     uint64_t relocated;
+    uint64_t touched_sources;
     uint64_t rounds;
     double residual_l1;
     // per expansion pass: 0 invalidation, 1 repair scan, 2 propagation, 3 PR amend
@@ -95,11 +99,12 @@ struct StatShard {
     uint64_t edges[4], vexp[4], host[4];  // per expansion pass (Stage)
     uint64_t deletes_applied, missing, invalid, adds_applied, relocated;
     uint64_t garbage[2];
+    uint64_t touched;      // touched sources (outgoing rows only)
     double l1;             // PR residual norm
-    uint64_t pad[12];
+    uint64_t pad[11];
 };
 static_assert(sizeof(StatShard) == 256, "StatShard layout");
-constexpr uint32_t kStatWords = 19;  // leading uint64 fields of a shard
+constexpr uint32_t kStatWords = 20;  // leading uint64 fields of a shard
 
 enum Stage : uint32_t { kStageInvalidate = 0, kStageRepair = 1, kStagePropagate = 2, kStageAmend = 3 };
 

@@ -449,6 +449,37 @@ __device__ T block_reserve(T x, T *counter, T *scratch) {
     return r;
 }
 
+// block_reserve for K counters at once: one scan pass, then K threads of
+// wave 0 issue the K atomics together (one memory round trip per call).
+// `scratch` is K*(kMaxWavesPerBlock+1) LDS words. Every thread must call it.
+template <int K>
+__device__ void block_reserve_n(const uint32_t (&x)[K], uint32_t *const (&counter)[K], uint32_t (&out)[K],
+                                uint32_t *scratch) {
+    constexpr uint32_t S = kMaxWavesPerBlock + 1;
+    const uint32_t lane = lane_id(), w = wave_in_block(), nw = blockDim.x / kWave;
+    uint32_t incl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        incl[k] = wave_incl_scan(x[k]);
+        if (lane == kWave - 1) scratch[k * S + w] = incl[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(K)) {
+        const uint32_t k = threadIdx.x;
+        uint32_t run = 0;
+        for (uint32_t i = 0; i < nw; ++i) {
+            const uint32_t t = scratch[k * S + i];
+            scratch[k * S + i] = run;
+            run += t;
+        }
+        scratch[k * S + kMaxWavesPerBlock] = run ? atomicAdd(counter[k], run) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = scratch[k * S + kMaxWavesPerBlock] + scratch[k * S + w] + incl[k] - x[k];
+    __syncthreads();
+}
+
 // One-element-per-thread append through block_reserve.
 __device__ __forceinline__ void block_append(bool f, uint32_t v, uint32_t *list, uint32_t *count,
                                              uint32_t *scratch) {

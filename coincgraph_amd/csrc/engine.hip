@@ -110,6 +110,8 @@ struct cig_engine {
     bool scan_timed = false;  // ev[8]/ev[9] bracket this call's repair scan kernel
 
     uint32_t n = 0;
+    uint32_t nrows = 0;  // n, or 2n with the incoming rows [n, 2n)
+    bool rev = false;    // incoming-row index kept (pull repair)
     bool loaded = false, initialized = false;
     uint32_t source = 0;
 
@@ -124,6 +126,7 @@ struct cig_engine {
     DBuf<AddPiece> pieces;              // insert-phase copy pieces
     DBuf<StatShard> stats;              // sharded statistics (see k_reduce_stats)
     DBuf<uint32_t> htoff, hkeys, hcnt;  // long-row delete tables
+    DBuf<uint8_t> dclass;               // per-row delete class of the current call
     DBuf<uint2> hcnts, dtiles;
     uint64_t rec_cap = 0;
     DBuf<cig_edge> rec_dels, rec_adds;
@@ -226,22 +229,43 @@ struct cig_engine {
             if (dst[e] >= nv) throw HipError(CIG_EINVAL, "destination id out of range");
         n = nv;
         budget_slots = cfg.cache_bytes / sizeof(uint32_t);
+        // incoming rows (the device reverse index) for deletion repair
+        rev = !is_pr() && !(cfg.flags & CIG_CFG_SCAN_REPAIR);
+        if (rev && n > 0x7FFFFFFFu) throw HipError(CIG_ELIMIT, "incoming index needs n < 2^31 (use CIG_CFG_SCAN_REPAIR)");
+        nrows = rev ? 2 * n : n;
+        std::vector<uint64_t> ioff;
+        std::vector<uint32_t> isrc;
+        if (rev) {  // counting sort by destination; sources ascending per row
+            ioff.assign(size_t(n) + 1, 0);
+            for (uint64_t e = 0; e < m; ++e) ++ioff[dst[e] + 1];
+            for (uint32_t v = 0; v < n; ++v) ioff[v + 1] += ioff[v];
+            isrc.resize(m);
+            std::vector<uint64_t> fillp(ioff.begin(), ioff.end() - 1);
+            for (uint32_t u = 0; u < n; ++u)
+                for (uint64_t e = off[u]; e < off[u + 1]; ++e) isrc[fillp[dst[e]]++] = u;
+        }
+        auto row_data = [&](uint32_t v) -> const uint32_t * {
+            return v < n ? dst + off[v] : isrc.data() + ioff[v - n];
+        };
 
-        std::vector<Row> hrows(n);
+        std::vector<Row> hrows(nrows);
         uint64_t total_cap = 0;
-        for (uint32_t v = 0; v < n; ++v) {
-            hrows[v].deg = uint32_t(off[v + 1] - off[v]);
+        for (uint32_t v = 0; v < nrows; ++v) {
+            hrows[v].deg = uint32_t(v < n ? off[v + 1] - off[v] : ioff[v - n + 1] - ioff[v - n]);
             hrows[v].cap = uint32_t(next_pow2_cap(hrows[v].deg));
             total_cap += hrows[v].cap;
         }
-        // tier placement: longest rows first while the budget lasts (90% fill)
-        std::vector<uint8_t> in_hbm(n, 1);
+        // tier placement: longest outgoing rows first while the budget lasts
+        // (90% fill), then incoming rows (read only by the deletion repair)
+        std::vector<uint8_t> in_hbm(nrows, 1);
         uint64_t hbm_need = total_cap, host_need = 0;
         if (budget_slots && total_cap > budget_slots * 9 / 10) {
-            std::vector<uint32_t> order(n);
+            std::vector<uint32_t> order(nrows);
             std::iota(order.begin(), order.end(), 0u);
-            std::stable_sort(order.begin(), order.end(),
-                             [&](uint32_t a, uint32_t b) { return hrows[a].deg > hrows[b].deg; });
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                if ((a < n) != (b < n)) return a < n;
+                return hrows[a].deg > hrows[b].deg;
+            });
             uint64_t used = 0;
             const uint64_t limit = budget_slots * 9 / 10;
             for (uint32_t v : order) {
@@ -260,24 +284,25 @@ struct cig_engine {
         std::vector<uint32_t> stage;
         stage.reserve(hbm_need);
         uint64_t htail = 0, ctail = 0;
-        for (uint32_t v = 0; v < n; ++v) {
+        for (uint32_t v = 0; v < nrows; ++v) {
             Row &r = hrows[v];
+            const uint32_t *src = row_data(v);
             if (in_hbm[v]) {
                 r.off = htail;
-                stage.insert(stage.end(), dst + off[v], dst + off[v + 1]);
+                stage.insert(stage.end(), src, src + r.deg);
                 stage.resize(htail + r.cap, 0u);
                 htail += r.cap;
             } else {
                 r.off = ctail | kHostTier;
-                std::memcpy(host_pool.p + ctail, dst + off[v], size_t(r.deg) * sizeof(uint32_t));
+                std::memcpy(host_pool.p + ctail, src, size_t(r.deg) * sizeof(uint32_t));
                 ctail += r.cap;
             }
         }
         if (!stage.empty())
             HIPCHK(hipMemcpyAsync(hbm_pool.p, stage.data(), stage.size() * sizeof(uint32_t),
                                   hipMemcpyHostToDevice, stream));
-        rows.alloc(n);
-        HIPCHK(hipMemcpyAsync(rows.p, hrows.data(), n * sizeof(Row), hipMemcpyHostToDevice, stream));
+        rows.alloc(nrows);
+        HIPCHK(hipMemcpyAsync(rows.p, hrows.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice, stream));
 
         // per-vertex arrays
         q0.alloc(n);
@@ -286,16 +311,18 @@ struct cig_engine {
         hot.alloc(n);
         hist.alloc(n);
         score.alloc(n);
-        cnt64.alloc(n);
-        fill64.alloc(n);
-        gofs.alloc(n);
-        touched.alloc(n);
-        small_list.alloc(n);
-        big_list.alloc(n);
-        add_list.alloc(n);
-        overflow.alloc(n);
-        htoff.alloc(n);
-        hcnt.alloc(n);
+        // per-row mutation arrays (outgoing and incoming rows)
+        cnt64.alloc(nrows);
+        fill64.alloc(nrows);
+        gofs.alloc(nrows);
+        touched.alloc(nrows);
+        small_list.alloc(nrows);
+        big_list.alloc(nrows);
+        add_list.alloc(nrows);
+        overflow.alloc(nrows);
+        htoff.alloc(nrows);
+        hcnt.alloc(nrows);
+        dclass.alloc(nrows);
         if (is_pr()) {
             rank.alloc(n);
             comp.alloc(n);
@@ -311,9 +338,9 @@ struct cig_engine {
         fill32(stamp.p, n, 0);
         fill32(hot.p, n, 0);
         fill32(hist.p, n, 0);
-        fill32(hcnt.p, n, 0);
-        fill32(reinterpret_cast<uint32_t *>(cnt64.p), uint64_t(n) * 2, 0);
-        fill32(reinterpret_cast<uint32_t *>(fill64.p), uint64_t(n) * 2, 0);
+        fill32(hcnt.p, nrows, 0);
+        fill32(reinterpret_cast<uint32_t *>(cnt64.p), uint64_t(nrows) * 2, 0);
+        fill32(reinterpret_cast<uint32_t *>(fill64.p), uint64_t(nrows) * 2, 0);
         if (!is_pr()) fill32(aff.p, (uint64_t(n) + 31) / 32, 0);
         // control block: pools
         std::memset(hctl.p, 0, sizeof(Ctl));
@@ -377,19 +404,21 @@ struct cig_engine {
     // Enqueues round kernels back to back in growing chunks (8, 16, ... 128)
     // and reads the control block once per chunk; `more` says whether the
     // fixed point still has work. Rounds past the bound raise an error.
+    // Runs one chain of round kernels. launch(tag, seq) enqueues round seq
+    // (1-based); every round posts one progress word at its start (see
+    // post_progress) saying whether it has work. Rounds are launched
+    // kLookahead ahead of the last word seen, so the host never blocks on
+    // the stream and at most kLookahead no-op rounds follow the fixed point.
+    // Returns the number of rounds that had work.
     template <class Launch>
     uint64_t run_chain(Launch launch, uint64_t bound, const char *what) {
-        // Rounds are launched kLookahead ahead of the last one the GPU has
-        // closed; every launch posts one progress word (see post_progress),
-        // so the host never blocks on the stream and at most kLookahead
-        // no-op rounds follow the fixed point.
         constexpr uint64_t kLookahead = 3;
         const uint64_t epoch = ++chain_epoch & 0xFFFFFFFFull;
         uint64_t issued = 0, seen = 0;
         for (;;) {
             while (issued < bound && issued < seen + kLookahead) {
                 ++issued;
-                launch((epoch << 31) | issued);
+                launch((epoch << 31) | issued, uint32_t(issued));
             }
             HIPCHK(hipGetLastError());
             uint64_t w = 0;
@@ -409,12 +438,18 @@ struct cig_engine {
                 __builtin_ia32_pause();
             }
             seen = (w >> 1) & 0x7FFFFFFFull;
-            if (!(w & 1ull)) return seen;
+            if (!(w & 1ull)) return seen - 1;  // round `seen` found nothing to do
             if (seen >= bound) {
                 if (cfg.max_rounds || is_pr()) return seen;  // a configured cap
                 throw HipError(CIG_EINTERNAL, std::string(what) + " did not converge within the round bound");
             }
         }
+    }
+
+    void advance_round(uint64_t by) {
+        if (!by) return;
+        k_advance_round<<<1, 64, 0, stream>>>(ctl.p, uint32_t(by));
+        HIPCHK(hipGetLastError());
     }
 
     static bool relax_pending(const Ctl &c) {
@@ -444,8 +479,11 @@ struct cig_engine {
         } else {
             LoopArgs a = loop_args(1);
             const uint64_t bound = cfg.max_rounds ? cfg.max_rounds : round_bound();
-            run_chain([&](uint64_t tag) { k_relax_round<W, S, QPL><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag); },
-                      bound, "relax fixed point");
+            advance_round(run_chain(
+                [&](uint64_t tag, uint32_t seq) {
+                    k_relax_round<W, S, QPL><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                },
+                bound, "relax fixed point"));
         }
         HIPCHK(hipGetLastError());
     }
@@ -498,11 +536,12 @@ struct cig_engine {
             grid_coop = grid;
             launch_coop(kernel, a, grid);
         } else {
-            run_chain([&](uint64_t tag) {
-                          k_pr_consume<<<grid_round, kBlock, 0, stream>>>(a);
-                          k_pr_scatter<1><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag);
-                      },
-                      cfg.pr_max_rounds, "PageRank");
+            advance_round(run_chain(
+                [&](uint64_t tag, uint32_t seq) {
+                    k_pr_consume<<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                    k_pr_scatter<1><<<grid_round, kBlock, 0, stream>>>(a, seq);
+                },
+                cfg.pr_max_rounds, "PageRank"));
         }
         HIPCHK(hipGetLastError());
     }
@@ -587,7 +626,7 @@ struct cig_engine {
         st->invalid_updates = c.invalid;
         st->adds_applied = c.adds_applied;
         st->relocated_rows = c.relocated;
-        st->touched_sources = c.touched;
+        st->touched_sources = c.touched_sources;
         st->converged = is_pr() ? (c.qcount[c.round % 3] == 0 ? 1u : 0u) : 1u;
         st->pr_residual_l1 = c.residual_l1;
     }
@@ -599,6 +638,7 @@ struct cig_engine {
         m.rows = rows.p;
         m.pools = pools();
         m.n = n;
+        m.rev = rev ? 1u : 0u;
         m.dels = d;
         m.nd = nd;
         m.adds = a;
@@ -618,6 +658,7 @@ struct cig_engine {
         m.hkeys = hkeys.p;
         m.hcnts = hcnts.p;
         m.hcnt = hcnt.p;
+        m.dclass = dclass.p;
         m.dtiles = dtiles.p;
         return m;
     }
@@ -630,9 +671,13 @@ struct cig_engine {
     }
 
     void mutate_apply(const MutArgs &m) {
-        k_del_rows<<<grid_small, kBlock, 0, stream>>>(m);
-        k_del_scan<<<grid_small, 256, 0, stream>>>(m);
-        k_del_fix<<<grid_small, 256, 0, stream>>>(m);
+        if (m.nd) {
+            k_del_lane<<<grid_small, 256, 0, stream>>>(m);
+            k_del_wave<<<grid_small, kBlock, 0, stream>>>(m);
+            k_del_scan<<<grid_small, 256, 0, stream>>>(m);
+            k_del_fix_rows<<<grid_small, 256, 0, stream>>>(m);
+            k_del_fix<<<grid_small, 256, 0, stream>>>(m);
+        }
         k_add_plan<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
         k_add_copy<<<grid_small, kBlock, 0, stream>>>(m);
         HIPCHK(hipGetLastError());
@@ -655,8 +700,8 @@ struct cig_engine {
         for (uint32_t i = 0; i < nover; ++i)
             HIPCHK(hipMemcpyAsync(&cnts[i], cnt64.p + list[i], sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
         sync();
-        std::vector<Row> hrows(n);
-        HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+        std::vector<Row> hrows(nrows);
+        HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
         uint64_t need[2] = {0, 0};
         for (uint32_t i = 0; i < nover; ++i) {
             const Row &r = hrows[list[i]];
@@ -671,7 +716,7 @@ struct cig_engine {
         HIPCHK(hipMemcpyAsync(&ctl.p->overflow_cnt, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(&ctl.p->copy_cnt, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         MutArgs mm = mut_args(m.dels, m.nd, m.adds, m.na);  // pools/rows may have moved
-        ensure_pieces(m.nd, m.na);
+        ensure_pieces(uint64_t(m.nd) << m.rev, uint64_t(m.na) << m.rev);
         mm.pieces = pieces.p;
         k_add_plan<<<grid_small, 256, 0, stream>>>(mm, add_list.p, &ctl.p->add_cnt);
         k_add_copy<<<grid_small, kBlock, 0, stream>>>(mm);
@@ -687,8 +732,8 @@ struct cig_engine {
     void compact_pool(int tier, uint64_t extra) {
         reduce_stats();  // garbage counted so far lands in the control block first
         sync();
-        std::vector<Row> hrows(n);
-        HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+        std::vector<Row> hrows(nrows);
+        HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
         auto in_tier = [&](const Row &r) { return ((r.off & kHostTier) != 0) == (tier == 1); };
         uint64_t live = 0;
         for (const Row &r : hrows)
@@ -701,20 +746,20 @@ struct cig_engine {
         }
         uint64_t cap = live + extra + live / 4 + (uint64_t(1) << 20);
         if (tier == 0 && budget_slots) cap = std::max<uint64_t>(budget_slots, live + extra);
-        std::vector<Row> nrows = hrows;
+        std::vector<Row> moved = hrows;
         uint64_t tail = 0;
-        for (uint32_t v = 0; v < n; ++v) {
+        for (uint32_t v = 0; v < nrows; ++v) {
             if (!in_tier(hrows[v])) continue;
-            nrows[v].off = tail | (tier ? kHostTier : 0ull);
+            moved[v].off = tail | (tier ? kHostTier : 0ull);
             tail += hrows[v].cap;
         }
         if (tier == 0) {
             DBuf<uint32_t> np;
             np.alloc(cap);
             DBuf<Row> drows;
-            drows.alloc(n);
-            HIPCHK(hipMemcpy(drows.p, nrows.data(), n * sizeof(Row), hipMemcpyHostToDevice));
-            k_relocate_rows<<<grid_small, 256, 0, stream>>>(rows.p, drows.p, n, 0, hbm_pool.p, np.p);
+            drows.alloc(nrows);
+            HIPCHK(hipMemcpy(drows.p, moved.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice));
+            k_relocate_rows<<<grid_small, 256, 0, stream>>>(rows.p, drows.p, nrows, 0, hbm_pool.p, np.p);
             HIPCHK(hipGetLastError());
             sync();
             std::swap(hbm_pool.p, np.p);
@@ -724,12 +769,12 @@ struct cig_engine {
         } else {
             HBuf<uint32_t> np;
             np.alloc(cap);
-            for (uint32_t v = 0; v < n; ++v) {
+            for (uint32_t v = 0; v < nrows; ++v) {
                 if (!in_tier(hrows[v]) || !hrows[v].deg) continue;
-                std::memcpy(np.p + (nrows[v].off & ~kHostTier), host_pool.p + (hrows[v].off & ~kHostTier),
+                std::memcpy(np.p + (moved[v].off & ~kHostTier), host_pool.p + (hrows[v].off & ~kHostTier),
                             size_t(hrows[v].deg) * sizeof(uint32_t));
             }
-            HIPCHK(hipMemcpy(rows.p, nrows.data(), n * sizeof(Row), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(rows.p, moved.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice));
             std::swap(host_pool.p, np.p);
             std::swap(host_pool.count, np.count);
             host_pool_dev = host_pool.device_ptr();
@@ -747,10 +792,10 @@ struct cig_engine {
     // Moves the coldest HBM rows (lowest activity score, then shortest) to the
     // host tier until `slots` are freed; updates hrows and the device rows.
     void demote_for_space(std::vector<Row> &hrows, uint64_t slots) {
-        std::vector<uint32_t> sc(n);
+        std::vector<uint32_t> sc(nrows, 0u);  // incoming rows score 0: they go first
         HIPCHK(hipMemcpy(sc.data(), score.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         std::vector<uint32_t> cand;
-        for (uint32_t v = 0; v < n; ++v)
+        for (uint32_t v = 0; v < nrows; ++v)
             if (!(hrows[v].off & kHostTier) && hrows[v].cap) cand.push_back(v);
         std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
             return sc[a] != sc[b] ? sc[a] < sc[b] : hrows[a].deg < hrows[b].deg;
@@ -774,7 +819,7 @@ struct cig_engine {
         uint64_t cap = hctl.p->pool_cap[tier];
         if (tail + need > cap) {
             compact_pool(tier, need);
-            HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
             read_ctl();
             tail = hctl.p->pool_tail[tier];
             cap = hctl.p->pool_cap[tier];
@@ -801,7 +846,7 @@ struct cig_engine {
         k_move_rows<<<grid_small, 256, 0, stream>>>(dj.p, dcount.p, pools());
         HIPCHK(hipGetLastError());
         sync();
-        HIPCHK(hipMemcpy(rows.p, hrows.data(), n * sizeof(Row), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(rows.p, hrows.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(&ctl.p->pool_tail[tier], &tail, sizeof(uint64_t), hipMemcpyHostToDevice));
         uint64_t g = 0;
         for (uint32_t v : vs) g += hrows[v].cap;
@@ -865,6 +910,30 @@ struct cig_engine {
         }
     }
 
+    template <int W, bool S, int QPL>
+    void repair_pull_qpl(const InvalArgs &ia) {
+        InvalArgs a = ia;
+        a.cfg = ecfg(kStageRepair);
+        HIPCHK(hipEventRecord(ev[8], stream));
+        k_repair_pull<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
+        HIPCHK(hipEventRecord(ev[9], stream));
+        scan_timed = true;
+        k_repair_pull_tiles<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
+        HIPCHK(hipGetLastError());
+    }
+
+    // Deletion repair: pull over the incoming rows of the affected vertices,
+    // or (without the incoming index) a push scan over every row.
+    template <int W, bool S>
+    void repair(const InvalArgs &ia) {
+        if (!rev) return repair_scan<W, S>(ia);
+        switch (qpl) {
+        case 1: repair_pull_qpl<W, S, 1>(ia); break;
+        case 4: repair_pull_qpl<W, S, 4>(ia); break;
+        default: repair_pull_qpl<W, S, 2>(ia); break;
+        }
+    }
+
     template <int W, bool S>
     void insert_seed(const InvalArgs &ia, bool skip_affected) {
         k_insert_seed<W, S><<<grid_small, 256, 0, stream>>>(ia, skip_affected ? 1u : 0u);
@@ -882,8 +951,11 @@ struct cig_engine {
             launch_coop(kernel, a, grid);
         } else {
             k_inval_begin<<<1, 64, 0, stream>>>(ctl.p);
-            run_chain([&](uint64_t tag) { k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag); },
-                      round_bound(), "invalidation");
+            run_chain(
+                [&](uint64_t tag, uint32_t seq) {
+                    k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                },
+                round_bound(), "invalidation");
         }
         k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
         HIPCHK(hipGetLastError());
@@ -908,7 +980,7 @@ struct cig_engine {
         InvalArgs ia = inval_args(d_dels, nd, d_adds, na);
         if (nd) invalidate_and_reset(ia);
         HIPCHK(hipEventRecord(ev[2], stream));
-        if (nd) repair_scan<W, S>(ia);
+        if (nd) repair<W, S>(ia);
         HIPCHK(hipEventRecord(ev[3], stream));
         if (na) insert_seed<W, S>(ia, nd != 0);
         HIPCHK(hipEventRecord(ev[5], stream));
@@ -931,7 +1003,7 @@ struct cig_engine {
         mutate_apply(m);
         InvalArgs ia = inval_args(d_dels, nd, nullptr, 0);
         invalidate_and_reset(ia);
-        repair_scan<W, S>(ia);
+        repair<W, S>(ia);
         relax();
         k_clear_affected<<<grid_small, 256, 0, stream>>>(ia);
         mutate_finish(m);
@@ -1056,9 +1128,11 @@ struct cig_engine {
 
     void apply_device(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, uint32_t flags,
                       cig_batch_stats *st) {
-        if (grouped.count < uint64_t(nd) + na) grouped.alloc(uint64_t(nd) + na);
-        ensure_delete_space(nd);
-        ensure_pieces(nd, na);
+        const uint64_t mirror = rev ? 2 : 1;  // the incoming rows take every update once more
+        if ((uint64_t(nd) + na) * mirror > 0xFFFFFFFFull) throw HipError(CIG_ELIMIT, "batch too large");
+        if (grouped.count < (uint64_t(nd) + na) * mirror) grouped.alloc((uint64_t(nd) + na) * mirror);
+        ensure_delete_space(nd * mirror);
+        ensure_pieces(nd * mirror, na * mirror);
         if (is_pr()) pr_batch(d, nd, a, na, st);
         else traversal_call(d, nd, a, na, (flags & CIG_BATCH_STAGED) != 0, st);
         if (st) st->pool_compactions = compactions;
@@ -1082,7 +1156,7 @@ struct cig_engine {
         next_batch = 0;
         deletes_done = false;
         add_off = del_off = 0;
-        if (grouped.count < max_batch) grouped.alloc(max_batch);
+        if (grouped.count < max_batch * (rev ? 2 : 1)) grouped.alloc(max_batch * (rev ? 2 : 1));
     }
 
     void apply_next(uint32_t flags, cig_batch_stats *st) {
@@ -1123,15 +1197,16 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
         if (budget_slots) {
             sync();
-            std::vector<uint32_t> sc(n);
-            std::vector<Row> hrows(n);
+            // incoming rows rank after every outgoing row (score 0, larger ids)
+            std::vector<uint32_t> sc(nrows, 0u);
+            std::vector<Row> hrows(nrows);
             HIPCHK(hipMemcpy(sc.data(), score.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
-            std::vector<uint32_t> order(n);
+            HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
+            std::vector<uint32_t> order(nrows);
             std::iota(order.begin(), order.end(), 0u);
             std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sc[a] > sc[b]; });
             const uint64_t limit = budget_slots * 9 / 10;
-            std::vector<uint8_t> admit(n, 0);
+            std::vector<uint8_t> admit(nrows, 0);
             uint64_t used = 0;
             for (uint32_t v : order) {
                 if (!hrows[v].cap) continue;
@@ -1141,7 +1216,7 @@ struct cig_engine {
             }
             std::vector<uint32_t> evict, load;
             uint64_t evict_need = 0, load_need = 0;
-            for (uint32_t v = 0; v < n; ++v) {
+            for (uint32_t v = 0; v < nrows; ++v) {
                 const bool cached = !(hrows[v].off & kHostTier);
                 if (!hrows[v].cap) continue;
                 if (cached && !admit[v]) { evict.push_back(v); evict_need += hrows[v].cap; }
@@ -1152,7 +1227,7 @@ struct cig_engine {
                 read_ctl();
                 if (hctl.p->pool_tail[0] + load_need > hctl.p->pool_cap[0]) {
                     compact_pool(0, load_need);
-                    HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
                 }
                 move_rows(hrows, load, 0, load_need);
             }
@@ -1194,11 +1269,14 @@ struct cig_engine {
         for (uint32_t v = 0; v < n; ++v) o[v] = field == CIG_FIELD_VALUE ? uint32_t(hs[v] >> 32) : uint32_t(hs[v]);
     }
 
-    void gather_topology(uint64_t *offs, uint32_t *dst, uint64_t capacity) {
+    // Rows [first, first+n) as CSR: the outgoing rows (first = 0) or the
+    // incoming index (first = n).
+    void gather_topology(uint64_t *offs, uint32_t *dst, uint64_t capacity, uint32_t first = 0) {
         if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        if (first && !rev) throw HipError(CIG_ESTATE, "this engine keeps no incoming-row index");
         sync();
         std::vector<Row> hrows(n);
-        HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hrows.data(), rows.p + first, n * sizeof(Row), hipMemcpyDeviceToHost));
         read_ctl();
         const uint64_t used0 = hctl.p->pool_tail[0];
         std::vector<uint32_t> hp(std::min<uint64_t>(used0, hbm_slots));
@@ -1348,6 +1426,13 @@ cig_status cig_gather_topology(cig_engine *e, uint64_t *offsets, uint32_t *dst, 
     return guarded(e, [&] {
         if (!offsets || (!dst && cap)) throw HipError(CIG_EINVAL, "null output");
         e->gather_topology(offsets, dst, cap);
+    });
+}
+
+cig_status cig_gather_incoming(cig_engine *e, uint64_t *offsets, uint32_t *src, uint64_t cap) {
+    return guarded(e, [&] {
+        if (!offsets || (!src && cap)) throw HipError(CIG_EINVAL, "null output");
+        e->gather_topology(offsets, src, cap, e->n);
     });
 }
 

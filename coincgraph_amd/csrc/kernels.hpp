@@ -11,6 +11,10 @@ namespace dev {
 
 namespace cg = cooperative_groups;
 
+// LDS staging of the round kernels: 2048 frontier items and 128 tiles per
+// workgroup (19 KB with the wave slots of a 256-thread workgroup).
+using RoundStage = StageLds<2048, 128>;
+
 __device__ __forceinline__ uint64_t next_pow2_dev(uint64_t d) {
     uint64_t c = d < kMinRowCap ? kMinRowCap : d;
     return c <= 1 ? 1 : (1ull << (64 - __builtin_clzll(c - 1)));
@@ -33,6 +37,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     c->acount = 0;
     c->icount[0] = c->icount[1] = c->icount[2] = 0;
+    c->ifront[0] = c->ifront[1] = c->ifront[2] = 0;
     c->scount = 0;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
@@ -41,6 +46,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
     c->group_cursor = 0;
     c->edges_scanned = c->host_edges = c->vexp = 0;
     c->deletes_applied = c->missing = c->invalid = c->adds_applied = c->relocated = 0;
+    c->touched_sources = 0;
     c->rounds = 0;
     c->residual_l1 = 0;
     c->converged = 0;
@@ -101,6 +107,7 @@ struct MutArgs {
     Row *rows;
     Pools pools;
     uint32_t n;
+    uint32_t rev;      // 1: rows [n, 2n) hold the incoming rows and get every update mirrored
     const cig_edge *dels;
     uint32_t nd;
     const cig_edge *adds;
@@ -119,18 +126,36 @@ struct MutArgs {
     uint32_t *hkeys;   // delete tables: destination + 1 (0 = empty)
     uint2 *hcnts;      // delete tables: (requested, claimed)
     uint32_t *hcnt;    // per vertex: holes found (zero between calls)
+    uint8_t *dclass;   // per row: delete class of this call (kDelClass*)
     uint2 *dtiles;     // (row, first slot) of each delete scan tile
     struct AddPiece *pieces;  // copy pieces of the insert phase (AddPiece below)
     StatShard *stats;
 };
 
-// Rows with more than 64 deletes, or longer than 4096 slots, are deleted
-// from in tiles of kDelTile slots spread over the whole grid. Their delete
-// multisets live in open-addressing tables of 2^bits >= 2*nd slots.
+// Deletes find their slot by row length (BuildDeletionPlan,
+// source_local_chunk_store.h:670-723, restated for the device):
+//  - tiny rows (<= 64 slots): one lane per delete record reads the whole row
+//    and claims a matching slot with a CAS (slot := kNone);
+//  - medium rows (<= 4096 slots): one wave per delete record, 1024 slots a
+//    step, same claim;
+//  - long rows: tiles of kDelTile slots over the whole grid against the row's
+//    delete multiset in an open-addressing table of 2^bits >= 2*nd slots.
+// Every claimed slot is recorded in the row's hole list (the row's group,
+// whose delete records are not read again). Rows with at most 16 deletes
+// (and no table) are then compacted by one thread, the others by a
+// workgroup: live slots of the last k positions move into the holes below
+// deg-k. Equal destinations are interchangeable, so the survivor multiset is
+// the reference's; their order may differ (CompactSourceInPlace,
+// source_local_chunk_store.h:733-780, keeps it).
 constexpr uint32_t kDelTile = 4096;
-constexpr uint32_t kDelLdsKeys = 8192;
+constexpr uint32_t kDelLdsKeys = 2048;
+constexpr uint32_t kDelTinyDeg = 64, kDelMediumDeg = 4096, kDelThreadFix = 16;
+enum : uint8_t { kDelClassNone = 0, kDelClassTiny = 1, kDelClassMedium = 2, kDelClassLong = 3 };
 
-__device__ __forceinline__ bool del_is_small(uint32_t nd, uint32_t deg) { return nd <= 64 && deg <= 4096; }
+__device__ __forceinline__ uint8_t del_class(uint32_t nd, uint32_t deg) {
+    if (nd == 0) return kDelClassNone;
+    return deg <= kDelTinyDeg ? kDelClassTiny : deg <= kDelMediumDeg ? kDelClassMedium : kDelClassLong;
+}
 
 __device__ __forceinline__ uint32_t del_table_bits(uint32_t nd) {
     uint32_t bits = 4;
@@ -142,95 +167,136 @@ __device__ __forceinline__ uint32_t ht_hash(uint32_t x, uint32_t bits) {
     return (x * 0x9E3779B1u) >> (32 - bits);
 }
 
-// GroupedUpdateBatch (effective_update_batch.h:35-87) on the device: count
-// records per source and collect the touched sources (one list reservation
-// per workgroup step).
-__global__ void k_count(MutArgs a) {
-    __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
+// Record i of a mutation call: deletes then adds; with the incoming index on,
+// records [nd+na, 2(nd+na)) are the same updates mirrored onto row n+v with
+// destination u.
+struct MutRecord {
+    uint32_t row, dst;
+    bool del, valid, mirror;
+};
+
+__device__ __forceinline__ MutRecord mut_record(const MutArgs &a, uint32_t i) {
     const uint32_t total = a.nd + a.na;
-    uint64_t bad_total = 0;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gthreads()) {
-        const uint32_t i = base + threadIdx.x;
-        bool first = false;
-        uint32_t u = 0;
-        bool bad = false;
-        if (i < total) {
-            const bool del = i < a.nd;
-            const cig_edge e = del ? a.dels[i] : a.adds[i - a.nd];
-            if (e.u >= a.n || e.v >= a.n) {
-                bad = true;
-            } else {
-                u = e.u;
-                const uint64_t old = atomicAdd((unsigned long long *)&a.cnt64[u],
-                                               del ? 1ull : (1ull << 32));
-                first = old == 0;
-            }
-        }
-        bad_total += uint32_t(__popcll(__ballot(bad)));
-        block_append(first, u, a.touched, &a.ctl->touched, scratch);
-    }
-    stat_add(&stat_shard(a.stats)->invalid, bad_total);
+    MutRecord r;
+    r.mirror = i >= total;
+    const uint32_t j = r.mirror ? i - total : i;
+    r.del = j < a.nd;
+    const cig_edge e = r.del ? a.dels[j] : a.adds[j - a.nd];
+    r.valid = e.u < a.n && e.v < a.n;
+    r.row = r.mirror ? a.n + e.v : e.u;
+    r.dst = r.mirror ? e.u : e.v;
+    return r;
 }
 
-// Reserves each touched source's group and classifies its work; long-row
-// deletes also reserve their table and scan tiles. Every cursor and list
-// takes one atomic per workgroup step.
-__global__ void k_plan(MutArgs a) {
+__device__ __forceinline__ uint32_t mut_records(const MutArgs &a) { return (a.nd + a.na) << a.rev; }
+
+// Delete record i of [0, nd << rev): the deletes, then their mirrors.
+__device__ __forceinline__ MutRecord del_record(const MutArgs &a, uint32_t i) {
+    return mut_record(a, i < a.nd ? i : i - a.nd + a.nd + a.na);
+}
+
+// GroupedUpdateBatch (effective_update_batch.h:35-87) on the device: count
+// records per row and collect the touched rows. A thread takes four records
+// a step, so the touched-list reservation is one atomic per 1024 records.
+__global__ void k_count(MutArgs a) {
     __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
-    __shared__ unsigned long long scratch64[kMaxWavesPerBlock + 1];
+    constexpr uint32_t R = 4;
+    const uint32_t total = mut_records(a);
+    uint64_t bad_total = 0, fwd_touched = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x * R; base < total; base += gthreads() * R) {
+        uint32_t u[R];
+        bool first[R];
+        uint32_t nf = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k) {
+            const uint32_t i = base + k * blockDim.x + threadIdx.x;
+            first[k] = false;
+            u[k] = 0;
+            bool bad = false;
+            if (i < total) {
+                const MutRecord r = mut_record(a, i);
+                if (!r.valid) {
+                    bad = !r.mirror;
+                } else {
+                    u[k] = r.row;
+                    const uint64_t old = atomicAdd((unsigned long long *)&a.cnt64[r.row],
+                                                   r.del ? 1ull : (1ull << 32));
+                    first[k] = old == 0;
+                }
+            }
+            bad_total += uint32_t(__popcll(__ballot(bad)));
+            fwd_touched += uint32_t(__popcll(__ballot(first[k] && u[k] < a.n)));
+            nf += first[k] ? 1u : 0u;
+        }
+        uint32_t at = block_reserve<uint32_t>(nf, &a.ctl->touched, scratch);
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k)
+            if (first[k]) a.touched[at++] = u[k];
+    }
+    StatShard *sh = stat_shard(a.stats);
+    stat_add(&sh->invalid, bad_total);
+    stat_add(&sh->touched, fwd_touched);
+}
+
+// Reserves each touched row's group and classifies its work; long rows
+// with deletes also reserve their table and scan tiles. All six cursors are
+// reserved together, one round trip per workgroup step.
+__global__ void k_plan(MutArgs a) {
+    __shared__ uint32_t scratch[6 * (kMaxWavesPerBlock + 1)];
     const uint32_t touched = ld_acquire(&a.ctl->touched);
+    uint32_t *const ctr[6] = {&a.ctl->group_cursor, &a.ctl->ht_cursor, &a.ctl->dtile_cnt,
+                              &a.ctl->small_cnt, &a.ctl->big_cnt, &a.ctl->add_cnt};
     for (uint32_t base = blockIdx.x * blockDim.x; base < touched; base += gthreads()) {
         const uint32_t t = base + threadIdx.x;
-        bool small = false, big = false, add = false;
-        uint32_t u = 0, len = 0, nd = 0, na = 0, deg = 0;
+        uint32_t u = 0, nd = 0, na = 0, deg = 0;
+        uint8_t cls = kDelClassNone;
         if (t < touched) {
             u = a.touched[t];
             const uint64_t c = a.cnt64[u];
             nd = uint32_t(c);
             na = uint32_t(c >> 32);
-            len = nd + na;
             deg = a.rows[u].deg;
-            // one wave scans a row of up to 4096 slots for up to 64 deletes;
-            // anything larger is cut into grid-wide tiles
-            small = nd != 0 && del_is_small(nd, deg);
-            big = nd != 0 && !small;
-            add = na != 0;
+            cls = del_class(nd, deg);
+            a.dclass[u] = cls;
         }
-        const uint64_t g = block_reserve<unsigned long long>(len, (unsigned long long *)&a.ctl->group_cursor, scratch64);
-        if (t < touched) a.gofs[u] = g;
-        const uint32_t tsz = big ? (1u << del_table_bits(nd)) : 0u;
-        const uint32_t ntl = big ? (deg + kDelTile - 1) / kDelTile : 0u;
-        const uint32_t hb = block_reserve<uint32_t>(tsz, &a.ctl->ht_cursor, scratch);
-        const uint32_t tb = block_reserve<uint32_t>(ntl, &a.ctl->dtile_cnt, scratch);
-        if (t < touched) a.htoff[u] = big ? hb : kNone;
-        for (uint32_t j = 0; j < ntl; ++j) a.dtiles[tb + j] = make_uint2(u, j * kDelTile);
-        block_append(small, u, a.small_list, &a.ctl->small_cnt, scratch);
-        block_append(big, u, a.big_list, &a.ctl->big_cnt, scratch);
-        block_append(add, u, a.add_list, &a.ctl->add_cnt, scratch);
+        const bool table = cls == kDelClassLong;
+        const bool thread_fix = nd != 0 && nd <= kDelThreadFix && !table;
+        const uint32_t x[6] = {nd + na, table ? (1u << del_table_bits(nd)) : 0u,
+                               table ? (deg + kDelTile - 1) / kDelTile : 0u, thread_fix ? 1u : 0u,
+                               (nd != 0 && !thread_fix) ? 1u : 0u, na != 0 ? 1u : 0u};
+        uint32_t at[6];
+        block_reserve_n<6>(x, ctr, at, scratch);
+        if (t < touched) {
+            a.gofs[u] = at[0];
+            a.htoff[u] = table ? at[1] : kNone;
+            for (uint32_t j = 0; j < x[2]; ++j) a.dtiles[at[2] + j] = make_uint2(u, j * kDelTile);
+            if (x[3]) a.small_list[at[3]] = u;
+            if (x[4]) a.big_list[at[4]] = u;
+            if (x[5]) a.add_list[at[5]] = u;
+        }
     }
 }
 
 // Groups the records by source; deletes of long rows also enter their
 // row's table (destination -> requested count).
 __global__ void k_scatter(MutArgs a) {
-    const uint32_t total = a.nd + a.na;
+    const uint32_t total = mut_records(a);
     for (uint32_t i = gthread(); i < total; i += gthreads()) {
-        const bool del = i < a.nd;
-        const cig_edge e = del ? a.dels[i] : a.adds[i - a.nd];
-        if (e.u >= a.n || e.v >= a.n) continue;
-        const uint64_t pos = atomicAdd((unsigned long long *)&a.fill64[e.u],
-                                       del ? 1ull : (1ull << 32));
-        const uint32_t nd = uint32_t(a.cnt64[e.u]);
-        const uint32_t slot = del ? uint32_t(pos) : nd + uint32_t(pos >> 32);
-        a.grouped[a.gofs[e.u] + slot] = e.v;
-        const uint32_t to = del ? a.htoff[e.u] : kNone;
+        const MutRecord r = mut_record(a, i);
+        if (!r.valid) continue;
+        const uint64_t pos = atomicAdd((unsigned long long *)&a.fill64[r.row],
+                                       r.del ? 1ull : (1ull << 32));
+        const uint32_t nd = uint32_t(a.cnt64[r.row]);
+        const uint32_t slot = r.del ? uint32_t(pos) : nd + uint32_t(pos >> 32);
+        a.grouped[a.gofs[r.row] + slot] = r.dst;
+        const uint32_t to = r.del ? a.htoff[r.row] : kNone;
         if (to != kNone) {
             // at most nd distinct keys in 2^bits >= 2*nd slots: the probe ends
             const uint32_t bits = del_table_bits(nd), mask = (1u << bits) - 1u;
-            uint32_t h = ht_hash(e.v, bits);
+            uint32_t h = ht_hash(r.dst, bits);
             for (;;) {
-                const uint32_t prev = atomicCAS(&a.hkeys[to + h], 0u, e.v + 1u);
-                if (prev == 0u || prev == e.v + 1u) break;
+                const uint32_t prev = atomicCAS(&a.hkeys[to + h], 0u, r.dst + 1u);
+                if (prev == 0u || prev == r.dst + 1u) break;
                 h = (h + 1u) & mask;
             }
             atomicAdd(&a.hcnts[to + h].x, 1u);
@@ -238,83 +304,116 @@ __global__ void k_scatter(MutArgs a) {
     }
 }
 
-// Deletes for a row of at most 4096 slots with at most 64 delete records,
-// one wave per row. Lane j holds delete j; the row streams through 64*4 slots at a
-// time and each pending delete claims the first live occurrence in row order
-// (BuildDeletionPlan, source_local_chunk_store.h:670-723); the scan stops once
-// every delete found its edge. The k holes are then refilled from the row's
-// last k slots, so the row is rewritten in O(k) stores instead of the
-// reference's order-preserving compaction (CompactSourceInPlace, :733-780):
-// the multiset of survivors is the same, their order may differ.
-__global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
-    __shared__ uint32_t slot_lds[kWavesPerBlock][kWave];
-    uint32_t *slots = slot_lds[wave_in_block()];
-    const uint32_t cnt = ld_acquire(&a.ctl->small_cnt);
+// Records a claimed slot in its row's hole list.
+__device__ __forceinline__ void del_hole(const MutArgs &a, uint32_t row, uint32_t slot) {
+    a.grouped[a.gofs[row] + atomicAdd(&a.hcnt[row], 1u)] = slot;
+}
+
+// Tiny rows: one lane per delete record.
+__global__ void k_del_lane(MutArgs a) {
+    const uint32_t total = a.nd << a.rev;
+    for (uint32_t i = gthread(); i < total; i += gthreads()) {
+        const MutRecord r = del_record(a, i);
+        if (!r.valid || a.dclass[r.row] != kDelClassTiny) continue;
+        const Row row = a.rows[r.row];
+        if (row.deg == 0) continue;  // nothing to delete: a missing record
+        uint32_t *p = a.pools.row(row.off);
+        const uint32_t nq = (row.deg + 3u) >> 2;  // 1..16
+        uint32_t d[kDelTinyDeg / 4][4], live[kDelTinyDeg / 4];
+#pragma unroll
+        for (uint32_t q = 0; q < kDelTinyDeg / 4; ++q) {
+            live[q] = load_quad(p, q < nq ? q : nq - 1, row.deg, d[q]);
+            if (q >= nq) live[q] = 0;
+        }
+        bool done = false;
+#pragma unroll
+        for (uint32_t q = 0; q < kDelTinyDeg / 4; ++q)
+#pragma unroll
+            for (uint32_t s = 0; s < 4; ++s)
+                if (!done && s < live[q] && d[q][s] == r.dst && atomicCAS(&p[4 * q + s], r.dst, kNone) == r.dst) {
+                    del_hole(a, r.row, 4 * q + s);
+                    done = true;
+                }
+    }
+}
+
+// Medium rows: one wave per delete record, 1024 slots a step, stopping at the
+// first slot it claims.
+__global__ __launch_bounds__(kBlock) void k_del_wave(MutArgs a) {
+    const uint32_t total = a.nd << a.rev;
     const uint32_t lane = lane_id();
-    uint64_t acc_applied = 0, acc_missing = 0;
-    constexpr int U = 4;
-    for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
-        const uint32_t u = a.small_list[j];
-        const Row r = a.rows[u];
-        const uint32_t nd = uint32_t(a.cnt64[u]);
-        const uint32_t *dl = a.grouped + a.gofs[u];
-        const uint32_t mine = lane < nd ? dl[lane] : kNone;
-        const uint64_t all = nd >= 64 ? ~0ull : ((1ull << nd) - 1);
-        uint64_t pend = all;
-        uint32_t hole = kNone;
-        uint32_t *row = a.pools.row(r.off);
-        for (uint32_t base = 0; base < r.deg && pend; base += kWave * U) {
-            uint32_t e[U];
-            bool live[U];
+    for (uint32_t i = gwave(); i < total; i += gwaves()) {
+        const MutRecord r = del_record(a, i);
+        if (!r.valid || a.dclass[r.row] != kDelClassMedium) continue;  // wave-uniform
+        const Row row = a.rows[r.row];
+        uint32_t *p = a.pools.row(row.off);
+        const uint32_t nq = (row.deg + 3u) >> 2;
+        bool done = false;
+        for (uint32_t q0 = 0; q0 < nq && !done; q0 += kWave * 4) {
+            uint32_t d[4][4], live[4], m = 0;
 #pragma unroll
-            for (int q = 0; q < U; ++q) {
-                const uint32_t p = base + uint32_t(q) * kWave + lane;
-                live[q] = p < r.deg;
-                e[q] = live[q] ? row[p] : kNone;
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t q = q0 + k * kWave + lane;
+                live[k] = load_quad(p, q < nq ? q : nq - 1, row.deg, d[k]);
+                if (q >= nq) live[k] = 0;
+#pragma unroll
+                for (uint32_t s = 0; s < 4; ++s)
+                    if (s < live[k] && d[k][s] == r.dst) m |= 1u << (4 * k + s);
             }
-            for (uint64_t todo = pend; todo; todo &= todo - 1) {
-                const int jj = __ffsll((unsigned long long)todo) - 1;
-                const uint32_t d = __shfl(mine, jj, kWave);
-#pragma unroll
-                for (int q = 0; q < U; ++q) {
-                    const uint64_t m = __ballot(live[q] && e[q] == d);
-                    if (m) {
-                        const uint32_t l = uint32_t(__ffsll((unsigned long long)m) - 1);
-                        if (lane == l) live[q] = false;
-                        if (lane == uint32_t(jj)) hole = base + uint32_t(q) * kWave + l;
-                        pend &= ~(1ull << jj);
-                        break;
-                    }
+            for (;;) {
+                const uint64_t any = __ballot(m != 0u);
+                if (any == 0) break;
+                const uint32_t l = uint32_t(__ffsll((unsigned long long)any) - 1);
+                uint32_t ok = 0;
+                if (lane == l) {
+                    const uint32_t b = uint32_t(__ffs(m) - 1);
+                    const uint32_t slot = 4u * (q0 + (b >> 2) * kWave + lane) + (b & 3u);
+                    ok = atomicCAS(&p[slot], r.dst, kNone) == r.dst ? 1u : 0u;
+                    if (ok) del_hole(a, r.row, slot);
+                    m &= m - 1u;
+                }
+                if (__shfl(ok, int(l), kWave)) {
+                    done = true;
+                    break;
                 }
             }
         }
-        const uint64_t matched = all & ~pend;
-        const uint32_t k = uint32_t(__popcll(matched));
+    }
+}
+
+// Compaction of rows with at most kDelThreadFix deletes, one thread per row.
+__global__ void k_del_fix_rows(MutArgs a) {
+    const uint32_t cnt = ld_acquire(&a.ctl->small_cnt);
+    uint64_t applied = 0, missing = 0;
+    for (uint32_t t = gthread(); t < cnt; t += gthreads()) {
+        const uint32_t u = a.small_list[t];
+        const uint32_t nd = uint32_t(a.cnt64[u]);
+        const uint32_t k = a.hcnt[u];
         if (k) {
+            const Row r = a.rows[u];
+            uint32_t *row = a.pools.row(r.off);
+            const uint32_t *hl = a.grouped + a.gofs[u];
             const uint32_t newdeg = r.deg - k;
-            // tail slot newdeg+lane (lane < k): dead if it is one of the holes
-            bool tail_dead = false;
-            for (uint64_t mm = matched; mm; mm &= mm - 1) {
-                const uint32_t h = __shfl(hole, __ffsll((unsigned long long)mm) - 1, kWave);
-                tail_dead |= (lane < k) && h == newdeg + lane;
+            // as many holes lie below newdeg as live slots lie at or above it
+            uint32_t hi = 0;
+            for (uint32_t p = newdeg; p < r.deg; ++p) {
+                const uint32_t v = row[p];
+                if (v == kNone) continue;
+                while (hi < k && hl[hi] >= newdeg) ++hi;
+                if (hi == k) break;  // unreachable while the hole list is exact
+                row[hl[hi++]] = v;
             }
-            const bool tail_live = lane < k && !tail_dead;
-            const uint32_t tval = tail_live ? row[newdeg + lane] : 0u;
-            const bool front = ((matched >> lane) & 1ull) && hole < newdeg;
-            const uint64_t fm = __ballot(front);
-            const uint64_t lm = __ballot(tail_live);
-            if (front) slots[__popcll(fm & __lanemask_lt())] = hole;
-            wave_lds_sync();
-            if (tail_live) row[slots[__popcll(lm & __lanemask_lt())]] = tval;
-            wave_lds_sync();
-            if (lane == 0) a.rows[u].deg = newdeg;
+            a.rows[u].deg = newdeg;
+            a.hcnt[u] = 0;
         }
-        acc_applied += k;
-        acc_missing += nd - k;
+        if (u < a.n) {
+            applied += k;
+            missing += nd - k;
+        }
     }
     StatShard *sh = stat_shard(a.stats);
-    stat_add(&sh->deletes_applied, acc_applied);
-    stat_add(&sh->missing, acc_missing);
+    stat_add(&sh->deletes_applied, wave_sum64(applied));
+    stat_add(&sh->missing, wave_sum64(missing));
 }
 
 // Exclusive scan of a 0/1 flag over a 256-thread workgroup; *total gets the
@@ -334,13 +433,10 @@ __device__ __forceinline__ uint32_t block_flag_scan(bool f, uint32_t *wsum, uint
     return before + uint32_t(__popcll(m & __lanemask_lt()));
 }
 
-// Long-row deletes, scan phase: one workgroup per kDelTile-slot tile. A slot
-// whose destination is in the row's table claims one requested occurrence
-// (first claims win; equal destinations are interchangeable, so the survivor
-// multiset is the reference's, BuildDeletionPlan,
-// source_local_chunk_store.h:670-723); a claimed slot is overwritten with
-// kNone and its position is recorded in the row's group (whose delete
-// records are no longer needed once the table is built).
+// Long rows, scan phase: one workgroup per kDelTile-slot tile. A slot whose
+// destination is in the row's table claims one requested occurrence (first
+// claims win). The tile's row loads are issued before the table is staged
+// in LDS (tables of more than kDelLdsKeys slots are probed in place).
 __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
     __shared__ uint32_t keys[kDelLdsKeys];
     const uint32_t nt = ld_acquire(&a.ctl->dtile_cnt);
@@ -352,52 +448,62 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
         const uint32_t nd = uint32_t(a.cnt64[u]);
         const uint32_t bits = del_table_bits(nd), mask = (1u << bits) - 1u;
         const uint32_t to = a.htoff[u];
-        const bool in_lds = (1u << bits) <= kDelLdsKeys;
+        uint32_t *row = a.pools.row(r.off) + start;  // start is a multiple of 4
+        const uint32_t len = min(kDelTile, r.deg - start);
+        const uint32_t nq = (len + 3u) >> 2;         // <= 4 * blockDim.x
+        uint32_t d[4][4], live[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t q = k * blockDim.x + tid;
+            live[k] = load_quad(row, q < nq ? q : nq - 1, len, d[k]);
+            if (q >= nq) live[k] = 0;
+        }
+        const bool in_lds = mask < kDelLdsKeys;
         __syncthreads();  // the previous tile is done with `keys`
         if (in_lds)
             for (uint32_t k = tid; k <= mask; k += blockDim.x) keys[k] = a.hkeys[to + k];
         __syncthreads();
-        const uint32_t *kt = in_lds ? keys : a.hkeys + to;
-        uint32_t *row = a.pools.row(r.off) + start;  // start is a multiple of 4
-        const uint32_t len = min(kDelTile, r.deg - start);
-        const uint32_t nq = (len + 3u) >> 2;
-        uint32_t *holes = a.grouped + a.gofs[u];
-        for (uint32_t q0 = 0; q0 < nq; q0 += blockDim.x * 4) {
-            uint32_t d[4][4], live[4];
+        const uint32_t *gkeys = a.hkeys + to;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t q = q0 + uint32_t(i) * blockDim.x + tid;
-                live[i] = q < nq ? load_quad(row, q, len, d[i]) : 0u;
-            }
+        for (uint32_t k = 0; k < 4; ++k) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t q = q0 + uint32_t(i) * blockDim.x + tid;
-                for (uint32_t sl = 0; sl < live[i]; ++sl) {
-                    const uint32_t x = d[i][sl];
-                    uint32_t h = ht_hash(x, bits);
-                    for (uint32_t probe = 0; probe <= mask; ++probe) {
-                        const uint32_t key = kt[h];
+            for (uint32_t s = 0; s < 4; ++s) {
+                if (s >= live[k]) continue;
+                const uint32_t x = d[k][s];
+                uint32_t h = ht_hash(x, bits), hit = kNone;
+                if (in_lds) {
+                    for (uint32_t probe = 0; probe <= mask; ++probe, h = (h + 1u) & mask) {
+                        const uint32_t key = keys[h];
                         if (key == 0u) break;
                         if (key == x + 1u) {
-                            if (atomicAdd(&a.hcnts[to + h].y, 1u) < a.hcnts[to + h].x) {
-                                row[4u * q + sl] = kNone;
-                                holes[atomicAdd(&a.hcnt[u], 1u)] = start + 4u * q + sl;
-                            }
+                            hit = h;
                             break;
                         }
-                        h = (h + 1u) & mask;
                     }
+                } else {
+                    for (uint32_t probe = 0; probe <= mask; ++probe, h = (h + 1u) & mask) {
+                        const uint32_t key = gkeys[h];
+                        if (key == 0u) break;
+                        if (key == x + 1u) {
+                            hit = h;
+                            break;
+                        }
+                    }
+                }
+                if (hit != kNone && atomicAdd(&a.hcnts[to + hit].y, 1u) < a.hcnts[to + hit].x) {
+                    const uint32_t slot = 4u * (k * blockDim.x + tid) + s;
+                    row[slot] = kNone;
+                    del_hole(a, u, start + slot);
                 }
             }
         }
     }
 }
 
-// Long-row deletes, fix-up phase: one workgroup per row. With k holes the
-// row shrinks to deg-k; the live slots of the last k positions move into the
-// holes below deg-k (O(k) stores; survivors keep their multiset, not their
-// order, unlike CompactSourceInPlace, source_local_chunk_store.h:733-780).
-// The row's table and hole counter are cleared for the next call.
+// Compaction of rows with more than kDelThreadFix deletes (or a table), one
+// workgroup per row: with k holes the row shrinks to deg-k and the live
+// slots of the last k positions move into the holes below deg-k (O(k)
+// stores). The row's table and hole counter are cleared for the next call.
 __global__ __launch_bounds__(256) void k_del_fix(MutArgs a) {
     __shared__ uint32_t wsum[256 / kWave];
     const uint32_t cnt = ld_acquire(&a.ctl->big_cnt);
@@ -432,11 +538,11 @@ __global__ __launch_bounds__(256) void k_del_fix(MutArgs a) {
             const bool live = v != kNone;
             uint32_t tot;
             const uint32_t rk = block_flag_scan(live, wsum, tot);
-            if (live) row[hl[run + rk]] = v;
+            if (live && run + rk < k) row[hl[run + rk]] = v;
             run += tot;
         }
-        // 3. clear the table for the next call
-        const uint32_t to = a.htoff[u], tsz = 1u << del_table_bits(nd);
+        // 3. clear the table (long rows) for the next call
+        const uint32_t to = a.htoff[u], tsz = to != kNone ? 1u << del_table_bits(nd) : 0u;
         for (uint32_t s = tid; s < tsz; s += blockDim.x) {
             a.hkeys[to + s] = 0u;
             a.hcnts[to + s] = make_uint2(0u, 0u);
@@ -445,8 +551,10 @@ __global__ __launch_bounds__(256) void k_del_fix(MutArgs a) {
             a.rows[u].deg = newdeg;
             a.hcnt[u] = 0u;
         }
-        applied += k;
-        missing += nd - k;
+        if (u < a.n) {
+            applied += k;
+            missing += nd - k;
+        }
         __syncthreads();
     }
     if (wave_in_block() == 0) {
@@ -526,11 +634,11 @@ __global__ void k_add_plan(MutArgs a, const uint32_t *list, const uint32_t *list
                 nr.cap = uint32_t(newcap);
                 a.rows[u] = nr;
                 acc_garbage[tier] += r.cap;
-                acc_reloc += 1;
+                if (u < a.n) acc_reloc += 1;
             } else {
                 a.rows[u].deg = uint32_t(newdeg);
             }
-            acc_adds += na;
+            if (u < a.n) acc_adds += na;
         }
     }
     StatShard *sh = stat_shard(a.stats);
@@ -657,41 +765,43 @@ __global__ __launch_bounds__(kBlock) void k_inval_loop(InvalArgs a) {
 // Opens the invalidation frontier after the seeds: [0, acount).
 __global__ void k_inval_begin(Ctl *c) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    c->ibegin = 0;
-    c->iend = c->acount;
     c->iround = 0;
+    c->istart[0] = 0;
+    c->ifront[0] = c->acount;
+    c->ifront[1] = c->ifront[2] = 0;
     c->icount[0] = c->icount[1] = c->icount[2] = 0;
 }
 
-// One invalidation wave as an ordinary launch: expands [ibegin, iend) of the
-// affected list; the closing workgroup moves the window to the items this
-// round appended.
-using RoundStage = StageLds<2048, 128>;
-
+// One invalidation wave as an ordinary launch, the seq-th of its chain:
+// round k expands its frontier alist[istart, istart+ifront) (and its
+// deferred tiles) and appends the next frontier right behind it. Slots
+// rotate mod 3: workgroup 0 opens slot k+1's start, clears slot k+2 and
+// posts the round's progress word at the start, so no workgroup waits for
+// another and the round needs no closing step.
 template <int QPL>
-__global__ __launch_bounds__(kBlock) void k_inval_round(InvalArgs a, uint64_t *prog, uint64_t tag) {
+__global__ __launch_bounds__(kBlock) void k_inval_round(InvalArgs a, uint64_t *prog, uint64_t tag, uint32_t seq) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ RoundStage stage;
     Ctl *c = a.ctl;
-    const uint32_t b = c->ibegin, e = c->iend, k = c->iround;
-    const uint32_t nt = c->icount[k % 3];
-    if (b == e && nt == 0) return post_idle(prog, tag);
+    const uint32_t k = c->iround + seq - 1;
+    const uint32_t start = c->istart[k % 3], cnt = c->ifront[k % 3], nt = c->icount[k % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->istart[(k + 1) % 3] = start + cnt;
+        c->ifront[(k + 2) % 3] = 0;
+        c->icount[(k + 2) % 3] = 0;
+        c->acount = start + cnt;  // affected so far; final once a round finds nothing
+        post_progress(prog, tag, cnt != 0 || nt != 0);
+    }
+    if (cnt == 0 && nt == 0) return;
     InvalidateOp op{a.rows, a.state, a.aff};
     Counters ctr;
-    const StagedOut<2048, 128> out{Sink{a.alist, &c->acount},
+    const StagedOut<2048, 128> out{Sink{a.alist + start + cnt, &c->ifront[(k + 1) % 3]},
                                    TileSink{(k & 1) ? a.t0 : a.t1, &c->icount[(k + 1) % 3]}, &stage};
     out_init(out);
-    expand_range<QPL>(op, a.cfg, a.alist + b, e - b, out, lds[wave_in_block()], ctr);
+    expand_range<QPL>(op, a.cfg, a.alist + start, cnt, out, lds[wave_in_block()], ctr);
     expand_tiles<QPL>(op, a.cfg, (k & 1) ? a.t1 : a.t0, nt, out);
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     out_flush(out);
-    close_round(c, [&] {
-        c->icount[k % 3] = 0;
-        c->ibegin = e;
-        c->iend = ld_acquire(&c->acount);
-        c->iround = k + 1;
-        post_progress(prog, tag, c->iend != e || c->icount[(k + 1) % 3] != 0);
-    });
 }
 
 // reset_affected_values (driver.cuh:418-433): affected vertices restart from
@@ -854,6 +964,64 @@ __global__ __launch_bounds__(kRepairBlock) void k_repair_tiles(InvalArgs a) {
     expand_tiles<QPL>(op, a.cfg, a.t0, nt, DirectOut{Sink{nullptr, &c->scount}, TileSink{nullptr, &c->scount}});
 }
 
+// -------------------------------------------------------------- pull repair
+// With the incoming index on, the repair reads only the incoming rows of the
+// affected vertices (GpuAffectedPullRelax, framework.cuh:137-204, over a
+// device-resident reverse index instead of the CPU-materialised one of
+// dynamic_reverse_index.h:223-274): y takes the minimum over its unaffected,
+// reached in-neighbours x of value(x) + w(x, y), ties to the smaller x. This
+// is the same candidate set the full scan offers y, so the state after the
+// repair is identical.
+template <int W, bool kStrict>
+struct PullOp {
+    const Row *in_rows;  // rows + n
+    uint64_t *state;
+    const uint32_t *aff;
+    uint64_t seed;
+    __device__ bool source(uint32_t y, uint32_t, Row &row, uint32_t &pay) const {
+        row = in_rows[y];
+        pay = y;
+        return row.deg != 0;
+    }
+    __device__ bool tile_source(const Tile &t, uint32_t &pay) const {
+        pay = t.v;
+        return true;
+    }
+    __device__ uint64_t peek(uint32_t x) const { return state[x]; }
+    __device__ bool edge(uint32_t y, uint32_t, uint32_t x, uint64_t sx) const {
+        const uint32_t vx = uint32_t(sx >> 32);
+        if (vx == kInf || bit_test(aff, x)) return false;
+        const uint64_t c = uint64_t(vx) + weight<W>(x, y, seed);
+        if (c < kInf) relax<kStrict>(state, y, uint32_t(c), x);
+        return false;
+    }
+};
+
+template <int W, bool kStrict, int QPL>
+__global__ __launch_bounds__(kBlock) void k_repair_pull(InvalArgs a) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ StageLds<2, 128> stage;
+    Ctl *c = a.ctl;
+    const uint32_t na = ld_acquire(&c->acount);
+    if (na == 0) return;
+    PullOp<W, kStrict> op{a.rows + a.n, a.state, a.aff, a.seed};
+    Counters ctr;
+    const StagedOut<2, 128> out{Sink{nullptr, &c->scount}, TileSink{a.t0, &c->scount}, &stage};
+    out_init(out);
+    expand_range<QPL>(op, a.cfg, a.alist, na, out, lds[wave_in_block()], ctr);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
+    out_flush(out);
+}
+
+template <int W, bool kStrict, int QPL>
+__global__ __launch_bounds__(kBlock) void k_repair_pull_tiles(InvalArgs a) {
+    Ctl *c = a.ctl;
+    const uint32_t nt = ld_acquire(&c->scount);
+    if (nt == 0) return;
+    PullOp<W, kStrict> op{a.rows + a.n, a.state, a.aff, a.seed};
+    expand_tiles<QPL>(op, a.cfg, a.t0, nt, DirectOut{Sink{nullptr, &c->scount}, TileSink{nullptr, &c->scount}});
+}
+
 // ------------------------------------------------- one-shot expansion passes
 template <int QPL, class Op>
 __global__ __launch_bounds__(kBlock) void k_expand_once(Op op, ExpandCfg cfg, const uint32_t *items,
@@ -932,35 +1100,27 @@ __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
     }
 }
 
-// The last workgroup of a round kernel to finish runs `close` on thread 0.
-// Every workgroup publishes its writes (fence) before counting itself done,
-// so `close` sees all appends of the round; the next kernel in the stream
-// sees what `close` wrote. No workgroup ever waits for another.
-template <class F>
-__device__ __forceinline__ void close_round(Ctl *c, F close) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&c->done, 1u) == gridDim.x - 1) {
-            __threadfence();
-            c->done = 0;
-            close();
-        }
-    }
-}
-
-// One relax round as an ordinary launch (the default driver): the host
-// keeps a few rounds enqueued ahead; a round whose frontier and deferred
-// tiles are empty returns at once.
+// One relax round as an ordinary launch (the default driver), the seq-th of
+// its chain: round r = round + seq - 1 consumes queue[r&1] / slot r%3 and
+// appends to the other queue / slot (r+1)%3. Workgroup 0 clears slot
+// (r+2)%3 and posts the round's progress word at the start; the host keeps
+// a few rounds enqueued ahead and advances `round` after the chain, so no
+// workgroup waits for another and a round has no closing step. A round
+// whose frontier and deferred tiles are empty returns at once.
 template <int W, bool kStrict, int QPL>
-__global__ __launch_bounds__(kBlock) void k_relax_round(LoopArgs a, uint64_t *prog, uint64_t tag) {
+__global__ __launch_bounds__(kBlock) void k_relax_round(LoopArgs a, uint64_t *prog, uint64_t tag, uint32_t seq) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ RoundStage stage;
     Ctl *c = a.ctl;
-    const uint32_t r = c->round;
+    const uint32_t r = c->round + seq - 1;
     const uint32_t nq = c->qcount[r % 3], nt = c->tcount[r % 3];
-    if (nq == 0 && nt == 0) return post_idle(prog, tag);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->qcount[(r + 2) % 3] = 0;
+        c->tcount[(r + 2) % 3] = 0;
+        if (nq || nt) c->rounds += 1;
+        post_progress(prog, tag, nq != 0 || nt != 0);
+    }
+    if (nq == 0 && nt == 0) return;
     Counters ctr;
     RelaxOp<W, kStrict> op{a.rows, a.state, a.stamp, a.hot, a.seed, r + 1};
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
@@ -971,13 +1131,11 @@ __global__ __launch_bounds__(kBlock) void k_relax_round(LoopArgs a, uint64_t *pr
     expand_tiles<QPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, out);
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     out_flush(out);
-    close_round(c, [&] {
-        c->qcount[r % 3] = 0;
-        c->tcount[r % 3] = 0;
-        c->round = r + 1;
-        c->rounds += 1;
-        post_progress(prog, tag, c->qcount[(r + 1) % 3] != 0 || c->tcount[(r + 1) % 3] != 0);
-    });
+}
+
+// Moves the round counter past a finished chain of `by` rounds.
+__global__ void k_advance_round(Ctl *c, uint32_t by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->round += by;
 }
 
 // ---------------------------------------------------------- PageRank rounds
@@ -1072,11 +1230,17 @@ __global__ __launch_bounds__(kBlock) void k_pr_loop(PrArgs a) {
 // PageRank round, consume phase (pr_residual.cuh:54-71) as an ordinary
 // launch; long rows are cut into tiles for this round's scatter (one tile
 // reservation per workgroup step).
-__global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a) {
+__global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a, uint64_t *prog, uint64_t tag, uint32_t seq) {
     __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     Ctl *c = a.ctl;
-    const uint32_t r = c->round;
+    const uint32_t r = c->round + seq - 1;
     const uint32_t nq = c->qcount[r % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->qcount[(r + 2) % 3] = 0;
+        c->tcount[(r + 2) % 3] = 0;
+        if (nq) c->rounds += 1;
+        post_progress(prog, tag, nq != 0);
+    }
     if (nq == 0) return;
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
     Tile *tiles = (r & 1) ? a.t1 : a.t0;
@@ -1126,16 +1290,16 @@ __global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a) {
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
 }
 
-// PageRank round, scatter phase (pr_residual.cuh:74-100); the closing
-// workgroup advances the round.
+// PageRank round, scatter phase (pr_residual.cuh:74-100) of round
+// round + seq - 1; the consume phase of the same round posted its progress.
 template <int QPL>
-__global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a, uint64_t *prog, uint64_t tag) {
+__global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a, uint32_t seq) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ RoundStage stage;
     Ctl *c = a.ctl;
-    const uint32_t r = c->round;
+    const uint32_t r = c->round + seq - 1;
     const uint32_t nq = c->qcount[r % 3];
-    if (nq == 0) return post_idle(prog, tag);
+    if (nq == 0) return;
     const uint32_t nt = c->tcount[r % 3];
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
     Tile *tiles = (r & 1) ? a.t1 : a.t0;
@@ -1149,14 +1313,6 @@ __global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a, uint64_t *prog,
     expand_tiles<QPL>(op, a.cfg, tiles, nt, out);
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     out_flush(out);
-    close_round(c, [&] {
-        c->qcount[r % 3] = 0;
-        c->tcount[r % 3] = 0;
-        c->round = r + 1;
-        c->rounds += 1;
-        c->converged = ld_acquire(&c->qcount[(r + 1) % 3]) == 0 ? 1u : 0u;
-        post_progress(prog, tag, c->converged == 0u);
-    });
 }
 
 // Plain streaming copy (16 B per lane per step): the HBM reference point for
@@ -1204,6 +1360,7 @@ __global__ void k_reduce_stats(StatShard *stats, Ctl *c) {
         c->relocated += v[16];
         c->garbage[0] += v[17];
         c->garbage[1] += v[18];
+        c->touched_sources += v[19];
         c->residual_l1 += l1;
     }
 }

@@ -71,7 +71,10 @@ enum {
     CIG_CFG_COOPERATIVE = 1u << 0, /* run fixed points as one persistent
                                       cooperative launch with grid barriers
                                       instead of chained round kernels */
-    CIG_CFG_NO_HOTNESS = 1u << 1   /* do not count row activity */
+    CIG_CFG_NO_HOTNESS = 1u << 1,  /* do not count row activity */
+    CIG_CFG_SCAN_REPAIR = 1u << 2  /* BFS/SSSP/CC: keep no incoming-row index and
+                                      repair deletions by scanning every row
+                                      (half the edge memory, slower repair) */
 };
 
 /* Engine configuration. Flags of the reference samples map as:
@@ -131,7 +134,8 @@ typedef struct cig_batch_stats {
     uint64_t stage_edges[4];
     uint64_t stage_vertices[4];
     uint64_t stage_host_edges[4];
-    double scan_kernel_ms;    /* the repair scan kernel alone (k_repair_scan), 0 if not run */
+    double scan_kernel_ms;    /* the main repair kernel alone (k_repair_pull, or k_repair_scan
+                                 with CIG_CFG_SCAN_REPAIR), 0 if not run */
     uint64_t reserved[3];
 } cig_batch_stats;
 
@@ -190,6 +194,13 @@ cig_status cig_graph_info_get(cig_engine *e, cig_graph_info *info);
  * received deletes; the multiset equals the reference's. Used by --check. */
 cig_status cig_gather_topology(cig_engine *e, uint64_t *offsets, uint32_t *dst,
                                uint64_t dst_capacity);
+/* No counterpart in the reference (whose reverse index is a CPU structure,
+ * dynamic_reverse_index.h:101-274): copies the device incoming-row index
+ * back in CSR form, row v = the sources u of the live edges (u,v), one entry
+ * per parallel edge. CIG_ESTATE when the engine keeps no incoming index (PR,
+ * or CIG_CFG_SCAN_REPAIR). For verification. */
+cig_status cig_gather_incoming(cig_engine *e, uint64_t *offsets, uint32_t *src,
+                               uint64_t src_capacity);
 
 /* ---- computation -------------------------------------------------------- */
 /* Engine::InitGraph (framework.cuh:991-1041). SSSP/BFS: exactly one source

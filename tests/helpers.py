@@ -67,6 +67,19 @@ def check_topology(eng: cg.Engine, og: O.Graph, where=""):
     assert np.array_equal(t.dst[a], dst[b]), f"{where}: row multisets differ"
 
 
+def check_incoming(eng: cg.Engine, og: O.Graph, where=""):
+    """The device incoming-row index: row v holds the multiset of u over the
+    live edges (u, v), one entry per parallel edge."""
+    t = eng.incoming()
+    off, dst = og.csr()
+    src = np.repeat(np.arange(og.n), np.diff(off).astype(np.int64))
+    assert np.array_equal(np.diff(t.offsets).astype(np.int64), np.bincount(dst, minlength=og.n)), \
+        f"{where}: in-degrees differ"
+    exp = src[np.lexsort((src, dst))]
+    rows = np.repeat(np.arange(t.n), np.diff(t.offsets).astype(np.int64))
+    assert np.array_equal(t.dst[np.lexsort((t.dst, rows))], exp), f"{where}: incoming rows differ"
+
+
 def pr_errors(ranks: np.ndarray, ref: np.ndarray):
     l1 = float(np.abs(ranks.astype(np.float64) - ref).sum())
     mass = float(np.abs(ref).sum())

@@ -14,8 +14,8 @@ import pytest
 
 import coincgraph_amd as cg
 from oracle import oracle as O
-from tests.helpers import (check_topology, check_traversal, oracle_apply, oracle_graph,
-                           pr_errors)
+from tests.helpers import (check_incoming, check_topology, check_traversal, oracle_apply,
+                           oracle_graph, pr_errors)
 
 pytestmark = pytest.mark.gpu
 
@@ -49,15 +49,18 @@ def test_converge_matches_oracle(algo, wmode, scale):
 
 @pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_REF128), ("sssp", cg.WEIGHT_HASH64),
                                         ("bfs", 0), ("cc", 0)])
-@pytest.mark.parametrize("staged", [False, True])
-def test_incremental_batches(algo, wmode, staged):
-    _announce(f"batches {algo} w{wmode} staged={staged}")
+@pytest.mark.parametrize("staged,repair", [(False, "pull"), (True, "pull"), (False, "scan")])
+def test_incremental_batches(algo, wmode, staged, repair):
+    """Deletion repair by pulling over the device incoming-row index (default)
+    or by scanning every row (CFG_SCAN_REPAIR) must give the same state."""
+    _announce(f"batches {algo} w{wmode} staged={staged} repair={repair}")
     scale = 12
     sym = algo == "cc"
     g = cg.rmat_graph(scale, 16, seed=5, symmetric=sym)
     s = cg.rmat_stream(g, scale, 6, 400, 400, seed=6, symmetric=sym)
     og = oracle_graph(g)
-    with cg.Engine(algo, weight_mode=wmode, weight_seed=7) as e:
+    flags = cg.CFG_SCAN_REPAIR if repair == "scan" else 0
+    with cg.Engine(algo, weight_mode=wmode, weight_seed=7, flags=flags) as e:
         e.load_graph(g)
         e.init_graph(0)
         e.start()
@@ -69,6 +72,8 @@ def test_incremental_batches(algo, wmode, staged):
             assert st["deletes_applied"] == len(dels) - miss
             check_traversal(e, og, algo, 0, wmode, 7, where=f"batch {b}")
         check_topology(e, og, "final")
+        if repair == "pull":
+            check_incoming(e, og, "final")
 
 
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
@@ -94,12 +99,16 @@ def test_delete_then_insert_calls(algo):
 
 
 def _multigraph_case():
-    """Hub rows with parallel edges, self-loops, many deletes on one row
-    (the workgroup path), missing and out-of-range records."""
+    """Rows of every delete class with parallel edges: a long out-row (vertex
+    0, tiled scan), a medium out-row (vertex 3, wave per delete), a long and
+    a medium incoming row (vertices 5 and 9), tiny rows (lane per delete);
+    self-loops, many deletes on one row, missing and out-of-range records."""
     rng = np.random.default_rng(1234)
     n = 3000
-    src = np.concatenate([np.zeros(6000, np.int64), rng.integers(0, n, 20000)])
-    dst = np.concatenate([rng.integers(1, n, 6000), rng.integers(0, n, 20000)])
+    src = np.concatenate([np.zeros(6000, np.int64), np.full(2000, 3), rng.integers(0, n, 5000),
+                          rng.integers(0, n, 1500), rng.integers(0, n, 20000)])
+    dst = np.concatenate([rng.integers(1, n, 6000), rng.integers(0, 40, 2000), np.full(5000, 5),
+                          np.full(1500, 9), rng.integers(0, n, 20000)])
     g_src = src.astype(np.uint32)
     g_dst = dst.astype(np.uint32)
     order = np.argsort(g_src, kind="stable")
@@ -109,11 +118,13 @@ def _multigraph_case():
     batches = []
     for b in range(4):
         pick = rng.integers(0, len(src), 2500)
-        du = np.concatenate([src[pick], np.zeros(900, np.int64), rng.integers(0, n, 100),
-                             [n + 5, 1]])
-        dv = np.concatenate([dst[pick], dst[:900], rng.integers(0, n, 100), [1, n + 9]])
-        au = np.concatenate([rng.integers(0, n, 2000), np.zeros(3000, np.int64), [7, n + 1]])
-        av = np.concatenate([rng.integers(0, n, 2000), rng.integers(0, n, 3000), [7, 2]])
+        hub = rng.integers(0, 8000, 900)  # rows 0 and 3, repeated records included
+        du = np.concatenate([src[pick], src[hub], rng.integers(0, n, 100), [n + 5, 1]])
+        dv = np.concatenate([dst[pick], dst[hub], rng.integers(0, n, 100), [1, n + 9]])
+        au = np.concatenate([rng.integers(0, n, 2000), np.zeros(3000, np.int64),
+                             rng.integers(0, n, 700), [7, n + 1]])
+        av = np.concatenate([rng.integers(0, n, 2000), rng.integers(0, n, 3000), np.full(700, 5),
+                             [7, 2]])
         batches.append((cg.edges_array(du, dv), cg.edges_array(au, av)))
     return g, batches
 
@@ -133,6 +144,7 @@ def test_multigraph_semantics(algo):
             assert (st["missing_deletes"], st["invalid_updates"]) == (miss, inval), b
             check_traversal(e, og, algo, 0, where=f"multigraph batch {b}")
             check_topology(e, og, f"multigraph batch {b}")
+            check_incoming(e, og, f"multigraph batch {b}")
 
 
 def test_empty_and_noop_batches():
