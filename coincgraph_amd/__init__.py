@@ -24,8 +24,8 @@ LIB_PATH = os.path.join(_HERE, "libcig.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cig.h")
 
 # cig_status
-OK, EINVAL, ENOMEM, EDEVICE, ESTATE, EIO, ELIMIT, EINTERNAL = range(8)
-STATUS_NAMES = ["OK", "EINVAL", "ENOMEM", "EDEVICE", "ESTATE", "EIO", "ELIMIT", "EINTERNAL"]
+OK, EINVAL, ENOMEM, EDEVICE, ESTATE, EIO, ELIMIT, EINTERNAL, EPENDING = range(9)
+STATUS_NAMES = ["OK", "EINVAL", "ENOMEM", "EDEVICE", "ESTATE", "EIO", "ELIMIT", "EINTERNAL", "EPENDING"]
 # cig_algo
 BFS, SSSP, CC, PR = range(4)
 ALGOS = {"bfs": BFS, "sssp": SSSP, "cc": CC, "pr": PR}
@@ -36,6 +36,7 @@ FIELD_VALUE, FIELD_PARENT, FIELD_RESIDUAL, FIELD_DEGREE, FIELD_TIER, FIELD_HOTNE
 # flags
 CFG_COOPERATIVE, CFG_NO_HOTNESS = 1, 2
 CFG_SCAN_REPAIR = 1 << 2
+CFG_KERNEL_TIMING = 1 << 3
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
@@ -81,7 +82,8 @@ class BatchStats(C.Structure):
         ("touched_sources", C.c_uint64), ("converged", C.c_uint32), ("launches", C.c_uint32),
         ("pr_residual_l1", C.c_double), ("stage_edges", C.c_uint64 * 4),
         ("stage_vertices", C.c_uint64 * 4), ("stage_host_edges", C.c_uint64 * 4),
-        ("scan_kernel_ms", C.c_double), ("reserved", C.c_uint64 * 3),
+        ("scan_kernel_ms", C.c_double), ("round_kernel_ms", C.c_double),
+        ("round_launches", C.c_uint64), ("reserved", C.c_uint64 * 1),
     ]
 
     def as_dict(self):
@@ -92,6 +94,31 @@ class BatchStats(C.Structure):
             v = getattr(self, name)
             out[name] = list(v) if name.startswith("stage_") else v
         return out
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("edges_per_lane", C.c_uint32), ("split_degree", C.c_uint32), ("tile_len", C.c_uint32),
+                ("block_size", C.c_uint32), ("round_blocks_per_cu", C.c_uint32), ("trials", C.c_uint32),
+                ("first_ms", C.c_double), ("best_ms", C.c_double), ("reserved", C.c_uint64 * 2)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("max_group", C.c_uint32), ("max_query_sources", C.c_uint32),
+                ("hbm_reserve", C.c_uint64), ("reserved", C.c_uint64 * 4)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("kind", C.c_uint32), ("version", C.c_uint64),
+                ("submit_ms", C.c_double), ("start_ms", C.c_double), ("done_ms", C.c_double),
+                ("group_size", C.c_uint32), ("reserved0", C.c_uint32), ("stats", BatchStats)]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("reserved0", "stats")}
+        d["stats"] = self.stats.as_dict()
+        return d
 
 
 class GraphInfo(C.Structure):
@@ -147,6 +174,8 @@ def lib():
                                       C.POINTER(BatchStats)]),
         "cig_load_updates": (C.c_int, [eng, P, C.c_uint64, P, C.c_uint64, P, C.c_uint32]),
         "cig_apply_next_batch": (C.c_int, [eng, C.c_uint32, C.POINTER(BatchStats)]),
+        "cig_apply_next_batches": (C.c_int, [eng, C.c_uint32, C.c_uint32, C.POINTER(BatchStats)]),
+        "cig_query": (C.c_int, [eng, C.c_uint32, P, C.c_uint32, P, C.c_uint64, C.POINTER(BatchStats)]),
         "cig_update_batches_left": (C.c_uint32, [eng]),
         "cig_cache_refresh": (C.c_int, [eng, C.POINTER(BatchStats)]),
         "cig_gather": (C.c_int, [eng, C.c_uint32, P, C.c_size_t]),
@@ -168,6 +197,15 @@ def lib():
         "cig_edge_weight": (C.c_uint32, [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
         "cig_device_count": (C.c_int, []),
         "cig_copy_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+        "cig_autotune": (C.c_int, [eng, C.c_uint32, C.POINTER(TuneResult)]),
+        "cig_serve_config_default": (C.c_int, [C.POINTER(ServeConfig)]),
+        "cig_serve_start": (C.c_int, [eng, C.POINTER(ServeConfig), C.POINTER(P)]),
+        "cig_submit_update": (C.c_int, [P, P, C.c_uint64, P, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "cig_submit_query": (C.c_int, [P, C.c_uint32, P, C.c_uint32, P, C.c_uint64,
+                                       C.POINTER(C.c_uint64)]),
+        "cig_poll": (C.c_int, [P, C.c_uint64, C.POINTER(RequestInfo)]),
+        "cig_wait": (C.c_int, [P, C.c_uint64, C.POINTER(RequestInfo)]),
+        "cig_serve_stop": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -417,6 +455,30 @@ class Engine:
                                               C.byref(st)))
         return st.as_dict()
 
+    def apply_next_batches(self, count: int) -> dict:
+        """The next `count` resident batches as one group (same results as
+        applying them one by one, one mutation and one fixed point)."""
+        st = BatchStats()
+        self._call(lib().cig_apply_next_batches(self._e, count, 0, C.byref(st)))
+        return st.as_dict()
+
+    def autotune(self, source: int = 0) -> dict:
+        """Picks the fastest launch shape for this graph (results unchanged);
+        leaves the engine converged from `source`."""
+        r = TuneResult()
+        self._call(lib().cig_autotune(self._e, source, C.byref(r)))
+        return r.as_dict()
+
+    def query(self, algo: str, sources) -> np.ndarray:
+        """BFS / SSSP from up to 64 sources on the live graph: array (nsrc, n)."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        out = np.zeros((len(src), self.n), dtype=np.uint32)
+        st = BatchStats()
+        self._call(lib().cig_query(self._e, ALGOS[algo], _ptr(src), len(src), _ptr(out), out.nbytes,
+                                   C.byref(st)))
+        self.last_query_stats = st.as_dict()
+        return out
+
     def batches_left(self) -> int:
         return int(lib().cig_update_batches_left(self._e))
 
@@ -467,3 +529,63 @@ class Engine:
         src = np.zeros(cap, dtype=np.uint32)
         self._call(lib().cig_gather_incoming(self._e, _ptr(off), _ptr(src), cap))
         return HostGraph(self.n, off, src[: int(off[-1])])
+
+
+class Server:
+    """The engine's serving loop (cig_serve_*): submit updates and queries
+    from any thread; results come back in submission order semantics."""
+
+    def __init__(self, engine: "Engine", max_group=32, max_query_sources=64, hbm_reserve=1 << 30):
+        cfg = ServeConfig()
+        lib().cig_serve_config_default(C.byref(cfg))
+        cfg.max_group = max_group
+        cfg.max_query_sources = max_query_sources
+        cfg.hbm_reserve = hbm_reserve
+        self._s = C.c_void_p()
+        self._eng = engine
+        engine._call(lib().cig_serve_start(engine._e, C.byref(cfg), C.byref(self._s)))
+        self._outs = {}
+
+    def submit_update(self, dels, adds) -> int:
+        d = np.ascontiguousarray(dels, dtype=EDGE_DTYPE)
+        a = np.ascontiguousarray(adds, dtype=EDGE_DTYPE)
+        t = C.c_uint64()
+        _check(lib().cig_submit_update(self._s, _ptr(d) if d.size else None, d.size,
+                                       _ptr(a) if a.size else None, a.size, C.byref(t)))
+        return t.value
+
+    def submit_query(self, algo: str, sources) -> int:
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        out = np.zeros((len(src), self._eng.n), dtype=np.uint32)
+        t = C.c_uint64()
+        _check(lib().cig_submit_query(self._s, ALGOS[algo], _ptr(src), len(src), _ptr(out), out.nbytes,
+                                      C.byref(t)))
+        self._outs[t.value] = out  # must outlive the request
+        return t.value
+
+    def _collect(self, fn, ticket):
+        info = RequestInfo()
+        st = fn(self._s, ticket, C.byref(info))
+        if st == EPENDING:
+            return None
+        _check(st)
+        d = info.as_dict()
+        d["result"] = self._outs.pop(ticket, None)
+        return d
+
+    def poll(self, ticket: int):
+        return self._collect(lib().cig_poll, ticket)
+
+    def wait(self, ticket: int) -> dict:
+        return self._collect(lib().cig_wait, ticket)
+
+    def stop(self):
+        if self._s:
+            _check(lib().cig_serve_stop(self._s))
+            self._s = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()

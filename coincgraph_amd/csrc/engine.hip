@@ -14,7 +14,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -108,6 +114,10 @@ struct cig_engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
     bool scan_timed = false;  // ev[8]/ev[9] bracket this call's repair scan kernel
+    // CIG_CFG_KERNEL_TIMING: an event pair per propagation round launch
+    static constexpr uint32_t kRoundEvents = 512;
+    std::vector<hipEvent_t> rev_begin, rev_end;
+    uint32_t round_timed = 0;
 
     uint32_t n = 0;
     uint32_t nrows = 0;  // n, or 2n with the incoming rows [n, 2n)
@@ -127,6 +137,14 @@ struct cig_engine {
     DBuf<StatShard> stats;              // sharded statistics (see k_reduce_stats)
     DBuf<uint32_t> htoff, hkeys, hcnt;  // long-row delete tables
     DBuf<uint8_t> dclass;               // per-row delete class of the current call
+    // grouped batches: (u,v) table, op lists, cancel flags, per-batch cuts
+    DBuf<unsigned long long> gkey;
+    DBuf<uint32_t> ghead, gnext, gcut, gover;
+    DBuf<uint8_t> gconf, gskip;
+    const uint8_t *cur_skip = nullptr;  // cancel flags of the group being applied
+    // multi-source queries
+    DBuf<uint32_t> ms_dist, ms_stamp, ms_src, ms_out;
+    DBuf<unsigned long long> ms_act, ms_snap;
     DBuf<uint2> hcnts, dtiles;
     uint64_t rec_cap = 0;
     DBuf<cig_edge> rec_dels, rec_adds;
@@ -150,6 +168,7 @@ struct cig_engine {
 
     // launch shape
     uint32_t split = kSplitDefault, tile_len = kTileLenDefault, qpl = kQplDefault;
+    uint32_t rblock = kBlock;  // relax round workgroup size (256 or 512)
     uint32_t grid_small = 0;   // one-shot kernels
     uint32_t grid_coop = 0;    // persistent cooperative kernels
     uint32_t grid_round = 0;   // chained round kernels
@@ -174,6 +193,14 @@ struct cig_engine {
         cus = prop.multiProcessorCount;
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        if (cfg.flags & CIG_CFG_KERNEL_TIMING) {
+            rev_begin.resize(kRoundEvents);
+            rev_end.resize(kRoundEvents);
+            for (uint32_t i = 0; i < kRoundEvents; ++i) {
+                HIPCHK(hipEventCreate(&rev_begin[i]));
+                HIPCHK(hipEventCreate(&rev_end[i]));
+            }
+        }
         ctl.alloc(1);
         hctl.alloc(1);
         prog.alloc(1, hipHostMallocCoherent | hipHostMallocMapped);
@@ -446,6 +473,19 @@ struct cig_engine {
         }
     }
 
+    uint64_t round_launches = 0;
+
+    // Launches one propagation round kernel, bracketed by an event pair when
+    // kernel timing is on.
+    template <class F>
+    void timed_round(F launch) {
+        ++round_launches;
+        const bool timed = !rev_begin.empty() && round_timed < kRoundEvents;
+        if (timed) HIPCHK(hipEventRecord(rev_begin[round_timed], stream));
+        launch();
+        if (timed) HIPCHK(hipEventRecord(rev_end[round_timed++], stream));
+    }
+
     void advance_round(uint64_t by) {
         if (!by) return;
         k_advance_round<<<1, 64, 0, stream>>>(ctl.p, uint32_t(by));
@@ -481,7 +521,12 @@ struct cig_engine {
             const uint64_t bound = cfg.max_rounds ? cfg.max_rounds : round_bound();
             advance_round(run_chain(
                 [&](uint64_t tag, uint32_t seq) {
-                    k_relax_round<W, S, QPL><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                    timed_round([&] {
+                        if (rblock == 512)
+                            k_relax_round<W, S, QPL, 512><<<grid_round, 512, 0, stream>>>(a, prog_dev, tag, seq);
+                        else
+                            k_relax_round<W, S, QPL><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                    });
                 },
                 bound, "relax fixed point"));
         }
@@ -539,7 +584,7 @@ struct cig_engine {
             advance_round(run_chain(
                 [&](uint64_t tag, uint32_t seq) {
                     k_pr_consume<<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
-                    k_pr_scatter<1><<<grid_round, kBlock, 0, stream>>>(a, seq);
+                    timed_round([&] { k_pr_scatter<1><<<grid_round, kBlock, 0, stream>>>(a, seq); });
                 },
                 cfg.pr_max_rounds, "PageRank"));
         }
@@ -575,6 +620,8 @@ struct cig_engine {
     void begin_call(cig_batch_stats *st) {
         if (st) std::memset(st, 0, sizeof(*st));
         scan_timed = false;
+        round_timed = 0;
+        round_launches = 0;
         reset_ctl(true);
         HIPCHK(hipEventRecord(ev[0], stream));
     }
@@ -611,6 +658,12 @@ struct cig_engine {
             st->propagate_ms = elapsed(5, 4);
         }
         if (scan_timed) st->scan_kernel_ms = elapsed(8, 9);
+        for (uint32_t i = 0; i < round_timed; ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, rev_begin[i], rev_end[i]));
+            st->round_kernel_ms += ms;
+        }
+        st->round_launches = round_launches;
         for (int i = 0; i < 4; ++i) {
             st->stage_edges[i] = c.st_edges[i];
             st->stage_vertices[i] = c.st_vexp[i];
@@ -659,6 +712,7 @@ struct cig_engine {
         m.hcnts = hcnts.p;
         m.hcnt = hcnt.p;
         m.dclass = dclass.p;
+        m.skip = cur_skip;
         m.dtiles = dtiles.p;
         return m;
     }
@@ -882,6 +936,8 @@ struct cig_engine {
         x.kind = cfg.algo == CIG_ALGO_CC ? 1u : 0u;
         x.max_rounds = round_bound();  // each round consumes at least one new item
         x.seed = cfg.weight_seed;
+        x.add_skip = cur_skip;
+        x.del_skip = cur_skip ? cur_skip + na : nullptr;
         return x;
     }
 
@@ -1185,6 +1241,285 @@ struct cig_engine {
     }
     bool deletes_done = false;
 
+    // Applies `k` consecutive update batches as one group (see k_group_match):
+    // d/a are the group's deletes and adds in batch order, dcut/acut their
+    // k+1 prefixes. Results equal applying the batches one by one.
+    void apply_group(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na,
+                     const std::vector<uint32_t> &dcut, const std::vector<uint32_t> &acut, uint32_t flags,
+                     cig_batch_stats *st) {
+        const uint32_t k = uint32_t(dcut.size() - 1);
+        if (k == 1 || nd == 0 || na == 0) return apply_device(d, nd, a, na, flags, st);
+        HIPCHK(hipEventRecord(ev[6], stream));
+        uint32_t bits = 10;
+        while ((1ull << bits) < 2ull * na) ++bits;
+        const uint64_t cap = 1ull << bits, ops = uint64_t(na) + nd;
+        if (gkey.count < cap) {
+            gkey.alloc(cap);
+            ghead.alloc(cap);
+            gconf.alloc((cap + 3) & ~3ull);
+        }
+        if (gnext.count < ops) {
+            gnext.alloc(ops);
+            gskip.alloc((ops + 3) & ~3ull);
+        }
+        if (gcut.count < 2 * (uint64_t(k) + 1)) gcut.alloc(2 * (uint64_t(k) + 1));
+        if (!gover.count) gover.alloc(1);
+        fill32(reinterpret_cast<uint32_t *>(gkey.p), cap * 2, 0);
+        fill32(ghead.p, cap, kNone);
+        fill32(reinterpret_cast<uint32_t *>(gconf.p), (cap + 3) / 4, 0);
+        fill32(reinterpret_cast<uint32_t *>(gskip.p), (ops + 3) / 4, 0);
+        fill32(gover.p, 1, 0);
+        std::vector<uint32_t> cuts(dcut);
+        cuts.insert(cuts.end(), acut.begin(), acut.end());
+        HIPCHK(hipMemcpyAsync(gcut.p, cuts.data(), cuts.size() * 4, hipMemcpyHostToDevice, stream));
+        GroupArgs g{};
+        g.dels = d;
+        g.nd = nd;
+        g.adds = a;
+        g.na = na;
+        g.del_cut = gcut.p;
+        g.add_cut = gcut.p + k + 1;
+        g.nb = k;
+        g.n = n;
+        g.tkey = gkey.p;
+        g.thead = ghead.p;
+        g.tconf = gconf.p;
+        g.tbits = bits;
+        g.next_op = gnext.p;
+        g.skip = gskip.p;
+        g.overflow = gover.p;
+        g.stats = stats.p;
+        k_group_adds<<<grid_small, 256, 0, stream>>>(g);
+        k_group_dels<<<grid_small, 256, 0, stream>>>(g);
+        k_group_match<<<grid_small, 256, 0, stream>>>(g);
+        HIPCHK(hipGetLastError());
+        uint32_t over = 0;
+        HIPCHK(hipMemcpyAsync(&over, gover.p, 4, hipMemcpyDeviceToHost, stream));
+        sync();
+        if (over) {
+            // a key with more than kGroupOps records: apply the batches one by
+            // one (the next call's control reset drops the match statistics)
+            reduce_stats();
+            cig_batch_stats one{}, sum{};
+            for (uint32_t b = 0; b < k; ++b) {
+                apply_device(d + dcut[b], dcut[b + 1] - dcut[b], a + acut[b], acut[b + 1] - acut[b],
+                             flags & ~uint32_t(CIG_BATCH_STAGED), st ? &one : nullptr);
+                if (st) accumulate(sum, one);
+            }
+            if (st) {
+                *st = sum;
+                st->total_ms = elapsed(6, 7);
+            }
+            return;
+        }
+        cur_skip = gskip.p;
+        try {
+            apply_device(d, nd, a, na, flags & ~uint32_t(CIG_BATCH_STAGED), st);
+        } catch (...) {
+            cur_skip = nullptr;
+            throw;
+        }
+        cur_skip = nullptr;
+        if (st) st->total_ms = elapsed(6, 7);
+    }
+
+    static void accumulate(cig_batch_stats &sum, const cig_batch_stats &x) {
+        sum.mutate_ms += x.mutate_ms;
+        sum.invalidate_ms += x.invalidate_ms;
+        sum.repair_ms += x.repair_ms;
+        sum.seed_ms += x.seed_ms;
+        sum.propagate_ms += x.propagate_ms;
+        sum.edges_scanned += x.edges_scanned;
+        sum.host_edges_scanned += x.host_edges_scanned;
+        sum.vertices_expanded += x.vertices_expanded;
+        sum.affected += x.affected;
+        sum.rounds += x.rounds;
+        sum.deletes_applied += x.deletes_applied;
+        sum.missing_deletes += x.missing_deletes;
+        sum.invalid_updates += x.invalid_updates;
+        sum.adds_applied += x.adds_applied;
+        sum.relocated_rows += x.relocated_rows;
+        sum.touched_sources += x.touched_sources;
+        sum.converged = x.converged;
+        sum.pool_compactions = x.pool_compactions;
+        sum.pr_residual_l1 = x.pr_residual_l1;
+        for (int i = 0; i < 4; ++i) {
+            sum.stage_edges[i] += x.stage_edges[i];
+            sum.stage_vertices[i] += x.stage_vertices[i];
+            sum.stage_host_edges[i] += x.stage_host_edges[i];
+        }
+        sum.scan_kernel_ms += x.scan_kernel_ms;
+        sum.round_kernel_ms += x.round_kernel_ms;
+        sum.round_launches += x.round_launches;
+    }
+
+    // The next `k` resident batches as one group.
+    void apply_next_group(uint32_t k, uint32_t flags, cig_batch_stats *st) {
+        check_ready();
+        if (deletes_done) throw HipError(CIG_ESTATE, "finish the add phase of this batch first");
+        const uint32_t left = uint32_t(batch_sizes.size() / 2) - next_batch;
+        if (!k || k > left) throw HipError(CIG_ELIMIT, "not that many update batches left");
+        std::vector<uint32_t> dcut(1, 0), acut(1, 0);
+        for (uint32_t b = 0; b < k; ++b) {
+            acut.push_back(acut.back() + batch_sizes[2 * (next_batch + b)]);
+            dcut.push_back(dcut.back() + batch_sizes[2 * (next_batch + b) + 1]);
+        }
+        apply_group(s_dels.p + del_off, dcut.back(), s_adds.p + add_off, acut.back(), dcut, acut, flags, st);
+        add_off += acut.back();
+        del_off += dcut.back();
+        next_batch += k;
+    }
+
+    // ------------------------------------------------------------ autotuner
+    // Coordinate descent over the launch shape of the propagation kernels
+    // (quads per lane, tile split threshold, tile length, relax workgroup
+    // size, relax workgroups per CU), each trial timing the fixed point from
+    // `src` on the loaded graph (best of two). Every shape gives the same
+    // values, so only time is compared. Leaves the engine converged from src
+    // with the best shape.
+    void autotune(uint32_t src, cig_tune_result *res) {
+        if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        if (is_pr() || coop) throw HipError(CIG_EINVAL, "the autotuner drives traversal engines with chained rounds");
+        struct Shape {
+            uint32_t qpl, split, tile, block, bpc;
+        };
+        auto apply = [&](const Shape &x) {
+            qpl = x.qpl;
+            split = x.split;
+            tile_len = x.tile;
+            rblock = x.block;
+            grid_round = uint32_t(cus) * x.bpc;
+            ensure_tiles();
+        };
+        auto trial = [&](const Shape &x) {
+            apply(x);
+            double best = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                cig_batch_stats st{};
+                init(&src, 1);
+                converge(&st);
+                best = std::min(best, st.total_ms);
+            }
+            return best;
+        };
+        Shape cur{qpl, split, tile_len, rblock, std::max<uint32_t>(1, grid_round / uint32_t(cus))};
+        const double first = trial(cur);
+        double best = first;
+        uint32_t trials = 1;
+        const std::vector<uint32_t> qv{1, 2, 4}, sv{32, 64, 128, 256}, tv{256, 512, 1024}, bv{256, 512},
+            pv{1, 2, 4};
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int dim = 0; dim < 5; ++dim) {
+                const std::vector<uint32_t> &vals = dim == 0 ? qv : dim == 1 ? sv : dim == 2 ? tv : dim == 3 ? bv : pv;
+                for (uint32_t v : vals) {
+                    Shape x = cur;
+                    (dim == 0 ? x.qpl : dim == 1 ? x.split : dim == 2 ? x.tile : dim == 3 ? x.block : x.bpc) = v;
+                    if (x.qpl == cur.qpl && x.split == cur.split && x.tile == cur.tile && x.block == cur.block &&
+                        x.bpc == cur.bpc)
+                        continue;
+                    const double t = trial(x);
+                    ++trials;
+                    if (t < best) {
+                        best = t;
+                        cur = x;
+                    }
+                }
+            }
+        }
+        apply(cur);
+        init(&src, 1);
+        converge(nullptr);
+        if (res) {
+            std::memset(res, 0, sizeof(*res));
+            res->edges_per_lane = 4 * cur.qpl;
+            res->split_degree = cur.split;
+            res->tile_len = cur.tile;
+            res->block_size = cur.block;
+            res->round_blocks_per_cu = cur.bpc;
+            res->trials = trials;
+            res->first_ms = first;
+            res->best_ms = best;
+        }
+    }
+
+    // ------------------------------------------------------------ queries
+    // Up to 64 BFS / SSSP sources on the live graph, independent of the
+    // engine's own incremental state (which the query leaves unchanged).
+    // out[s * n + v] = distance of v from sources[s] (kInf: unreachable).
+    template <int W, int QPL>
+    void ms_run(MsArgs &a) {
+        advance_round(run_chain(
+            [&](uint64_t tag, uint32_t seq) {
+                k_ms_consume<<<grid_small, 256, 0, stream>>>(a, prog_dev, tag, seq);
+                k_ms_expand<W, QPL><<<grid_round, kBlock, 0, stream>>>(a, seq);
+                k_ms_tiles<W, QPL><<<grid_round, kBlock, 0, stream>>>(a, seq);
+            },
+            round_bound(), "multi-source query"));
+    }
+
+    template <int QPL>
+    void ms_dispatch(uint32_t wkind, MsArgs &a) {
+        switch (wkind) {
+        case 0: ms_run<0, QPL>(a); break;
+        case 1: ms_run<1, QPL>(a); break;
+        default: ms_run<2, QPL>(a); break;
+        }
+    }
+
+    void query(uint32_t algo, const uint32_t *sources, uint32_t k, uint32_t *out, uint64_t bytes,
+               cig_batch_stats *st) {
+        if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        if (algo != CIG_ALGO_BFS && algo != CIG_ALGO_SSSP) throw HipError(CIG_EINVAL, "queries are BFS or SSSP");
+        if (k == 0 || k > 64 || !sources) throw HipError(CIG_EINVAL, "1..64 sources");
+        for (uint32_t s = 0; s < k; ++s)
+            if (sources[s] >= n) throw HipError(CIG_EINVAL, "source out of range");
+        if (!out || bytes < uint64_t(n) * k * 4) throw HipError(CIG_EINVAL, "output buffer too small");
+        const uint32_t stride = (k + 3u) & ~3u;
+        if (ms_dist.count < uint64_t(n) * stride) ms_dist.alloc(uint64_t(n) * stride);
+        if (ms_act.count < n) {
+            ms_act.alloc(n);
+            ms_snap.alloc(n);
+            ms_stamp.alloc(n);
+            fill32(ms_stamp.p, n, 0);
+        }
+        if (!ms_src.count) ms_src.alloc(64);
+        if (ms_out.count < uint64_t(n) * k) ms_out.alloc(uint64_t(n) * k);
+        begin_call(st);
+        fill32(ms_dist.p, uint64_t(n) * stride, kInf);
+        fill32(reinterpret_cast<uint32_t *>(ms_act.p), uint64_t(n) * 2, 0);
+        fill32(reinterpret_cast<uint32_t *>(ms_snap.p), uint64_t(n) * 2, 0);
+        HIPCHK(hipMemcpyAsync(ms_src.p, sources, k * 4, hipMemcpyHostToDevice, stream));
+        MsArgs a{};
+        a.ctl = ctl.p;
+        a.rows = rows.p;
+        a.dist = ms_dist.p;
+        a.act = ms_act.p;
+        a.snap = ms_snap.p;
+        a.stamp = ms_stamp.p;
+        a.q0 = q0.p;
+        a.q1 = q1.p;
+        a.t0 = t0.p;
+        a.t1 = t1.p;
+        a.cfg = ecfg(kStagePropagate);
+        a.n = n;
+        a.stride = stride;
+        a.seed = cfg.weight_seed;
+        k_ms_seed<<<1, 64, 0, stream>>>(a, ms_src.p, k);
+        HIPCHK(hipGetLastError());
+        for (int m : {1, 2, 3, 5}) HIPCHK(hipEventRecord(ev[m], stream));
+        const uint32_t wkind = algo == CIG_ALGO_BFS ? 2u : (cfg.weight_mode == CIG_WEIGHT_HASH64 ? 1u : 0u);
+        switch (qpl) {
+        case 1: ms_dispatch<1>(wkind, a); break;
+        case 4: ms_dispatch<4>(wkind, a); break;
+        default: ms_dispatch<2>(wkind, a); break;
+        }
+        HIPCHK(hipEventRecord(ev[4], stream));
+        k_ms_gather<<<grid_small, 256, 0, stream>>>(ms_dist.p, n, stride, k, ms_out.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, ms_out.p, uint64_t(n) * k * 4, hipMemcpyDeviceToHost, stream));
+        end_call(st, 4);
+    }
+
     // ------------------------------------------------------------ cache
     // Sort-based admission (compute_hot_vertices + confirm_candidate_batch,
     // framework.cuh:605-729): rows by score descending (stable: ties by id),
@@ -1332,6 +1667,8 @@ struct cig_engine {
             (void)hipStreamSynchronize(stream);
             for (auto &e : ev)
                 if (e) (void)hipEventDestroy(e);
+            for (auto &e : rev_begin) (void)hipEventDestroy(e);
+            for (auto &e : rev_end) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(stream);
         }
     }
@@ -1358,7 +1695,198 @@ cig_status guarded(cig_engine *e, F f) {
     }
 }
 
+
 }  // namespace
+
+// ------------------------------------------------------------------ serving
+// One server thread owns the engine (see cig.h). Requests are served in
+// submission order; consecutive update requests form one group, consecutive
+// queries of one algorithm one multi-source run.
+struct ServeRequest {
+    uint64_t ticket = 0;
+    uint32_t kind = 0;  // 0 update, 1 query
+    std::vector<cig_edge> dels, adds;
+    uint32_t algo = 0;
+    std::vector<uint32_t> sources;
+    uint32_t *out = nullptr;
+    uint64_t out_bytes = 0;
+    cig_request_info info{};
+    bool done = false;
+};
+
+struct cig_server {
+    cig_engine *e = nullptr;
+    cig_serve_config cfg{};
+    std::mutex mu;
+    std::condition_variable work_cv, done_cv;
+    std::deque<std::shared_ptr<ServeRequest>> queue;
+    std::unordered_map<uint64_t, std::shared_ptr<ServeRequest>> live;
+    uint64_t next_ticket = 1, version = 0;
+    bool stopping = false;
+    std::thread th;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    DBuf<cig_edge> sd, sa;  // device staging of a group's records
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    uint64_t free_hbm() {
+        size_t fr = 0, total = 0;
+        HIPCHK(hipMemGetInfo(&fr, &total));
+        return fr;
+    }
+
+    // Bytes a group of `nrec` records adds to the staging buffers.
+    uint64_t staging_need(uint64_t nd, uint64_t na) const {
+        uint64_t need = 0;
+        if (nd > sd.count) need += nd * sizeof(cig_edge);
+        if (na > sa.count) need += na * sizeof(cig_edge);
+        return need;
+    }
+
+    void finish(std::vector<std::shared_ptr<ServeRequest>> &batch, cig_status st, const cig_batch_stats &stats,
+                uint64_t version_of_first, bool per_request_version) {
+        const double t = now_ms();
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < batch.size(); ++i) {
+            ServeRequest &r = *batch[i];
+            r.info.status = st;
+            r.info.done_ms = t;
+            r.info.group_size = uint32_t(batch.size());
+            r.info.version = version_of_first + (per_request_version ? i : 0);
+            r.info.stats = stats;
+            r.done = true;
+        }
+        done_cv.notify_all();
+    }
+
+    void serve_updates(std::vector<std::shared_ptr<ServeRequest>> &batch) {
+        uint64_t nd = 0, na = 0;
+        std::vector<uint32_t> dcut(1, 0), acut(1, 0);
+        for (auto &r : batch) {
+            nd += r->dels.size();
+            na += r->adds.size();
+            dcut.push_back(uint32_t(nd));
+            acut.push_back(uint32_t(na));
+        }
+        cig_batch_stats stats{};
+        cig_status st = CIG_OK;
+        try {
+            if (nd > 0x7FFFFFFFull || na > 0x7FFFFFFFull) throw HipError(CIG_ELIMIT, "group too large");
+            if (sd.count < nd) sd.alloc(nd);
+            if (sa.count < na) sa.alloc(na);
+            uint64_t od = 0, oa = 0;
+            for (auto &r : batch) {
+                if (!r->dels.empty())
+                    HIPCHK(hipMemcpyAsync(sd.p + od, r->dels.data(), r->dels.size() * sizeof(cig_edge),
+                                          hipMemcpyHostToDevice, e->stream));
+                if (!r->adds.empty())
+                    HIPCHK(hipMemcpyAsync(sa.p + oa, r->adds.data(), r->adds.size() * sizeof(cig_edge),
+                                          hipMemcpyHostToDevice, e->stream));
+                od += r->dels.size();
+                oa += r->adds.size();
+            }
+            e->check_ready();
+            e->apply_group(sd.p, uint32_t(nd), sa.p, uint32_t(na), dcut, acut, 0, &stats);
+        } catch (const HipError &x) {
+            st = x.code;
+            e->err = x.what();
+        }
+        const uint64_t v0 = version + 1;
+        if (st == CIG_OK) version += batch.size();
+        finish(batch, st, stats, v0, true);
+    }
+
+    void serve_queries(std::vector<std::shared_ptr<ServeRequest>> &batch) {
+        std::vector<uint32_t> src;
+        for (auto &r : batch) src.insert(src.end(), r->sources.begin(), r->sources.end());
+        cig_batch_stats stats{};
+        cig_status st = CIG_OK;
+        try {
+            std::vector<uint32_t> merged(size_t(e->n) * src.size());
+            e->query(batch[0]->algo, src.data(), uint32_t(src.size()), merged.data(), merged.size() * 4, &stats);
+            size_t at = 0;
+            for (auto &r : batch) {
+                const size_t words = size_t(e->n) * r->sources.size();
+                std::memcpy(r->out, merged.data() + at, words * 4);
+                at += words;
+            }
+        } catch (const HipError &x) {
+            st = x.code;
+            e->err = x.what();
+        }
+        finish(batch, st, stats, version, false);
+    }
+
+    void fail(std::shared_ptr<ServeRequest> r, cig_status st) {
+        std::vector<std::shared_ptr<ServeRequest>> one{r};
+        finish(one, st, cig_batch_stats{}, version, false);
+    }
+
+    void loop() {
+        (void)hipSetDevice(e->cfg.device);
+        for (;;) {
+            std::vector<std::shared_ptr<ServeRequest>> batch;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                work_cv.wait(lk, [&] { return stopping || !queue.empty(); });
+                if (queue.empty()) return;  // stopping and drained
+                const uint32_t kind = queue.front()->kind;
+                uint32_t nsrc = 0;
+                while (!queue.empty() && queue.front()->kind == kind) {
+                    const ServeRequest &r = *queue.front();
+                    if (kind == 0) {
+                        if (batch.size() >= cfg.max_group) break;
+                    } else {
+                        if (r.algo != queue.front()->algo || (!batch.empty() && r.algo != batch[0]->algo)) break;
+                        if (nsrc + r.sources.size() > cfg.max_query_sources) break;
+                        nsrc += uint32_t(r.sources.size());
+                    }
+                    batch.push_back(queue.front());
+                    queue.pop_front();
+                }
+                for (auto &r : batch) r->info.start_ms = now_ms();
+            }
+            // admission by free HBM: shrink the group until its staging fits
+            if (batch[0]->kind == 0) {
+                try {
+                    const uint64_t fr = free_hbm();
+                    uint64_t nd = 0, na = 0;
+                    for (auto &r : batch) nd += r->dels.size(), na += r->adds.size();
+                    while (batch.size() > 1 && staging_need(nd, na) + cfg.hbm_reserve > fr) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        nd -= batch.back()->dels.size();
+                        na -= batch.back()->adds.size();
+                        queue.push_front(batch.back());
+                        batch.pop_back();
+                    }
+                    if (staging_need(nd, na) + cfg.hbm_reserve > fr) {
+                        fail(batch[0], CIG_ENOMEM);
+                        continue;
+                    }
+                } catch (const HipError &x) {
+                    for (auto &r : batch) fail(r, x.code);
+                    continue;
+                }
+                serve_updates(batch);
+            } else {
+                try {
+                    const uint32_t k = [&] { uint32_t t = 0; for (auto &r : batch) t += uint32_t(r->sources.size()); return t; }();
+                    const uint64_t need = uint64_t(e->n) * (((k + 3) & ~3u) + k) * 4 + uint64_t(e->n) * 20;
+                    if (need + cfg.hbm_reserve > free_hbm() + e->ms_dist.bytes() + e->ms_out.bytes() + e->ms_act.bytes() * 3) {
+                        for (auto &r : batch) fail(r, CIG_ENOMEM);
+                        continue;
+                    }
+                } catch (const HipError &x) {
+                    for (auto &r : batch) fail(r, x.code);
+                    continue;
+                }
+                serve_queries(batch);
+            }
+        }
+    }
+};
 
 extern "C" {
 
@@ -1465,6 +1993,22 @@ cig_status cig_load_updates(cig_engine *e, const cig_edge *adds, uint64_t na, co
     });
 }
 
+cig_status cig_autotune(cig_engine *e, uint32_t source, cig_tune_result *res) {
+    return guarded(e, [&] {
+        if (source >= e->n && e->loaded) throw HipError(CIG_EINVAL, "source out of range");
+        e->autotune(source, res);
+    });
+}
+
+cig_status cig_query(cig_engine *e, uint32_t algo, const uint32_t *sources, uint32_t nsrc, uint32_t *dist,
+                     uint64_t dist_bytes, cig_batch_stats *st) {
+    return guarded(e, [&] { e->query(algo, sources, nsrc, dist, dist_bytes, st); });
+}
+
+cig_status cig_apply_next_batches(cig_engine *e, uint32_t count, uint32_t flags, cig_batch_stats *st) {
+    return guarded(e, [&] { e->apply_next_group(count, flags, st); });
+}
+
 cig_status cig_apply_next_batch(cig_engine *e, uint32_t flags, cig_batch_stats *st) {
     return guarded(e, [&] { e->apply_next(flags, st); });
 }
@@ -1512,6 +2056,110 @@ cig_status cig_copy_probe(int32_t device, uint64_t bytes, uint32_t iters, double
         g_create_error = x.what();
         return x.code;
     }
+}
+
+
+cig_status cig_serve_config_default(cig_serve_config *c) {
+    if (!c) return CIG_EINVAL;
+    std::memset(c, 0, sizeof(*c));
+    c->max_group = 32;
+    c->max_query_sources = 64;
+    c->hbm_reserve = uint64_t(1) << 30;
+    return CIG_OK;
+}
+
+cig_status cig_serve_start(cig_engine *e, const cig_serve_config *c, cig_server **out) {
+    if (!e || !out) return CIG_EINVAL;
+    return guarded(e, [&] {
+        e->check_ready();
+        cig_serve_config cfg;
+        cig_serve_config_default(&cfg);
+        if (c) cfg = *c;
+        if (!cfg.max_group || !cfg.max_query_sources || cfg.max_query_sources > 64)
+            throw HipError(CIG_EINVAL, "max_group >= 1 and max_query_sources in 1..64");
+        auto *s = new cig_server;
+        s->e = e;
+        s->cfg = cfg;
+        s->th = std::thread([s] { s->loop(); });
+        *out = s;
+    });
+}
+
+static cig_status submit(cig_server *s, std::shared_ptr<ServeRequest> r, uint64_t *ticket) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return CIG_ESTATE;
+    r->ticket = s->next_ticket++;
+    r->info.status = CIG_EPENDING;
+    r->info.submit_ms = s->now_ms();
+    s->live[r->ticket] = r;
+    s->queue.push_back(r);
+    *ticket = r->ticket;
+    s->work_cv.notify_one();
+    return CIG_OK;
+}
+
+cig_status cig_submit_update(cig_server *s, const cig_edge *dels, uint64_t nd, const cig_edge *adds, uint64_t na,
+                             uint64_t *ticket) {
+    if (!s || !ticket || (nd && !dels) || (na && !adds)) return CIG_EINVAL;
+    try {
+        auto r = std::make_shared<ServeRequest>();
+        r->kind = 0;
+        r->info.kind = 0;
+        r->dels.assign(dels, dels + nd);
+        r->adds.assign(adds, adds + na);
+        return submit(s, r, ticket);
+    } catch (const std::bad_alloc &) {
+        return CIG_ENOMEM;
+    }
+}
+
+cig_status cig_submit_query(cig_server *s, uint32_t algo, const uint32_t *sources, uint32_t nsrc, uint32_t *dist,
+                            uint64_t dist_bytes, uint64_t *ticket) {
+    if (!s || !ticket || !sources || !dist || nsrc == 0 || nsrc > s->cfg.max_query_sources) return CIG_EINVAL;
+    if (algo != CIG_ALGO_BFS && algo != CIG_ALGO_SSSP) return CIG_EINVAL;
+    if (dist_bytes < uint64_t(s->e->n) * nsrc * 4) return CIG_EINVAL;
+    for (uint32_t i = 0; i < nsrc; ++i)
+        if (sources[i] >= s->e->n) return CIG_EINVAL;
+    try {
+        auto r = std::make_shared<ServeRequest>();
+        r->kind = 1;
+        r->info.kind = 1;
+        r->algo = algo;
+        r->sources.assign(sources, sources + nsrc);
+        r->out = dist;
+        r->out_bytes = dist_bytes;
+        return submit(s, r, ticket);
+    } catch (const std::bad_alloc &) {
+        return CIG_ENOMEM;
+    }
+}
+
+static cig_status collect(cig_server *s, uint64_t ticket, cig_request_info *info, bool block) {
+    if (!s || !info) return CIG_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->live.find(ticket);
+    if (it == s->live.end()) return CIG_EINVAL;
+    auto r = it->second;
+    if (block) s->done_cv.wait(lk, [&] { return r->done; });
+    if (!r->done) return CIG_EPENDING;
+    *info = r->info;
+    s->live.erase(it);
+    return CIG_OK;
+}
+
+cig_status cig_poll(cig_server *s, uint64_t ticket, cig_request_info *info) { return collect(s, ticket, info, false); }
+cig_status cig_wait(cig_server *s, uint64_t ticket, cig_request_info *info) { return collect(s, ticket, info, true); }
+
+cig_status cig_serve_stop(cig_server *s) {
+    if (!s) return CIG_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+        s->work_cv.notify_all();
+    }
+    if (s->th.joinable()) s->th.join();
+    delete s;
+    return CIG_OK;
 }
 
 int cig_device_count(void) {

@@ -127,6 +127,7 @@ struct MutArgs {
     uint2 *hcnts;      // delete tables: (requested, claimed)
     uint32_t *hcnt;    // per vertex: holes found (zero between calls)
     uint8_t *dclass;   // per row: delete class of this call (kDelClass*)
+    const uint8_t *skip;  // grouped batches: 1 = record cancelled (adds [0,na), deletes [na,na+nd)); may be null
     uint2 *dtiles;     // (row, first slot) of each delete scan tile
     struct AddPiece *pieces;  // copy pieces of the insert phase (AddPiece below)
     StatShard *stats;
@@ -172,7 +173,7 @@ __device__ __forceinline__ uint32_t ht_hash(uint32_t x, uint32_t bits) {
 // destination u.
 struct MutRecord {
     uint32_t row, dst;
-    bool del, valid, mirror;
+    bool del, valid, bad, mirror;
 };
 
 __device__ __forceinline__ MutRecord mut_record(const MutArgs &a, uint32_t i) {
@@ -182,7 +183,8 @@ __device__ __forceinline__ MutRecord mut_record(const MutArgs &a, uint32_t i) {
     const uint32_t j = r.mirror ? i - total : i;
     r.del = j < a.nd;
     const cig_edge e = r.del ? a.dels[j] : a.adds[j - a.nd];
-    r.valid = e.u < a.n && e.v < a.n;
+    r.bad = !(e.u < a.n && e.v < a.n);
+    r.valid = !r.bad && !(a.skip && a.skip[r.del ? a.na + j : j - a.nd]);
     r.row = r.mirror ? a.n + e.v : e.u;
     r.dst = r.mirror ? e.u : e.v;
     return r;
@@ -216,7 +218,7 @@ __global__ void k_count(MutArgs a) {
             if (i < total) {
                 const MutRecord r = mut_record(a, i);
                 if (!r.valid) {
-                    bad = !r.mirror;
+                    bad = r.bad && !r.mirror;
                 } else {
                     u[k] = r.row;
                     const uint64_t old = atomicAdd((unsigned long long *)&a.cnt64[r.row],
@@ -687,6 +689,128 @@ __global__ __launch_bounds__(kBlock) void k_add_copy(MutArgs a) {
     }
 }
 
+// ------------------------------------------------------- grouped batches
+// Several consecutive batches applied as one: every delete is matched to
+// the latest earlier unmatched add of the same (u,v) in the group (adds of
+// batch b come after its deletes); matched pairs cancel, since the delete
+// removes an instance the add created. The records left then give the
+// sequential result when all deletes are applied before all adds, so the
+// group needs one mutation and one fixed point. Only (u,v) keys that occur
+// both as add and as delete in the group are examined one by one.
+struct GroupArgs {
+    const cig_edge *dels;
+    uint32_t nd;
+    const cig_edge *adds;
+    uint32_t na;
+    const uint32_t *del_cut;  // nb+1 prefix of the deletes per batch
+    const uint32_t *add_cut;  // nb+1 prefix of the adds per batch
+    uint32_t nb;
+    uint32_t n;
+    unsigned long long *tkey;  // (u<<32 | v) + 1; 0 = empty
+    uint32_t *thead;           // op list head (kNone = empty)
+    uint8_t *tconf;            // key seen as add and as delete
+    uint32_t tbits;
+    uint32_t *next_op;         // ops: adds [0,na), deletes [na,na+nd)
+    uint8_t *skip;             // 1 = cancelled op
+    uint32_t *overflow;        // set when a key holds more than kGroupOps ops
+    StatShard *stats;
+};
+constexpr uint32_t kGroupOps = 32;
+
+__device__ __forceinline__ uint32_t key_hash(unsigned long long k, uint32_t bits) {
+    return uint32_t((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+__global__ void k_group_adds(GroupArgs g) {
+    const uint32_t mask = (1u << g.tbits) - 1u;
+    for (uint32_t j = gthread(); j < g.na; j += gthreads()) {
+        const cig_edge e = g.adds[j];
+        if (e.u >= g.n || e.v >= g.n) continue;
+        const unsigned long long key = ((unsigned long long)e.u << 32 | e.v) + 1ull;
+        uint32_t h = key_hash(key, g.tbits);
+        for (;;) {  // the table has >= 2 * na slots: the probe ends
+            const unsigned long long prev = atomicCAS(&g.tkey[h], 0ull, key);
+            if (prev == 0ull || prev == key) break;
+            h = (h + 1u) & mask;
+        }
+        g.next_op[j] = atomicExch(&g.thead[h], j);
+    }
+}
+
+__global__ void k_group_dels(GroupArgs g) {
+    const uint32_t mask = (1u << g.tbits) - 1u;
+    for (uint32_t i = gthread(); i < g.nd; i += gthreads()) {
+        const cig_edge e = g.dels[i];
+        if (e.u >= g.n || e.v >= g.n) continue;
+        const unsigned long long key = ((unsigned long long)e.u << 32 | e.v) + 1ull;
+        for (uint32_t h = key_hash(key, g.tbits), probe = 0; probe <= mask; ++probe, h = (h + 1u) & mask) {
+            const unsigned long long k = g.tkey[h];
+            if (k == 0ull) break;
+            if (k == key) {
+                g.tconf[h] = 1;
+                g.next_op[g.na + i] = atomicExch(&g.thead[h], g.na + i);
+                break;
+            }
+        }
+    }
+}
+
+// Batch of entry `i` given the nb+1 prefix `cut` (largest b with cut[b] <= i).
+__device__ __forceinline__ uint32_t batch_of(const uint32_t *cut, uint32_t nb, uint32_t i) {
+    uint32_t lo = 0, hi = nb;  // cut[lo] <= i < cut[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (cut[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One thread per key seen as both add and delete: order its ops by time
+// (delete of batch b at 2b, add at 2b+1) and cancel matched pairs.
+__global__ void k_group_match(GroupArgs g) {
+    const uint32_t cap = 1u << g.tbits;
+    uint64_t cancel_d = 0, cancel_a = 0;
+    for (uint32_t h = gthread(); h < cap; h += gthreads()) {
+        if (!g.tconf[h]) continue;
+        uint32_t op[kGroupOps], tm[kGroupOps], cnt = 0;
+        bool over = false;
+        for (uint32_t o = g.thead[h]; o != kNone; o = g.next_op[o]) {
+            if (cnt == kGroupOps) {
+                over = true;
+                break;
+            }
+            const bool del = o >= g.na;
+            tm[cnt] = del ? 2u * batch_of(g.del_cut, g.nb, o - g.na) : 2u * batch_of(g.add_cut, g.nb, o) + 1u;
+            op[cnt++] = o;
+        }
+        if (over) {
+            atomicExch(g.overflow, 1u);
+            continue;
+        }
+        for (uint32_t x = 1; x < cnt; ++x)  // insertion sort by time
+            for (uint32_t y = x; y > 0 && tm[y - 1] > tm[y]; --y) {
+                const uint32_t t0 = tm[y]; tm[y] = tm[y - 1]; tm[y - 1] = t0;
+                const uint32_t o0 = op[y]; op[y] = op[y - 1]; op[y - 1] = o0;
+            }
+        uint32_t stack[kGroupOps], top = 0;
+        for (uint32_t x = 0; x < cnt; ++x) {
+            if (op[x] < g.na) {
+                stack[top++] = op[x];
+            } else if (top) {
+                g.skip[op[x]] = 1;
+                g.skip[stack[--top]] = 1;
+                ++cancel_d;
+                ++cancel_a;
+            }
+        }
+    }
+    // a cancelled pair applied both records (the delete found the instance)
+    StatShard *sh = stat_shard(g.stats);
+    stat_add(&sh->deletes_applied, wave_sum64(cancel_d));
+    stat_add(&sh->adds_applied, wave_sum64(cancel_a));
+}
+
 __global__ void k_clear_touched(MutArgs a) {
     const uint32_t touched = ld_acquire(&a.ctl->touched);
     for (uint32_t t = gthread(); t < touched; t += gthreads()) {
@@ -716,6 +840,7 @@ struct InvalArgs {
     uint32_t kind;  // 0 = single source (SSSP/BFS), 1 = vertex seeds (CC)
     uint32_t max_rounds;
     uint64_t seed;
+    const uint8_t *add_skip, *del_skip;  // grouped batches: cancelled records (may be null)
 };
 
 // reset_del_edges (driver.cuh:371-411): a deleted (u,v) whose v has parent u
@@ -728,7 +853,7 @@ __global__ void k_inval_seed(InvalArgs a) {
         uint32_t v = 0;
         if (i < a.nd) {
             const cig_edge e = a.dels[i];
-            if (e.u < a.n && e.v < a.n && uint32_t(a.state[e.v]) == e.u) {
+            if (e.u < a.n && e.v < a.n && !(a.del_skip && a.del_skip[i]) && uint32_t(a.state[e.v]) == e.u) {
                 v = e.v;
                 hit = bit_claim(a.aff, v);
             }
@@ -842,7 +967,8 @@ __global__ void k_insert_seed(InvalArgs a, uint32_t skip_affected) {
         uint32_t v = 0;
         if (i < a.na) {
             const cig_edge e = a.adds[i];
-            if (e.u < a.n && e.v < a.n && !(skip_affected && bit_test(a.aff, e.u))) {
+            if (e.u < a.n && e.v < a.n && !(a.add_skip && a.add_skip[i]) &&
+                !(skip_affected && bit_test(a.aff, e.u))) {
                 const uint32_t val = uint32_t(a.state[e.u] >> 32);
                 if (val != kInf) {
                     const uint64_t cand = uint64_t(val) + weight<W>(e.u, e.v, a.seed);
@@ -1107,9 +1233,9 @@ __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
 // a few rounds enqueued ahead and advances `round` after the chain, so no
 // workgroup waits for another and a round has no closing step. A round
 // whose frontier and deferred tiles are empty returns at once.
-template <int W, bool kStrict, int QPL>
-__global__ __launch_bounds__(kBlock) void k_relax_round(LoopArgs a, uint64_t *prog, uint64_t tag, uint32_t seq) {
-    __shared__ WaveLds lds[kWavesPerBlock];
+template <int W, bool kStrict, int QPL, int B = kBlock>
+__global__ __launch_bounds__(B) void k_relax_round(LoopArgs a, uint64_t *prog, uint64_t tag, uint32_t seq) {
+    __shared__ WaveLds lds[B / kWave];
     __shared__ RoundStage stage;
     Ctl *c = a.ctl;
     const uint32_t r = c->round + seq - 1;
@@ -1313,6 +1439,144 @@ __global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a, uint32_t seq) {
     expand_tiles<QPL>(op, a.cfg, tiles, nt, out);
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     out_flush(out);
+}
+
+// --------------------------------------------------- multi-source queries
+// Up to 64 BFS / SSSP sources at once on the live graph: dist holds 64
+// distances per vertex (vertex-major, `stride` words per vertex), act the
+// sources whose distance at the vertex improved since it was last expanded.
+// A round first takes each frontier vertex's mask (k_ms_consume: snap :=
+// act, act := 0), then expands the rows (k_ms_expand): for every edge (u,w)
+// and every source s in snap[u], dist[w][s] = min(dist[w][s],
+// dist[u][s] + w(u,w)); a destination that improved for any source joins
+// the next frontier. Label-correcting per source, so each distance is the
+// exact shortest-path value (unit weights: BFS depth).
+struct MsArgs {
+    Ctl *ctl;
+    const Row *rows;
+    uint32_t *dist;
+    unsigned long long *act, *snap;
+    uint32_t *stamp;
+    uint32_t *q0, *q1;
+    Tile *t0, *t1;
+    ExpandCfg cfg;
+    uint32_t n, stride;
+    uint64_t seed;
+};
+
+template <int W>
+struct MsOp {
+    const Row *rows;
+    uint32_t *dist;
+    unsigned long long *act;
+    const unsigned long long *snap;
+    uint32_t *stamp;
+    uint32_t stride, tag;
+    uint64_t seed;
+    __device__ bool source(uint32_t v, uint32_t, Row &row, uint32_t &pay) const {
+        if (snap[v] == 0ull) return false;
+        row = rows[v];
+        pay = 0;
+        return row.deg != 0;
+    }
+    __device__ bool tile_source(const Tile &t, uint32_t &pay) const {
+        pay = 0;
+        return snap[t.v] != 0ull;
+    }
+    __device__ uint64_t peek(uint32_t) const { return 0; }
+    __device__ bool edge(uint32_t u, uint32_t, uint32_t w, uint64_t) const {
+        const uint32_t wt = weight<W>(u, w, seed);
+        const uint32_t *du = dist + uint64_t(u) * stride;
+        uint32_t *dw = dist + uint64_t(w) * stride;
+        unsigned long long gained = 0;
+        for (unsigned long long m = snap[u]; m; m &= m - 1) {
+            const uint32_t s = uint32_t(__ffsll(m) - 1);
+            const uint64_t c = uint64_t(du[s]) + wt;
+            if (c >= dw[s]) continue;  // also rejects c >= kInf
+            if (atomicMin(&dw[s], uint32_t(c)) > uint32_t(c)) gained |= 1ull << s;
+        }
+        if (!gained) return false;
+        atomicOr(&act[w], gained);
+        return claim(stamp, w, tag);
+    }
+};
+
+__global__ void k_ms_seed(MsArgs a, const uint32_t *sources, uint32_t k) {
+    const uint32_t r = a.ctl->round;
+    for (uint32_t s = gthread(); s < k; s += gthreads()) {
+        const uint32_t v = sources[s];
+        a.dist[uint64_t(v) * a.stride + s] = 0;
+        atomicOr(&a.act[v], 1ull << s);
+        if (claim(a.stamp, v, r)) {
+            const uint32_t at = atomicAdd(&a.ctl->qcount[r % 3], 1u);
+            ((r & 1) ? a.q1 : a.q0)[at] = v;
+        }
+    }
+}
+
+// Round round+seq-1, phase 1: snapshot and clear the masks of the frontier.
+__global__ void k_ms_consume(MsArgs a, uint64_t *prog, uint64_t tag, uint32_t seq) {
+    Ctl *c = a.ctl;
+    const uint32_t r = c->round + seq - 1;
+    const uint32_t nq = c->qcount[r % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->qcount[(r + 2) % 3] = 0;
+        c->scount = 0;
+        if (nq) c->rounds += 1;
+        post_progress(prog, tag, nq != 0);
+    }
+    const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
+    for (uint32_t i = gthread(); i < nq; i += gthreads()) {
+        const uint32_t v = qin[i];
+        a.snap[v] = atomicExch(&a.act[v], 0ull);
+    }
+}
+
+// Phase 2: expand the frontier rows; rows of at least `split` slots become
+// tiles for phase 3 of the same round, so every part of a row is expanded
+// with the same snapshot.
+template <int W, int QPL>
+__global__ __launch_bounds__(kBlock) void k_ms_expand(MsArgs a, uint32_t seq) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ RoundStage stage;
+    Ctl *c = a.ctl;
+    const uint32_t r = c->round + seq - 1;
+    const uint32_t nq = c->qcount[r % 3];
+    if (nq == 0) return;
+    MsOp<W> op{a.rows, a.dist, a.act, a.snap, a.stamp, a.stride, r + 1, a.seed};
+    Counters ctr;
+    const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
+    const StagedOut<2048, 128> out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
+                                   TileSink{a.t0, &c->scount}, &stage};
+    out_init(out);
+    expand_range<QPL>(op, a.cfg, qin, nq, out, lds[wave_in_block()], ctr);
+    flush_counters(ctr, a.cfg.stats, a.cfg.stage);
+    out_flush(out);
+}
+
+// Phase 3: the tiles cut in phase 2.
+template <int W, int QPL>
+__global__ __launch_bounds__(kBlock) void k_ms_tiles(MsArgs a, uint32_t seq) {
+    __shared__ RoundStage stage;
+    Ctl *c = a.ctl;
+    const uint32_t r = c->round + seq - 1;
+    const uint32_t nt = c->scount;
+    if (nt == 0) return;
+    MsOp<W> op{a.rows, a.dist, a.act, a.snap, a.stamp, a.stride, r + 1, a.seed};
+    const StagedOut<2048, 128> out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
+                                   TileSink{nullptr, &c->scount}, &stage};
+    out_init(out);
+    expand_tiles<QPL>(op, a.cfg, a.t0, nt, out);
+    out_flush(out);
+}
+
+// Source-major result: out[s * n + v] = dist[v][s].
+__global__ void k_ms_gather(const uint32_t *dist, uint32_t n, uint32_t stride, uint32_t k, uint32_t *out) {
+    const uint64_t total = uint64_t(n) * k;
+    for (uint64_t i = gthread(); i < total; i += gthreads()) {
+        const uint32_t s = uint32_t(i / n), v = uint32_t(i % n);
+        out[i] = dist[uint64_t(v) * stride + s];
+    }
 }
 
 // Plain streaming copy (16 B per lane per step): the HBM reference point for

@@ -42,7 +42,8 @@ typedef enum cig_status {
     CIG_ESTATE = 4,   /* call out of order (reference: protocol abort) */
     CIG_EIO = 5,      /* file read/parse error */
     CIG_ELIMIT = 6,   /* a documented size limit was exceeded */
-    CIG_EINTERNAL = 7 /* internal invariant violated (reference: std::abort) */
+    CIG_EINTERNAL = 7, /* internal invariant violated (reference: std::abort) */
+    CIG_EPENDING = 8   /* serving: the request has not finished yet */
 } cig_status;
 
 /* App policy (reference: the AppBase subclass a sample instantiates). */
@@ -72,9 +73,11 @@ enum {
                                       cooperative launch with grid barriers
                                       instead of chained round kernels */
     CIG_CFG_NO_HOTNESS = 1u << 1,  /* do not count row activity */
-    CIG_CFG_SCAN_REPAIR = 1u << 2  /* BFS/SSSP/CC: keep no incoming-row index and
+    CIG_CFG_SCAN_REPAIR = 1u << 2, /* BFS/SSSP/CC: keep no incoming-row index and
                                       repair deletions by scanning every row
                                       (half the edge memory, slower repair) */
+    CIG_CFG_KERNEL_TIMING = 1u << 3 /* bracket every propagation round kernel with
+                                       HIP events (cig_batch_stats.round_kernel_ms) */
 };
 
 /* Engine configuration. Flags of the reference samples map as:
@@ -136,7 +139,10 @@ typedef struct cig_batch_stats {
     uint64_t stage_host_edges[4];
     double scan_kernel_ms;    /* the main repair kernel alone (k_repair_pull, or k_repair_scan
                                  with CIG_CFG_SCAN_REPAIR), 0 if not run */
-    uint64_t reserved[3];
+    double round_kernel_ms;   /* CIG_CFG_KERNEL_TIMING: summed durations of the propagation
+                                 round kernels (k_relax_round / k_pr_scatter) */
+    uint64_t round_launches;  /* propagation round kernels launched (incl. no-op lookahead) */
+    uint64_t reserved[1];
 } cig_batch_stats;
 
 typedef struct cig_graph_info {
@@ -243,6 +249,79 @@ cig_status cig_load_updates(cig_engine *e, const cig_edge *adds, uint64_t n_add,
 /* Applies the next resident batch (advances the reference's local_begin,
  * framework.cuh:2161-2162). Returns CIG_ELIMIT when no batch is left. */
 cig_status cig_apply_next_batch(cig_engine *e, uint32_t flags, cig_batch_stats *stats);
+/* ---- serving loop (no counterpart in the reference, whose samples run one
+ * batch at a time on the caller's thread) ----------------------------------
+ * A server thread owns the engine: requests are queued FIFO and served in
+ * submission order. Consecutive update requests (one batch each) are
+ * applied as one group (cig_apply_next_batches' method) and consecutive
+ * queries of one algorithm share one multi-source run of up to 64 sources,
+ * so the result of every request equals the result it has when served alone
+ * at the same position of the order (same graph version). A group is only
+ * admitted when its staging memory fits in free HBM (hipMemGetInfo) minus
+ * hbm_reserve; a request that cannot fit even alone fails with CIG_ENOMEM.
+ * While a server runs, the engine may only be used through it. */
+typedef struct cig_server cig_server;
+
+typedef struct cig_serve_config {
+    uint32_t max_group;          /* update requests per group (default 32) */
+    uint32_t max_query_sources;  /* sources merged into one query run (1..64, default 64) */
+    uint64_t hbm_reserve;        /* bytes of free HBM admission leaves untouched */
+    uint64_t reserved[4];
+} cig_serve_config;
+
+typedef struct cig_request_info {
+    uint32_t status;     /* cig_status of the request (CIG_EPENDING while queued or running) */
+    uint32_t kind;       /* 0 update, 1 query */
+    uint64_t version;    /* update requests applied before (query) / including (update) it */
+    double submit_ms, start_ms, done_ms;  /* server clock: ms since cig_serve_start */
+    uint32_t group_size; /* requests served by the same launch chain */
+    uint32_t reserved0;
+    cig_batch_stats stats; /* of the group or merged query it ran in */
+} cig_request_info;
+
+cig_status cig_serve_config_default(cig_serve_config *cfg);
+cig_status cig_serve_start(cig_engine *e, const cig_serve_config *cfg, cig_server **out);
+/* Copies the records; dels/adds follow cig_apply_batch's semantics. */
+cig_status cig_submit_update(cig_server *s, const cig_edge *dels, uint64_t nd, const cig_edge *adds,
+                             uint64_t na, uint64_t *ticket);
+/* dist (nsrc*n words, source-major as in cig_query) must stay valid until
+ * the request is reported done. */
+cig_status cig_submit_query(cig_server *s, uint32_t algo, const uint32_t *sources, uint32_t nsrc,
+                            uint32_t *dist, uint64_t dist_bytes, uint64_t *ticket);
+/* CIG_EPENDING until the request is done; then fills info and forgets the ticket. */
+cig_status cig_poll(cig_server *s, uint64_t ticket, cig_request_info *info);
+cig_status cig_wait(cig_server *s, uint64_t ticket, cig_request_info *info);
+/* Serves what is queued, then stops the thread and frees the server. */
+cig_status cig_serve_stop(cig_server *s);
+
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed
+ * by --block_size and the CTA scheduler): times the fixed point from
+ * `source` on the loaded graph over edges-per-lane x split degree x tile
+ * length x relax workgroup size x relax workgroups per CU (coordinate
+ * descent, best of two runs each) and keeps the fastest. Results do not
+ * depend on the shape. Leaves the engine initialised and converged from
+ * `source`. BFS/SSSP/CC engines with chained rounds. */
+typedef struct cig_tune_result {
+    uint32_t edges_per_lane, split_degree, tile_len, block_size, round_blocks_per_cu, trials;
+    double first_ms, best_ms; /* fixed point with the starting and the chosen shape */
+    uint64_t reserved[2];
+} cig_tune_result;
+cig_status cig_autotune(cig_engine *e, uint32_t source, cig_tune_result *result);
+/* Multi-source query (no counterpart in the reference's samples): BFS depths
+ * or SSSP distances (the engine's weight mode) from up to 64 sources at once
+ * on the live graph, both tiers. dist receives nsrc*n words, source-major:
+ * dist[s*n + v], UINT32_MAX for unreachable. The engine's own incremental
+ * state is not touched. */
+cig_status cig_query(cig_engine *e, uint32_t algo, const uint32_t *sources, uint32_t nsrc, uint32_t *dist,
+                     uint64_t dist_bytes, cig_batch_stats *stats);
+/* Batched form (no counterpart in the reference, whose driver applies one
+ * batch per call): the next `count` resident batches as one group. Every
+ * delete is matched to the latest earlier add of the same (u,v) in the
+ * group and the pair cancels; the rest is applied deletes-first with one
+ * mutation and one fixed point. Values, parents' validity and the topology
+ * multiset after the call equal applying the batches one by one; statistics
+ * cover the whole group. Not allowed between the two phases of a batch. */
+cig_status cig_apply_next_batches(cig_engine *e, uint32_t count, uint32_t flags, cig_batch_stats *stats);
 uint32_t cig_update_batches_left(const cig_engine *e);
 
 /* ---- edge cache (hot rows in HBM, cold rows zero-copy from pinned host) --

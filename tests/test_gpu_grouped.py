@@ -1,0 +1,101 @@
+"""Batched form: consecutive update batches applied as one group
+(cig_apply_next_batches) must leave exactly the state of applying them one
+by one: bit-exact values, valid parents, the same topology and incoming-row
+multisets, and the same applied / missing / invalid counts. The stream is
+built so that later batches delete edges that earlier batches of the group
+added (and re-add deleted ones), with parallel edges and missing records."""
+import numpy as np
+import pytest
+
+import coincgraph_amd as cg
+from tests.helpers import (check_incoming, check_topology, check_traversal, oracle_apply,
+                           oracle_graph, pr_errors)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def _conflict_stream(g, nb, size, seed):
+    rng = np.random.default_rng(seed)
+    src = np.repeat(np.arange(g.n), np.diff(g.offsets).astype(np.int64))
+    adds, dels, sizes = [], [], []
+    prev_adds = None
+    prev_dels = None
+    for b in range(nb):
+        pick = rng.integers(0, len(src), size)
+        du, dv = src[pick], g.dst[pick]
+        if prev_adds is not None:  # delete some of the previous batch's adds (twice for a few)
+            k = size // 3
+            du = np.concatenate([du, prev_adds[0][:k], prev_adds[0][:8]])
+            dv = np.concatenate([dv, prev_adds[1][:k], prev_adds[1][:8]])
+        du = np.concatenate([du, rng.integers(0, g.n, 20), [g.n + 3]])
+        dv = np.concatenate([dv, rng.integers(0, g.n, 20), [0]])
+        au = rng.integers(0, g.n, size)
+        av = rng.integers(0, g.n, size)
+        if prev_dels is not None:  # re-add some edges the previous batch deleted
+            k = size // 4
+            au = np.concatenate([au, prev_dels[0][:k]])
+            av = np.concatenate([av, prev_dels[1][:k]])
+        au = np.concatenate([au, au[:10], [1]])  # parallel adds, one invalid
+        av = np.concatenate([av, av[:10], [g.n + 7]])
+        adds.append(cg.edges_array(au, av))
+        dels.append(cg.edges_array(du, dv))
+        sizes.append((len(au), len(du)))
+        prev_adds, prev_dels = (au, av), (du, dv)
+    return cg.UpdateStream(np.concatenate(adds), np.concatenate(dels),
+                           np.asarray(sizes, dtype=np.uint32).reshape(-1, 2))
+
+
+@pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_HASH64), ("bfs", 0), ("cc", 0)])
+def test_grouped_batches_match_sequential(algo, wmode):
+    print(f"[gpu-case] grouped {algo}", flush=True)
+    g = cg.rmat_graph(12, 16, seed=41, symmetric=algo == "cc")
+    s = _conflict_stream(g, 9, 600, 42)
+    og = oracle_graph(g)
+    with cg.Engine(algo, weight_mode=wmode, weight_seed=3) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        e.load_updates(s)
+        b = 0
+        for k in (1, 3, 5):
+            st = e.apply_next_batches(k)
+            miss = inval = applied = added = 0
+            for j in range(b, b + k):
+                dels, adds = s.batch(j)
+                m, i = oracle_apply(og, dels, adds)
+                miss += m
+                inval += i
+                applied += len(dels) - m - int(((dels["u"] >= g.n) | (dels["v"] >= g.n)).sum())
+                added += len(adds) - int(((adds["u"] >= g.n) | (adds["v"] >= g.n)).sum())
+            b += k
+            assert (st["missing_deletes"], st["invalid_updates"]) == (miss, inval), k
+            assert (st["deletes_applied"], st["adds_applied"]) == (applied, added), k
+            check_traversal(e, og, algo, 0, wmode, 3, where=f"group of {k}")
+            check_topology(e, og, f"group of {k}")
+            check_incoming(e, og, f"group of {k}")
+        assert e.batches_left() == 0
+
+
+def test_grouped_pagerank_matches_fp64():
+    print("[gpu-case] grouped pr", flush=True)
+    g = cg.rmat_graph(11, 8, seed=43)
+    s = _conflict_stream(g, 6, 300, 44)
+    og = oracle_graph(g)
+    with cg.Engine("pr", pr_epsilon=1e-7, pr_max_rounds=2000) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        e.load_updates(s)
+        e.apply_next_batches(2)
+        e.apply_next_batches(4)
+        for j in range(6):
+            oracle_apply(og, *s.batch(j))
+        check_topology(e, og, "pr group")
+        l1n, _ = pr_errors(e.values(), og.pagerank()[0])
+        assert l1n <= 1e-6

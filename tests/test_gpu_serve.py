@@ -1,0 +1,83 @@
+"""Serving loop (cig_serve_*): requests submitted together are coalesced
+(update groups, merged multi-source queries) and every result equals the
+result of the same request served alone at the same position of the
+submission order. Admission by free HBM refuses what cannot fit."""
+import numpy as np
+import pytest
+
+import coincgraph_amd as cg
+from oracle import oracle as O
+from tests.helpers import check_traversal, oracle_apply, oracle_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+PLAN = ["U", ("sssp", [0, 5]), "U", "U", ("bfs", [3]), ("sssp", [7, 0, 9]), "U", "U", "U",
+        ("sssp", [0]), "U", "U", ("bfs", [0, 1]), ("bfs", list(range(40)))]
+
+
+def test_serving_alone_equals_mixed():
+    print("[gpu-case] serve mixed", flush=True)
+    g = cg.rmat_graph(11, 16, seed=61)
+    nupd = sum(1 for p in PLAN if p == "U")
+    s = cg.rmat_stream(g, 11, nupd, 400, 400, seed=62)
+    kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=4)
+    # mixed: everything submitted at once to one server
+    with cg.Engine("sssp", **kw) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        with cg.Server(e, max_group=4) as srv:
+            tickets, b = [], 0
+            for p in PLAN:
+                if p == "U":
+                    tickets.append(srv.submit_update(*s.batch(b)))
+                    b += 1
+                else:
+                    tickets.append(srv.submit_query(p[0], p[1]))
+            mixed = [srv.wait(t) for t in tickets]
+        final = e.values().copy()
+    assert all(r["status"] == 0 for r in mixed), [r["status"] for r in mixed]
+    assert max(r["group_size"] for r in mixed) > 1, "nothing was coalesced"
+    # alone: the same requests one at a time on a fresh engine
+    og = oracle_graph(g)
+    with cg.Engine("sssp", **kw) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        b = 0
+        for p, r in zip(PLAN, mixed):
+            if p == "U":
+                dels, adds = s.batch(b)
+                e.apply_batch(dels, adds)
+                oracle_apply(og, dels, adds)
+                b += 1
+                assert r["version"] == b
+            else:
+                alone = e.query(p[0], p[1])
+                assert r["version"] == b
+                assert np.array_equal(r["result"], alone), f"query {p} differs from alone"
+        assert np.array_equal(final, e.values())
+        check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 4, where="after serving")
+
+
+def test_serving_admission_by_free_hbm():
+    print("[gpu-case] serve admission", flush=True)
+    g = cg.rmat_graph(9, 8, seed=63)
+    s = cg.rmat_stream(g, 9, 1, 50, 50, seed=64)
+    with cg.Engine("bfs") as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        before = e.values().copy()
+        with cg.Server(e, hbm_reserve=1 << 62) as srv:
+            u = srv.wait(srv.submit_update(*s.batch(0)))
+            q = srv.wait(srv.submit_query("bfs", [0]))
+        assert u["status"] == cg.ENOMEM and q["status"] == cg.ENOMEM
+        assert np.array_equal(e.values(), before)

@@ -1,0 +1,36 @@
+"""Autotuner (cig_autotune): picks a launch shape from its candidate sets,
+never slower than where it started, and the engine keeps bit-exact results
+with the chosen shape through later batches."""
+import pytest
+
+import coincgraph_amd as cg
+from tests.helpers import check_traversal, oracle_apply, oracle_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+@pytest.mark.parametrize("algo", ["sssp", "cc"])
+def test_autotune_keeps_results(algo):
+    print(f"[gpu-case] autotune {algo}", flush=True)
+    g = cg.rmat_graph(12, 16, seed=71, symmetric=algo == "cc")
+    s = cg.rmat_stream(g, 12, 3, 500, 500, seed=72, symmetric=algo == "cc")
+    og = oracle_graph(g)
+    with cg.Engine(algo, weight_mode=cg.WEIGHT_HASH64, weight_seed=2) as e:
+        e.load_graph(g)
+        r = e.autotune(0)
+        assert r["edges_per_lane"] in (4, 8, 16) and r["split_degree"] in (32, 64, 128, 256)
+        assert r["tile_len"] in (256, 512, 1024) and r["block_size"] in (256, 512)
+        assert r["round_blocks_per_cu"] in (1, 2, 4) and r["trials"] > 10
+        assert r["best_ms"] <= r["first_ms"]
+        check_traversal(e, og, algo, 0, cg.WEIGHT_HASH64, 2, where="after autotune")
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            e.apply_batch(dels, adds)
+            oracle_apply(og, dels, adds)
+            check_traversal(e, og, algo, 0, cg.WEIGHT_HASH64, 2, where=f"tuned batch {b}")
