@@ -1,21 +1,24 @@
 #!/usr/bin/env python3
 """Headline benchmark: incremental SSSP batch updates on one MI355X.
 
-Workload (default): R-MAT scale 22 (4,194,304 vertices, edge factor 16,
-a/b/c/d = 0.57/0.19/0.19/0.05, self-loops and duplicates removed, ids not
-permuted), source 0, weights 1 + splitmix64(seed ^ (u<<32|v)) % 64 (uniform
-in [1,64]). Each step is one update batch of 100,000 deletes sampled
-uniformly from the current edges and 100,000 new R-MAT inserts, applied by
-the HIP engine with the whole graph resident in HBM and the update stream
-already resident in HBM. Distances are exact (the parity tests check them
-bit for bit against the CPU oracle).
+Default workload (--config 2, the headline): R-MAT scale 22 (4,194,304
+vertices, edge factor 16, a/b/c/d = 0.57/0.19/0.19/0.05, self-loops and
+duplicates removed, ids not permuted), source 0, weights
+1 + splitmix64(seed ^ (u<<32|v)) % 64 (uniform in [1,64]). Each step is one
+update batch of 100,000 deletes sampled uniformly from the current edges and
+100,000 new R-MAT inserts, applied by the HIP engine with the whole graph
+resident in HBM and the update stream already resident in HBM. Distances are
+exact (the parity tests check them bit for bit against the CPU oracle).
 
 Prints ONE JSON line: value = GTEPS (edges scanned by the invalidation,
 repair and relax kernels / wall time of the timed batches), with p50/p99
-batch latency, updates/s, the roofline of the dominant kernel and a CPU
-baseline (the oracle's restatement of the reference CPU path: multigraph
-mutation + label-correcting full recompute, single thread, a bounded sample
-of the same batches).
+batch latency, updates/s, the batched form (groups of batches in one call),
+the roofline of the dominant kernel and a CPU baseline (the oracle's
+restatement of the reference CPU path: multigraph mutation + full
+recompute, single thread, a bounded sample of the same batches).
+
+--config 1 / 3 / 4 run the other incremental workloads at their own sizes
+(BFS S16, CC on a symmetrised S24, PageRank S25) with the same report.
 """
 from __future__ import annotations
 
@@ -32,7 +35,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_relax_sssp_s22.txt")
+
+# algo, scale, edge factor, symmetric, deletes/inserts per batch (pairs when
+# symmetric), timed per-batch steps, batches per group
+CONFIGS = {
+    1: dict(algo="bfs", scale=16, ef=16, sym=False, batch=1000, steps=100, group=10,
+            what="incremental BFS, R-MAT scale 16 edge factor 16, source 0, batches of 1000 deletes + "
+                 "1000 inserts"),
+    2: dict(algo="sssp", scale=22, ef=16, sym=False, batch=100_000, steps=100, group=10,
+            what="incremental SSSP, R-MAT scale 22 edge factor 16 (a,b,c,d=0.57,0.19,0.19,0.05, deduped, "
+                 "unpermuted), source 0, uint32 weights uniform in [1,64], batches of 100000 deletes + "
+                 "100000 inserts, graph and stream resident in HBM"),
+    3: dict(algo="cc", scale=24, ef=16, sym=True, batch=1_000_000, steps=10, group=10,
+            what="incremental min-label CC, symmetrised R-MAT scale 24 edge factor 16 (both directions "
+                 "stored), batches of 1M symmetric delete pairs + 1M symmetric insert pairs"),
+    4: dict(algo="pr", scale=25, ef=16, sym=False, batch=500_000, steps=10, group=10,
+            what="incremental PageRank (alpha 0.85, epsilon 1e-7), R-MAT scale 25 edge factor 16, "
+                 "batches of 500000 deletes + 500000 inserts"),
+}
 
 
 def log(msg):
@@ -43,12 +63,14 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
 
 
-def stage_bytes(st, stage):
-    """Algorithmic bytes of one expansion pass: 8 B per scanned edge (4-B
-    destination id + 4-B destination state touch) + 28 B per expanded row
-    (16-B descriptor, 8-B state word, 4-B worklist entry); weights are not
-    stored."""
-    return 8.0 * st["stage_edges"][stage] + 28.0 * st["stage_vertices"][stage]
+def stage_bytes(st, stage, pr=False):
+    """Algorithmic bytes of one expansion pass. Traversal: 8 B per scanned
+    edge (4-B destination id + 4-B destination state touch) + 28 B per
+    expanded row (16-B descriptor, 8-B state word, 4-B worklist entry).
+    PageRank: 8 B per edge (id + residual atomic) + 40 B per consumed vertex.
+    Weights are not stored."""
+    per_v = 40.0 if pr else 28.0
+    return 8.0 * st["stage_edges"][stage] + per_v * st["stage_vertices"][stage]
 
 
 def read_traffic(path):
@@ -64,10 +86,11 @@ def read_traffic(path):
     return None
 
 
-def cpu_baseline(g, stream, batches, wseed, budget_s):
-    """Reference CPU path on the same batches: chunk-store mutation + full
-    label-correcting SSSP recompute, single-threaded (the reference's --check
-    recompute and SSSPHostNaive, restated in oracle/)."""
+def cpu_baseline(g, stream, batches, algo, wseed, budget_s):
+    """Reference CPU path on the same batches, single-threaded: chunk-store
+    mutation + full recompute (label-correcting SSSP as SSSPHostNaive with
+    the engine's +1, queue BFS, CCCheck flood, FP64 Jacobi PageRank), the
+    reference's --check oracles restated in oracle/."""
     from oracle import oracle as O
     og = O.Graph(g.n, g.offsets, g.dst)
     done = 0
@@ -78,19 +101,31 @@ def cpu_baseline(g, stream, batches, wseed, budget_s):
         t0 = time.perf_counter()
         og.apply_deletes(dels["u"], dels["v"])
         og.apply_adds(adds["u"], adds["v"])
-        _, sc = og.sssp_label_correcting(0, O.W_HASH64, wseed, with_count=True)
+        if algo == "sssp":
+            _, sc = og.sssp_label_correcting(0, O.W_HASH64, wseed, with_count=True)
+        elif algo == "bfs":
+            og.bfs(0)
+            sc = og.edge_count()
+        elif algo == "cc":
+            og.cc()
+            sc = og.edge_count()
+        else:
+            _, rounds = og.pagerank()
+            sc = og.edge_count() * max(int(rounds), 1)
         t_total += time.perf_counter() - t0
         scanned += sc
         done += 1
         log(f"cpu baseline batch {b}: {t_total / done * 1e3:.0f} ms/batch")
         if t_total >= budget_s:
             break
+    what = {"sssp": "label-correcting SSSP", "bfs": "queue BFS", "cc": "flood-fill CC",
+            "pr": "FP64 Jacobi PageRank"}[algo]
     return {
         "value": round(scanned / t_total / 1e9, 4),
         "unit": "GTEPS",
         "cores": 1,
         "kind": "port",
-        "sample": (f"{done} batches of the same stream: mutation + label-correcting SSSP "
+        "sample": (f"{done} batches of the same stream: mutation + {what} "
                    f"recompute, {t_total / done * 1e3:.1f} ms/batch, "
                    f"{scanned / done / 1e6:.1f} M edges scanned/batch"),
         "ms_per_batch": round(t_total / done * 1e3, 3),
@@ -100,20 +135,22 @@ def cpu_baseline(g, stream, batches, wseed, budget_s):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=None, help="timed per-batch steps")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--batch", type=int, default=100_000, help="deletes and inserts per batch")
+    ap.add_argument("--scale", type=int, default=None)
+    ap.add_argument("--edge-factor", type=int, default=None)
+    ap.add_argument("--batch", type=int, default=None, help="deletes and inserts per batch")
     ap.add_argument("--seed", type=int, default=2022)
     ap.add_argument("--weight-seed", type=int, default=64)
     ap.add_argument("--staged", action="store_true", help="reference two-stage batches")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--check", action="store_true", help="verify the final distances on the CPU")
-    ap.add_argument("--group", type=int, default=10,
+    ap.add_argument("--check", action="store_true", help="verify the final values on the CPU")
+    ap.add_argument("--group", type=int, default=None,
                     help="batched form: batches per group (cig_apply_next_batches)")
-    ap.add_argument("--group-steps", type=int, default=10, help="batched form: timed groups (0 = skip)")
+    ap.add_argument("--group-steps", type=int, default=None, help="batched form: timed groups (0 = skip)")
+    ap.add_argument("--autotune", action="store_true", help="pick the launch shape first (cig_autotune)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -121,23 +158,42 @@ def main(argv=None):
               f"(--gpus {args.gpus} requested). Run with --gpus 1.", file=sys.stderr)
         return 2
 
+    c = CONFIGS[args.config]
+    algo = c["algo"]
+    scale = args.scale or c["scale"]
+    ef = args.edge_factor or c["ef"]
+    batch = args.batch or c["batch"]
+    steps = args.steps if args.steps is not None else c["steps"]
+    group = args.group or c["group"]
+    group_steps = args.group_steps if args.group_steps is not None else (10 if args.config <= 2 else 1)
+    pr = algo == "pr"
+    mult = 2 if c["sym"] else 1  # directed records per requested update
+
     import coincgraph_amd as cg
 
     t0 = time.perf_counter()
-    g = cg.rmat_graph(args.scale, args.edge_factor, seed=args.seed)
+    g = cg.rmat_graph(scale, ef, seed=args.seed, symmetric=c["sym"])
     log(f"graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f} s)")
-    nb = args.warmup + args.steps + args.group * args.group_steps
+    nb = args.warmup + steps + group * group_steps
     t0 = time.perf_counter()
-    stream = cg.rmat_stream(g, args.scale, nb, args.batch, args.batch, seed=args.seed + 1)
-    log(f"stream: {nb} batches of {args.batch}+{args.batch} ({time.perf_counter() - t0:.1f} s)")
+    stream = cg.rmat_stream(g, scale, nb, batch, batch, seed=args.seed + 1, symmetric=c["sym"])
+    log(f"stream: {nb} batches of {batch}+{batch}{' pairs' if c['sym'] else ''} "
+        f"({time.perf_counter() - t0:.1f} s)")
 
-    eng = cg.Engine("sssp", weight_mode=cg.WEIGHT_HASH64, weight_seed=args.weight_seed,
-                    flags=cg.CFG_KERNEL_TIMING)
+    kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=args.weight_seed, flags=cg.CFG_KERNEL_TIMING)
+    if pr:
+        kw.update(pr_epsilon=1e-7, pr_max_rounds=1000)
+    eng = cg.Engine(algo, **kw)
     eng.load_graph(g)
-    eng.init_graph(0)
-    st0 = eng.start()
-    log(f"initial fixed point: {st0['total_ms']:.2f} ms, rounds={st0['rounds']}, "
-        f"E_scan={st0['edges_scanned']}")
+    tune = None
+    if args.autotune and not pr:
+        tune = eng.autotune(0)
+        log(f"autotune: {tune}")
+    else:
+        eng.init_graph(0)
+        st0 = eng.start()
+        log(f"initial fixed point: {st0['total_ms']:.2f} ms, rounds={st0['rounds']}, "
+            f"E_scan={st0['edges_scanned']}")
     eng.load_updates(stream)
     for _ in range(args.warmup):
         eng.apply_next_batch(staged=args.staged)
@@ -145,55 +201,56 @@ def main(argv=None):
 
     stats = []
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         stats.append(eng.apply_next_batch(staged=args.staged))
         if (k + 1) % 25 == 0:
-            log(f"timed batches done: {k + 1}/{args.steps}")
+            log(f"timed batches done: {k + 1}/{steps}")
     wall = time.perf_counter() - t0
     info = eng.info()
 
     # Batched form: the next batches applied `group` at a time (one mutation
     # and one fixed point per group; same final state as one by one).
     grouped = None
-    if args.group_steps:
+    if group_steps:
         gstats = []
         t1 = time.perf_counter()
-        for _ in range(args.group_steps):
-            gstats.append(eng.apply_next_batches(args.group))
+        for _ in range(group_steps):
+            gstats.append(eng.apply_next_batches(group))
         gwall = time.perf_counter() - t1
         glat = [x["total_ms"] for x in gstats]
-        gup = args.group_steps * args.group * 2 * args.batch
+        gup = group_steps * group * 2 * batch * mult
         grouped = {
-            "batches_per_group": args.group,
-            "groups": args.group_steps,
+            "batches_per_group": group,
+            "groups": group_steps,
             "updates_per_s": round(gup / gwall, 1),
             "ms_per_group": {"p50": round(percentile(glat, 50), 4), "p99": round(percentile(glat, 99), 4)},
-            "speedup_vs_per_batch": round((gup / gwall) / (args.steps * 2 * args.batch / wall), 3),
+            "speedup_vs_per_batch": round((gup / gwall) / (steps * 2 * batch * mult / wall), 3),
         }
         log(f"batched form: {grouped}")
 
     lat = [s["total_ms"] for s in stats]
     e_scan = sum(s["edges_scanned"] for s in stats)
-    updates = args.steps * 2 * args.batch
+    updates = steps * 2 * batch * mult
     gteps = e_scan / wall / 1e9
 
-    stage_ms = {"relax": sum(s["propagate_ms"] for s in stats),
+    stage_ms = {"propagate": sum(s["propagate_ms"] for s in stats),
                 "repair": sum(s["repair_ms"] for s in stats),
                 "mutation": sum(s["mutate_ms"] for s in stats),
                 "invalidation": sum(s["invalidate_ms"] for s in stats)}
-    # Dominant kernel: k_relax_round, the propagation round kernel (about ten
-    # launches per batch). Its duration comes from the HIP event pair the
-    # engine records around every launch on its own stream; its algorithmic
-    # bytes are the relax pass's 8 B per scanned edge + 28 B per expanded
-    # row, spread over all its launches (no-op lookahead rounds included, as
-    # rocprofv3 counts them).
+    # Dominant kernel: the propagation round kernel (k_relax_round, or
+    # k_pr_scatter for PageRank), about ten launches per batch. Its duration
+    # comes from the HIP event pair the engine records around every launch
+    # on its own stream; its algorithmic bytes are the propagation pass's
+    # (stage_bytes) spread over all its launches (no-op lookahead rounds
+    # included, as rocprofv3 counts them).
     launches = sum(s["round_launches"] for s in stats)
     kern_ms = sum(s["round_kernel_ms"] for s in stats)
-    relax_bytes = sum(stage_bytes(s, 2) for s in stats)
-    bytes_per_launch = relax_bytes / max(launches, 1)
+    prop_bytes = sum(stage_bytes(s, 2, pr) for s in stats)
+    bytes_per_launch = prop_bytes / max(launches, 1)
     ms_per_launch = kern_ms / max(launches, 1)
     achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9 if ms_per_launch > 0 else 0.0
-    traffic = read_traffic(PROFILE_TRAFFIC)
+    kernel = "k_pr_scatter" if pr else "k_relax_round"
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{kernel}_config{args.config}.txt"))
 
     if args.check:
         from oracle import oracle as O
@@ -202,48 +259,55 @@ def main(argv=None):
             d, a = stream.batch(b)
             og.apply_deletes(d["u"], d["v"])
             og.apply_adds(a["u"], a["v"])
-        ok = np.array_equal(eng.values(), og.sssp(0, O.W_HASH64, args.weight_seed))
-        log(f"check vs oracle after {nb} batches: {'ok' if ok else 'MISMATCH'}")
+        if pr:
+            ref = og.pagerank()[0]
+            l1n = float(np.abs(eng.values().astype(np.float64) - ref).sum() / np.abs(ref).sum())
+            ok = l1n <= 1e-6
+            log(f"check vs FP64 oracle after {nb} batches: normalised L1 {l1n:.3g} ({'ok' if ok else 'FAIL'})")
+        else:
+            exp = {"sssp": lambda: og.sssp(0, O.W_HASH64, args.weight_seed), "bfs": lambda: og.bfs(0),
+                   "cc": og.cc}[algo]()
+            ok = np.array_equal(eng.values(), exp)
+            log(f"check vs oracle after {nb} batches: {'ok' if ok else 'MISMATCH'}")
         if not ok:
             return 1
 
     eng.close()
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(g, stream, nb, args.weight_seed, args.cpu_seconds)
+        cpu = cpu_baseline(g, stream, nb, algo, args.weight_seed, args.cpu_seconds)
 
+    default_shape = (args.scale, args.edge_factor, args.batch) == (None, None, None)
     out = {
         "metric": "batch-update p50/p99 latency + GTEPS (% HBM peak), incremental SSSP, 1 GPU",
         "value": round(gteps, 4),
         "unit": "GTEPS",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(wall / steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "f32" if pr else "u32",
         "data": "synthetic",
         "config": {
-            "workload": (f"incremental SSSP, R-MAT scale {args.scale} edge factor {args.edge_factor} "
-                         f"(a,b,c,d=0.57,0.19,0.19,0.05, deduped, unpermuted), source 0, "
-                         f"uint32 weights uniform in [1,64], batches of {args.batch} deletes + "
-                         f"{args.batch} inserts, graph and stream resident in HBM"),
-            "vertices": g.n, "edges": g.m, "batch_deletes": args.batch,
-            "batch_inserts": args.batch, "mode": "staged" if args.staged else "fused",
+            "workload": c["what"] if default_shape else
+            f"{algo} R-MAT scale {scale} edge factor {ef}, batches of {batch} deletes + {batch} inserts",
+            "config": args.config, "vertices": g.n, "edges": g.m, "batch_deletes": batch,
+            "batch_inserts": batch, "symmetric_pairs": c["sym"], "mode": "staged" if args.staged else "fused",
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4),
                        "mean": round(float(np.mean(lat)), 4), "max": round(max(lat), 4)},
         "updates_per_s": round(updates / wall, 1),
-        "edges_scanned_per_batch": int(e_scan / args.steps),
-        "stage_ms_per_batch": {k: round(v / args.steps, 4) for k, v in stage_ms.items()},
-        "repair_kernel_ms_per_batch": round(sum(s["scan_kernel_ms"] for s in stats) / args.steps, 4),
+        "edges_scanned_per_batch": int(e_scan / steps),
+        "stage_ms_per_batch": {k: round(v / steps, 4) for k, v in stage_ms.items()},
+        "repair_kernel_ms_per_batch": round(sum(s["scan_kernel_ms"] for s in stats) / steps, 4),
         "affected_per_batch": round(float(np.mean([s["affected"] for s in stats])), 1),
-        "batched_form": grouped,
         "rounds_per_batch": round(float(np.mean([s["rounds"] for s in stats])), 1),
+        "batched_form": grouped,
         "roofline": {
-            "kernel": "k_relax_round (SSSP relax round, chained launches)",
+            "kernel": kernel,
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
@@ -252,12 +316,13 @@ def main(argv=None):
             "traffic": traffic,
             "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
             "ms_per_launch": round(ms_per_launch, 5),
-            "launches_per_batch": round(launches / args.steps, 2),
+            "launches_per_batch": round(launches / max(steps, 1), 2),
         },
         "cpu_baseline": cpu,
+        "autotune": tune,
         "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
-        "engine": {k: info[k] for k in ("grid_blocks", "block_size", "edges_per_lane",
-                                        "split_degree", "cu_count", "hbm_pool_slots")},
+        "engine": {k: info[k] for k in ("block_size", "edges_per_lane", "split_degree", "cu_count",
+                                        "hbm_pool_slots")},
     }
     print(json.dumps(out), flush=True)
     return 0

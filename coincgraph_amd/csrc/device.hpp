@@ -683,16 +683,12 @@ __device__ __forceinline__ void flush_counters(Counters &ctr, StatShard *stats, 
 }
 
 // Round progress for the host driver: every round launch posts exactly one
-// word to pinned host memory, (tag << 1) | more, where tag = chain epoch and
-// launch sequence number and `more` says whether the following round has
-// work. The closing workgroup posts it, or workgroup 0 when the round found
-// nothing to do.
+// word to pinned host memory, (tag << 1) | has_work, where tag = chain epoch
+// and launch sequence number; workgroup 0 posts it when the round starts.
+// The host reads nothing else through it (results are read after a stream
+// synchronisation), so the store needs no ordering, only system scope.
 __device__ __forceinline__ void post_progress(uint64_t *prog, uint64_t tag, bool more) {
-    __hip_atomic_store(prog, (tag << 1) | (more ? 1ull : 0ull), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-__device__ __forceinline__ void post_idle(uint64_t *prog, uint64_t tag) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) post_progress(prog, tag, false);
+    __hip_atomic_store(prog, (tag << 1) | (more ? 1ull : 0ull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace dev
