@@ -109,9 +109,9 @@ def cpu_baseline(g, stream, batches, algo, wseed, budget_s):
         elif algo == "cc":
             og.cc()
             sc = og.edge_count()
-        else:
-            _, rounds = og.pagerank()
-            sc = og.edge_count() * max(int(rounds), 1)
+        else:  # three sweeps of the FP64 Jacobi recompute (a full one needs ~170)
+            og.pagerank(max_rounds=3)
+            sc = og.edge_count() * 3
         t_total += time.perf_counter() - t0
         scanned += sc
         done += 1
@@ -119,7 +119,7 @@ def cpu_baseline(g, stream, batches, algo, wseed, budget_s):
         if t_total >= budget_s:
             break
     what = {"sssp": "label-correcting SSSP", "bfs": "queue BFS", "cc": "flood-fill CC",
-            "pr": "FP64 Jacobi PageRank"}[algo]
+            "pr": "3 sweeps of FP64 Jacobi PageRank"}[algo]
     return {
         "value": round(scanned / t_total / 1e9, 4),
         "unit": "GTEPS",

@@ -417,7 +417,8 @@ struct cig_engine {
         a.rows = rows.p;
         a.state = state.p;
         a.stamp = stamp.p;
-        a.hot = (cfg.flags & CIG_CFG_NO_HOTNESS) ? nullptr : hot.p;
+        // activity only feeds cache admission: no budget, no counting
+        a.hot = (cfg.flags & CIG_CFG_NO_HOTNESS) || !budget_slots ? nullptr : hot.p;
         a.q0 = q0.p;
         a.q1 = q1.p;
         a.t0 = t0.p;

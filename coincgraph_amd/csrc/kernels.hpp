@@ -137,10 +137,11 @@ struct MutArgs {
 // source_local_chunk_store.h:670-723, restated for the device):
 //  - tiny rows (<= 64 slots): one lane per delete record reads the whole row
 //    and claims a matching slot with a CAS (slot := kNone);
-//  - medium rows (<= 4096 slots): one wave per delete record, 1024 slots a
-//    step, same claim;
-//  - long rows: tiles of kDelTile slots over the whole grid against the row's
-//    delete multiset in an open-addressing table of 2^bits >= 2*nd slots.
+//  - medium rows (<= 4096 slots, <= kDelWaveMax deletes): one wave per delete
+//    record, 1024 slots a step, same claim;
+//  - other rows: tiles of kDelTile slots over the whole grid against the
+//    row's delete multiset in an open-addressing table of 2^bits >= 2*nd
+//    slots.
 // Every claimed slot is recorded in the row's hole list (the row's group,
 // whose delete records are not read again). Rows with at most 16 deletes
 // (and no table) are then compacted by one thread, the others by a
@@ -153,9 +154,14 @@ constexpr uint32_t kDelLdsKeys = 2048;
 constexpr uint32_t kDelTinyDeg = 64, kDelMediumDeg = 4096, kDelThreadFix = 16;
 enum : uint8_t { kDelClassNone = 0, kDelClassTiny = 1, kDelClassMedium = 2, kDelClassLong = 3 };
 
+// A medium row with many deletes is cheaper as one tile against its table
+// than as one wave per delete (each of which may read the whole row).
+constexpr uint32_t kDelWaveMax = 8;
+
 __device__ __forceinline__ uint8_t del_class(uint32_t nd, uint32_t deg) {
     if (nd == 0) return kDelClassNone;
-    return deg <= kDelTinyDeg ? kDelClassTiny : deg <= kDelMediumDeg ? kDelClassMedium : kDelClassLong;
+    if (deg <= kDelTinyDeg) return kDelClassTiny;
+    return deg <= kDelMediumDeg && nd <= kDelWaveMax ? kDelClassMedium : kDelClassLong;
 }
 
 __device__ __forceinline__ uint32_t del_table_bits(uint32_t nd) {

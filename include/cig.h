@@ -72,7 +72,8 @@ enum {
     CIG_CFG_COOPERATIVE = 1u << 0, /* run fixed points as one persistent
                                       cooperative launch with grid barriers
                                       instead of chained round kernels */
-    CIG_CFG_NO_HOTNESS = 1u << 1,  /* do not count row activity */
+    CIG_CFG_NO_HOTNESS = 1u << 1,  /* do not count row activity (never counted
+                                      without a cache budget, cache_bytes = 0) */
     CIG_CFG_SCAN_REPAIR = 1u << 2, /* BFS/SSSP/CC: keep no incoming-row index and
                                       repair deletions by scanning every row
                                       (half the edge memory, slower repair) */
