@@ -145,7 +145,8 @@ struct cig_engine {
     // multi-source queries
     DBuf<uint32_t> ms_dist, ms_stamp, ms_src, ms_out;
     DBuf<unsigned long long> ms_act, ms_snap;
-    DBuf<uint2> hcnts, dtiles;
+    DBuf<uint2> hcnts;
+    DBuf<DelTile> dtiles;
     uint64_t rec_cap = 0;
     DBuf<cig_edge> rec_dels, rec_adds;
     DBuf<Ctl> ctl;

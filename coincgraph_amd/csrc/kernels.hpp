@@ -128,7 +128,7 @@ struct MutArgs {
     uint32_t *hcnt;    // per vertex: holes found (zero between calls)
     uint8_t *dclass;   // per row: delete class of this call (kDelClass*)
     const uint8_t *skip;  // grouped batches: 1 = record cancelled (adds [0,na), deletes [na,na+nd)); may be null
-    uint2 *dtiles;     // (row, first slot) of each delete scan tile
+    struct DelTile *dtiles;  // long-row delete scan tiles (DelTile below)
     struct AddPiece *pieces;  // copy pieces of the insert phase (AddPiece below)
     StatShard *stats;
 };
@@ -151,6 +151,14 @@ struct MutArgs {
 // source_local_chunk_store.h:733-780, keeps it).
 constexpr uint32_t kDelTile = 4096;
 constexpr uint32_t kDelLdsKeys = 2048;
+
+// Everything a scan tile needs, written by k_plan so that the tile's row
+// loads depend on one record read only.
+struct DelTile {
+    uint64_t off;   // pool offset of the tile's first slot (tier bit kept)
+    uint32_t row, start, len, to, bits, pad;
+};
+static_assert(sizeof(DelTile) == 32, "DelTile layout");
 constexpr uint32_t kDelTinyDeg = 64, kDelMediumDeg = 4096, kDelThreadFix = 16;
 enum : uint8_t { kDelClassNone = 0, kDelClassTiny = 1, kDelClassMedium = 2, kDelClassLong = 3 };
 
@@ -277,7 +285,21 @@ __global__ void k_plan(MutArgs a) {
         if (t < touched) {
             a.gofs[u] = at[0];
             a.htoff[u] = table ? at[1] : kNone;
-            for (uint32_t j = 0; j < x[2]; ++j) a.dtiles[at[2] + j] = make_uint2(u, j * kDelTile);
+            if (x[2]) {
+                const uint64_t off = a.rows[u].off;
+                const uint32_t bits = del_table_bits(nd);
+                for (uint32_t j = 0; j < x[2]; ++j) {
+                    DelTile dt;
+                    dt.off = off + uint64_t(j) * kDelTile;
+                    dt.row = u;
+                    dt.start = j * kDelTile;
+                    dt.len = min(kDelTile, deg - j * kDelTile);
+                    dt.to = at[1];
+                    dt.bits = bits;
+                    dt.pad = 0;
+                    a.dtiles[at[2] + j] = dt;
+                }
+            }
             if (x[3]) a.small_list[at[3]] = u;
             if (x[4]) a.big_list[at[4]] = u;
             if (x[5]) a.add_list[at[5]] = u;
@@ -345,44 +367,58 @@ __global__ void k_del_lane(MutArgs a) {
     }
 }
 
-// Medium rows: one wave per delete record, 1024 slots a step, stopping at the
-// first slot it claims.
+// Medium rows: one 16-lane group per delete record (four records per wave),
+// 256 slots a step, stopping at the first slot the record claims.
 __global__ __launch_bounds__(kBlock) void k_del_wave(MutArgs a) {
+    constexpr uint32_t G = 16, GPW = kWave / G;
     const uint32_t total = a.nd << a.rev;
-    const uint32_t lane = lane_id();
-    for (uint32_t i = gwave(); i < total; i += gwaves()) {
-        const MutRecord r = del_record(a, i);
-        if (!r.valid || a.dclass[r.row] != kDelClassMedium) continue;  // wave-uniform
-        const Row row = a.rows[r.row];
-        uint32_t *p = a.pools.row(row.off);
-        const uint32_t nq = (row.deg + 3u) >> 2;
-        bool done = false;
-        for (uint32_t q0 = 0; q0 < nq && !done; q0 += kWave * 4) {
-            uint32_t d[4][4], live[4], m = 0;
+    const uint32_t lane = lane_id(), g = lane / G, gl = lane % G;
+    const uint64_t gmask = ((1ull << G) - 1ull) << (g * G);
+    for (uint32_t base = gwave() * GPW; base < total; base += gwaves() * GPW) {
+        const uint32_t i = base + g;
+        bool active = false;
+        MutRecord r{};
+        Row row{};
+        uint32_t *p = nullptr;
+        uint32_t nq = 0;
+        if (i < total) {
+            r = del_record(a, i);
+            active = r.valid && a.dclass[r.row] == kDelClassMedium;
+        }
+        if (active) {
+            row = a.rows[r.row];
+            p = a.pools.row(row.off);
+            nq = (row.deg + 3u) >> 2;
+        }
+        for (uint32_t q0 = 0; __ballot(active && q0 < nq) != 0; q0 += G * 4) {
+            uint32_t m = 0;
+            if (active && q0 < nq) {
+                uint32_t d[4][4], live[4];
 #pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) {
-                const uint32_t q = q0 + k * kWave + lane;
-                live[k] = load_quad(p, q < nq ? q : nq - 1, row.deg, d[k]);
-                if (q >= nq) live[k] = 0;
+                for (uint32_t k = 0; k < 4; ++k) {
+                    const uint32_t q = q0 + k * G + gl;
+                    live[k] = load_quad(p, q < nq ? q : nq - 1, row.deg, d[k]);
+                    if (q >= nq) live[k] = 0;
 #pragma unroll
-                for (uint32_t s = 0; s < 4; ++s)
-                    if (s < live[k] && d[k][s] == r.dst) m |= 1u << (4 * k + s);
+                    for (uint32_t s = 0; s < 4; ++s)
+                        if (s < live[k] && d[k][s] == r.dst) m |= 1u << (4 * k + s);
+                }
             }
             for (;;) {
                 const uint64_t any = __ballot(m != 0u);
                 if (any == 0) break;
-                const uint32_t l = uint32_t(__ffsll((unsigned long long)any) - 1);
+                const uint64_t mine = any & gmask;  // this group's candidate lanes
                 uint32_t ok = 0;
-                if (lane == l) {
+                if (mine && lane == uint32_t(__ffsll((unsigned long long)mine) - 1)) {
                     const uint32_t b = uint32_t(__ffs(m) - 1);
-                    const uint32_t slot = 4u * (q0 + (b >> 2) * kWave + lane) + (b & 3u);
+                    const uint32_t slot = 4u * (q0 + (b >> 2) * G + gl) + (b & 3u);
                     ok = atomicCAS(&p[slot], r.dst, kNone) == r.dst ? 1u : 0u;
                     if (ok) del_hole(a, r.row, slot);
                     m &= m - 1u;
                 }
-                if (__shfl(ok, int(l), kWave)) {
-                    done = true;
-                    break;
+                if (__ballot(ok != 0u) & gmask) {  // the record found its slot
+                    active = false;
+                    m = 0;
                 }
             }
         }
@@ -450,15 +486,10 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
     const uint32_t nt = ld_acquire(&a.ctl->dtile_cnt);
     const uint32_t tid = threadIdx.x;
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        const uint2 dt = a.dtiles[t];
-        const uint32_t u = dt.x, start = dt.y;
-        const Row r = a.rows[u];
-        const uint32_t nd = uint32_t(a.cnt64[u]);
-        const uint32_t bits = del_table_bits(nd), mask = (1u << bits) - 1u;
-        const uint32_t to = a.htoff[u];
-        uint32_t *row = a.pools.row(r.off) + start;  // start is a multiple of 4
-        const uint32_t len = min(kDelTile, r.deg - start);
-        const uint32_t nq = (len + 3u) >> 2;         // <= 4 * blockDim.x
+        const DelTile dt = a.dtiles[t];
+        const uint32_t bits = dt.bits, mask = (1u << bits) - 1u, to = dt.to, len = dt.len;
+        uint32_t *row = a.pools.row(dt.off);     // the tile start is a multiple of 4
+        const uint32_t nq = (len + 3u) >> 2;     // <= 4 * blockDim.x
         uint32_t d[4][4], live[4];
 #pragma unroll
         for (uint32_t k = 0; k < 4; ++k) {
@@ -501,7 +532,7 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
                 if (hit != kNone && atomicAdd(&a.hcnts[to + hit].y, 1u) < a.hcnts[to + hit].x) {
                     const uint32_t slot = 4u * (k * blockDim.x + tid) + s;
                     row[slot] = kNone;
-                    del_hole(a, u, start + slot);
+                    del_hole(a, dt.row, dt.start + slot);
                 }
             }
         }
