@@ -739,8 +739,8 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
     }
 
-    void mutate_finish(const MutArgs &m) {
-        k_clear_touched<<<grid_small, 256, 0, stream>>>(m);
+    void mutate_finish(const MutArgs &m, bool clear_affected = false) {
+        k_clear_touched<<<grid_small, 256, 0, stream>>>(m, clear_affected ? aff.p : nullptr, alist.p);
         HIPCHK(hipGetLastError());
     }
 
@@ -1008,7 +1008,6 @@ struct cig_engine {
             if (!grid) grid = coop_grid_for(kernel);
             launch_coop(kernel, a, grid);
         } else {
-            k_inval_begin<<<1, 64, 0, stream>>>(ctl.p);
             run_chain(
                 [&](uint64_t tag, uint32_t seq) {
                     k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
@@ -1044,14 +1043,17 @@ struct cig_engine {
         HIPCHK(hipEventRecord(ev[5], stream));
         relax();
         HIPCHK(hipEventRecord(ev[4], stream));
-        if (nd) k_clear_affected<<<grid_small, 256, 0, stream>>>(ia);
-        HIPCHK(hipGetLastError());
-        if (na && recover_overflow(m)) {
+        // One host read per batch: end_call's control-block read also tells
+        // whether any insert found no room (then recover, re-seed those
+        // inserts and converge again, and refresh the statistics).
+        end_call(st, 4);
+        if (na && hctl.p->overflow_cnt) {
+            recover_overflow(m);
             insert_seed<W, S>(ia, false);
             relax();
+            end_call(st, 4);  // the control block still holds the whole call's totals
         }
-        mutate_finish(m);
-        end_call(st, 4);
+        mutate_finish(m, nd != 0);
     }
 
     template <int W, bool S>
@@ -1063,9 +1065,7 @@ struct cig_engine {
         invalidate_and_reset(ia);
         repair<W, S>(ia);
         relax();
-        k_clear_affected<<<grid_small, 256, 0, stream>>>(ia);
-        mutate_finish(m);
-        HIPCHK(hipGetLastError());
+        mutate_finish(m, true);
     }
 
     template <int W, bool S>

@@ -38,6 +38,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
     c->acount = 0;
     c->icount[0] = c->icount[1] = c->icount[2] = 0;
     c->ifront[0] = c->ifront[1] = c->ifront[2] = 0;
+    c->iround = 0;
     c->scount = 0;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
@@ -848,12 +849,18 @@ __global__ void k_group_match(GroupArgs g) {
     stat_add(&sh->adds_applied, wave_sum64(cancel_a));
 }
 
-__global__ void k_clear_touched(MutArgs a) {
+// End of a call: clears the per-row counters of the touched rows and, when
+// `aff` is given, the affected bitmap words of the affected list.
+__global__ void k_clear_touched(MutArgs a, uint32_t *aff, const uint32_t *alist) {
     const uint32_t touched = ld_acquire(&a.ctl->touched);
     for (uint32_t t = gthread(); t < touched; t += gthreads()) {
         const uint32_t u = a.touched[t];
         a.cnt64[u] = 0;
         a.fill64[u] = 0;
+    }
+    if (aff) {
+        const uint32_t cnt = ld_acquire(&a.ctl->acount);
+        for (uint32_t i = gthread(); i < cnt; i += gthreads()) aff[alist[i] >> 5] = 0u;
     }
 }
 
@@ -924,19 +931,10 @@ __global__ __launch_bounds__(kBlock) void k_inval_loop(InvalArgs a) {
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
 }
 
-// Opens the invalidation frontier after the seeds: [0, acount).
-__global__ void k_inval_begin(Ctl *c) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    c->iround = 0;
-    c->istart[0] = 0;
-    c->ifront[0] = c->acount;
-    c->ifront[1] = c->ifront[2] = 0;
-    c->icount[0] = c->icount[1] = c->icount[2] = 0;
-}
-
 // One invalidation wave as an ordinary launch, the seq-th of its chain:
-// round k expands its frontier alist[istart, istart+ifront) (and its
-// deferred tiles) and appends the next frontier right behind it. Slots
+// round k expands its frontier alist[istart, istart+ifront) (the first round:
+// the seeds, alist[0, acount)) and its deferred tiles, and appends the next
+// frontier right behind it. Slots
 // rotate mod 3: workgroup 0 opens slot k+1's start, clears slot k+2 and
 // posts the round's progress word at the start, so no workgroup waits for
 // another and the round needs no closing step.
@@ -946,7 +944,10 @@ __global__ __launch_bounds__(kBlock) void k_inval_round(InvalArgs a, uint64_t *p
     __shared__ RoundStage stage;
     Ctl *c = a.ctl;
     const uint32_t k = c->iround + seq - 1;
-    const uint32_t start = c->istart[k % 3], cnt = c->ifront[k % 3], nt = c->icount[k % 3];
+    // the first round's frontier is the seeds: alist[0, acount)
+    const uint32_t start = seq == 1 ? 0u : c->istart[k % 3];
+    const uint32_t cnt = seq == 1 ? c->acount : c->ifront[k % 3];
+    const uint32_t nt = c->icount[k % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->istart[(k + 1) % 3] = start + cnt;
         c->ifront[(k + 2) % 3] = 0;
@@ -984,11 +985,6 @@ __global__ void k_reset_affected(InvalArgs a) {
         }
         block_append(push, v, q, &a.ctl->qcount[r % 3], scratch);
     }
-}
-
-__global__ void k_clear_affected(InvalArgs a) {
-    const uint32_t cnt = ld_acquire(&a.ctl->acount);
-    for (uint32_t i = gthread(); i < cnt; i += gthreads()) a.aff[a.alist[i] >> 5] = 0u;
 }
 
 // SeedAddedEdgeFrontier (framework.cuh:238-294): each inserted (u,v) offers
