@@ -67,9 +67,12 @@ def stage_bytes(st, stage, pr=False):
     """Algorithmic bytes of one expansion pass. Traversal: 8 B per scanned
     edge (4-B destination id + 4-B destination state touch) + 28 B per
     expanded row (16-B descriptor, 8-B state word, 4-B worklist entry).
-    PageRank: 8 B per edge (id + residual atomic) + 40 B per consumed vertex.
-    Weights are not stored."""
-    per_v = 40.0 if pr else 28.0
+    PageRank scatter (stage 2): 8 B per edge (id + residual atomic) + 40 B per
+    consumed vertex. PageRank pull rounds (stage 1): 8 B per incoming edge
+    (4-B source id + 4-B contribution read) + 24 B per gathering vertex
+    (16-B incoming-row descriptor, residual read and write). Weights are not
+    stored."""
+    per_v = (24.0 if stage == 1 else 40.0) if pr else 28.0
     return 8.0 * st["stage_edges"][stage] + per_v * st["stage_vertices"][stage]
 
 
@@ -151,6 +154,8 @@ def main(argv=None):
                     help="batched form: batches per group (cig_apply_next_batches)")
     ap.add_argument("--group-steps", type=int, default=None, help="batched form: timed groups (0 = skip)")
     ap.add_argument("--autotune", action="store_true", help="pick the launch shape first (cig_autotune)")
+    ap.add_argument("--pr-push", action="store_true",
+                    help="PageRank: scatter every round with atomics (keeps no incoming rows)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -183,6 +188,8 @@ def main(argv=None):
     kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=args.weight_seed, flags=cg.CFG_KERNEL_TIMING)
     if pr:
         kw.update(pr_epsilon=1e-7, pr_max_rounds=1000)
+        if args.pr_push:
+            kw["flags"] |= cg.CFG_PR_PUSH_ONLY
     eng = cg.Engine(algo, **kw)
     eng.load_graph(g)
     tune = None
@@ -237,19 +244,21 @@ def main(argv=None):
                 "repair": sum(s["repair_ms"] for s in stats),
                 "mutation": sum(s["mutate_ms"] for s in stats),
                 "invalidation": sum(s["invalidate_ms"] for s in stats)}
-    # Dominant kernel: the propagation round kernel (k_relax_round, or
-    # k_pr_scatter for PageRank), about ten launches per batch. Its duration
-    # comes from the HIP event pair the engine records around every launch
-    # on its own stream; its algorithmic bytes are the propagation pass's
-    # (stage_bytes) spread over all its launches (no-op lookahead rounds
-    # included, as rocprofv3 counts them).
+    # Dominant kernel: the propagation round kernel (k_relax_round; for
+    # PageRank k_pr_pull, the gather form of dense rounds, or k_pr_scatter
+    # when no round gathered). Its duration comes from the HIP event pair the
+    # engine records around every launch on its own stream; its algorithmic
+    # bytes are its pass's (stage_bytes) spread over all its launches (no-op
+    # lookahead rounds, and for k_pr_pull the sparse rounds it leaves to the
+    # scatter, included, as rocprofv3 counts them).
     launches = sum(s["round_launches"] for s in stats)
     kern_ms = sum(s["round_kernel_ms"] for s in stats)
-    prop_bytes = sum(stage_bytes(s, 2, pr) for s in stats)
+    pull = pr and sum(s["pull_rounds"] for s in stats) > 0
+    prop_bytes = sum(stage_bytes(s, 1 if pull else 2, pr) for s in stats)
     bytes_per_launch = prop_bytes / max(launches, 1)
     ms_per_launch = kern_ms / max(launches, 1)
     achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9 if ms_per_launch > 0 else 0.0
-    kernel = "k_pr_scatter" if pr else "k_relax_round"
+    kernel = ("k_pr_pull" if pull else "k_pr_scatter") if pr else "k_relax_round"
     traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{kernel}_config{args.config}.txt"))
 
     if args.check:
@@ -305,6 +314,7 @@ def main(argv=None):
         "repair_kernel_ms_per_batch": round(sum(s["scan_kernel_ms"] for s in stats) / steps, 4),
         "affected_per_batch": round(float(np.mean([s["affected"] for s in stats])), 1),
         "rounds_per_batch": round(float(np.mean([s["rounds"] for s in stats])), 1),
+        **({"pull_rounds_per_batch": round(float(np.mean([s["pull_rounds"] for s in stats])), 1)} if pr else {}),
         "batched_form": grouped,
         "roofline": {
             "kernel": kernel,

@@ -37,6 +37,7 @@ FIELD_VALUE, FIELD_PARENT, FIELD_RESIDUAL, FIELD_DEGREE, FIELD_TIER, FIELD_HOTNE
 CFG_COOPERATIVE, CFG_NO_HOTNESS = 1, 2
 CFG_SCAN_REPAIR = 1 << 2
 CFG_KERNEL_TIMING = 1 << 3
+CFG_PR_PUSH_ONLY = 1 << 4
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
@@ -65,7 +66,7 @@ class Config(C.Structure):
         ("split_degree", C.c_uint32), ("max_rounds", C.c_uint32),
         ("pr_max_rounds", C.c_uint32), ("pr_epsilon", C.c_float),
         ("tile_len", C.c_uint32), ("round_blocks_per_cu", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("pr_pull_divisor", C.c_uint32), ("pull_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -83,7 +84,7 @@ class BatchStats(C.Structure):
         ("pr_residual_l1", C.c_double), ("stage_edges", C.c_uint64 * 4),
         ("stage_vertices", C.c_uint64 * 4), ("stage_host_edges", C.c_uint64 * 4),
         ("scan_kernel_ms", C.c_double), ("round_kernel_ms", C.c_double),
-        ("round_launches", C.c_uint64), ("reserved", C.c_uint64 * 1),
+        ("round_launches", C.c_uint64), ("pull_rounds", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -350,7 +351,7 @@ class Engine:
     def __init__(self, algo="sssp", weight_mode=WEIGHT_REF128, weight_seed=0, cache_bytes=0,
                  device=0, pr_epsilon=1e-6, pr_max_rounds=100, flags=0, edges_per_lane=0,
                  split_degree=0, max_rounds=0, n_stream=3, segments=1, tile_len=0,
-                 round_blocks_per_cu=0):
+                 round_blocks_per_cu=0, pr_pull_divisor=0, pull_blocks_per_cu=0):
         L = lib()
         cfg = Config()
         _check(L.cig_config_default(C.byref(cfg)))
@@ -369,6 +370,8 @@ class Engine:
         cfg.segments = segments
         cfg.tile_len = tile_len
         cfg.round_blocks_per_cu = round_blocks_per_cu
+        cfg.pr_pull_divisor = pr_pull_divisor
+        cfg.pull_blocks_per_cu = pull_blocks_per_cu
         self.config = cfg
         self._e = C.c_void_p()
         st = L.cig_create(C.byref(cfg), C.byref(self._e))

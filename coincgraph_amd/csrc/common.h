@@ -69,7 +69,11 @@ struct Ctl {
     uint32_t dtile_cnt, ht_cursor;  // long-row delete tiles and table slots
     uint32_t pr_invalid;
     uint32_t group_cursor;  // records grouped so far (< 2^32 per call)
-    uint32_t pad_gc;
+    uint32_t pl_tiles;      // PR pull plan: tiles of long incoming rows
+    uint32_t pl_multi;      // PR pull plan: rows of more than one tile
+    uint32_t pull_rounds;   // PR rounds of the current call run in pull form
+    uint32_t amend_pull;    // PR batch amendment gathers (1) or scatters (amend_push = 1)
+    uint32_t amend_push;
     uint64_t pool_tail[2]; // bump tails of the HBM (0) and host (1) pools
     uint64_t pool_cap[2];
     uint64_t garbage[2];   // slots abandoned by relocations

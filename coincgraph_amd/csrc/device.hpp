@@ -230,8 +230,10 @@ struct PrAmendOp {
     const Row *rows;
     const float *rank;
     float sign;
+    uint32_t n;  // touched rows >= n are incoming rows, which carry no rank
     PrScatterOp scatter;
     __device__ bool source(uint32_t v, uint32_t, Row &row, uint32_t &pay) const {
+        if (v >= n) return false;
         row = rows[v];
         const float x = rank[v];
         if (row.deg == 0 || x == 0.0f) return false;

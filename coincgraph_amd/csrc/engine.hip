@@ -103,6 +103,14 @@ struct HBuf {  // pinned host memory (hipHostMalloc; never hipHostRegister)
 constexpr uint32_t kTileLenDefault = 512;
 constexpr uint32_t kSplitDefault = 64;
 constexpr uint32_t kQplDefault = 2;  // quads (4 slots) per lane and step
+// PR rounds whose frontier holds at least n / kPullDivisorDefault vertices
+// run in pull form; below n / kPullMaskDivisor the pull tests the
+// active-source mask before each gather. (R-MAT scale 25: a masked sweep of
+// all incoming rows costs less than scattering with atomics from a frontier
+// of n/4096 hub-heavy vertices.)
+constexpr uint32_t kPullDivisorDefault = 4096;
+constexpr uint32_t kPullMaskDivisor = 4;
+constexpr uint32_t kPullBlocksPerCu = 16;
 
 }  // namespace
 
@@ -131,6 +139,12 @@ struct cig_engine {
     DBuf<uint32_t> q0, q1, stamp, aff, alist, filt, hot, hist, score;
     DBuf<Tile> t0, t1;
     uint64_t tile_cap = 0;
+    // PR pull plan: tiles of long incoming rows, rows of several tiles, partial sums
+    DBuf<Tile> ptiles;
+    DBuf<PullRow> prow;
+    DBuf<float> tpart;
+    DBuf<unsigned long long> amask;  // active-source mask of the pull passes
+    uint64_t ptile_cap = 0;
     DBuf<uint64_t> cnt64, fill64, gofs;
     DBuf<uint32_t> touched, grouped, small_list, big_list, add_list, overflow;
     DBuf<AddPiece> pieces;              // insert-phase copy pieces
@@ -173,6 +187,7 @@ struct cig_engine {
     uint32_t grid_small = 0;   // one-shot kernels
     uint32_t grid_coop = 0;    // persistent cooperative kernels
     uint32_t grid_round = 0;   // chained round kernels
+    uint32_t grid_pull = 0;    // PR pull-form kernels (whole-graph sweeps)
     bool coop = true;
 
     // ------------------------------------------------------------ plumbing
@@ -218,6 +233,7 @@ struct cig_engine {
         coop = (cfg.flags & CIG_CFG_COOPERATIVE) != 0;
         if (coop && !prop.cooperativeLaunch) throw HipError(CIG_EDEVICE, "device lacks cooperative launch");
         grid_small = uint32_t(cus) * 8;
+        grid_pull = uint32_t(cus) * (cfg.pull_blocks_per_cu ? std::min<uint32_t>(cfg.pull_blocks_per_cu, 32) : kPullBlocksPerCu);
         grid_round = uint32_t(cus) * (cfg.round_blocks_per_cu ? std::min<uint32_t>(cfg.round_blocks_per_cu, 16) : 1);
     }
 
@@ -258,7 +274,8 @@ struct cig_engine {
         n = nv;
         budget_slots = cfg.cache_bytes / sizeof(uint32_t);
         // incoming rows (the device reverse index) for deletion repair
-        rev = !is_pr() && !(cfg.flags & CIG_CFG_SCAN_REPAIR);
+        // (PR: the pull rounds' gather rows; the cooperative driver only scatters)
+        rev = !(cfg.flags & CIG_CFG_SCAN_REPAIR) && !(is_pr() && ((cfg.flags & CIG_CFG_PR_PUSH_ONLY) || coop));
         if (rev && n > 0x7FFFFFFFu) throw HipError(CIG_ELIMIT, "incoming index needs n < 2^31 (use CIG_CFG_SCAN_REPAIR)");
         nrows = rev ? 2 * n : n;
         std::vector<uint64_t> ioff;
@@ -284,7 +301,8 @@ struct cig_engine {
             total_cap += hrows[v].cap;
         }
         // tier placement: longest outgoing rows first while the budget lasts
-        // (90% fill), then incoming rows (read only by the deletion repair)
+        // (90% fill), then incoming rows (read by the deletion repair and by
+        // PageRank's pull rounds)
         std::vector<uint8_t> in_hbm(nrows, 1);
         uint64_t hbm_need = total_cap, host_need = 0;
         if (budget_slots && total_cap > budget_slots * 9 / 10) {
@@ -569,12 +587,41 @@ struct cig_engine {
         a.cfg = ecfg(kStagePropagate);
         a.eps = cfg.pr_epsilon;
         a.max_rounds = max_rounds;
+        a.n = n;
+        a.pull_min = kNone;
+        if (rev && !coop) {
+            const uint32_t div = cfg.pr_pull_divisor ? cfg.pr_pull_divisor : kPullDivisorDefault;
+            a.pull_min = std::max<uint32_t>(1u, n / div);
+            a.in_rows = rows.p + n;
+            a.ptiles = ptiles.p;
+            a.prow = prow.p;
+            a.tpart = tpart.p;
+            a.amask = amask.p;
+            a.mask_max = std::max<uint32_t>(1u, n / kPullMaskDivisor);
+        }
         return a;
+    }
+
+    // Pull plan capacity: every incoming row of kPullSplit slots or more is
+    // cut into tiles of kPullTile slots, and rows hold at most the pools' slots.
+    void ensure_pull() {
+        const uint64_t slots = hbm_slots + host_slots;
+        const uint64_t rows_long = std::min<uint64_t>(n, slots / kPullSplit);
+        const uint64_t need = slots / kPullTile + rows_long + 1;
+        if (!amask.p) amask.alloc((uint64_t(n) + 63) / 64);
+        if (need <= ptile_cap) return;
+        ptiles.alloc(need);
+        tpart.alloc(need);
+        prow.alloc(std::min<uint64_t>(n, slots / kPullTile) + 1);
+        ptile_cap = need;
     }
 
     // Runtime::Converge (pr_residual.cuh:171-187): at most pr_max_rounds
     // consume/scatter rounds; residual left over stays pending.
-    void pr_converge() {
+    // plan_ready: the batch amendment already built the pull plan of the
+    // current incoming rows.
+    void pr_converge(bool plan_ready = false) {
+        if (rev && !coop) ensure_pull();
         PrArgs a = pr_args(cfg.pr_max_rounds);
         if (coop) {
             auto kernel = k_pr_loop<1>;
@@ -582,11 +629,30 @@ struct cig_engine {
             if (!grid) grid = coop_grid_for(kernel);
             grid_coop = grid;
             launch_coop(kernel, a, grid);
-        } else {
+        } else if (a.pull_min == kNone) {
             advance_round(run_chain(
                 [&](uint64_t tag, uint32_t seq) {
                     k_pr_consume<<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
                     timed_round([&] { k_pr_scatter<1><<<grid_round, kBlock, 0, stream>>>(a, seq); });
+                },
+                cfg.pr_max_rounds, "PageRank"));
+        } else {
+            // a dense round gathers along the incoming rows (k_pr_pull +
+            // k_pr_pull_long), a sparse one scatters; each kernel checks
+            // the round's frontier size and returns when it is not its form
+            if (!plan_ready) pull_plan(a, nullptr);
+            advance_round(run_chain(
+                [&](uint64_t tag, uint32_t seq) {
+                    k_pr_consume<<<grid_pull, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                    k_pr_scatter<1><<<grid_round, kBlock, 0, stream>>>(a, seq);
+                    timed_round([&] {
+                        switch (qpl) {
+                        case 1: k_pr_pull<1><<<grid_pull, kBlock, 0, stream>>>(a, seq); break;
+                        case 4: k_pr_pull<4><<<grid_pull, kBlock, 0, stream>>>(a, seq); break;
+                        default: k_pr_pull<2><<<grid_pull, kBlock, 0, stream>>>(a, seq); break;
+                        }
+                    });
+                    k_pr_pull_long<<<grid_small, kBlock, 0, stream>>>(a, seq);
                 },
                 cfg.pr_max_rounds, "PageRank"));
         }
@@ -684,6 +750,7 @@ struct cig_engine {
         st->touched_sources = c.touched_sources;
         st->converged = is_pr() ? (c.qcount[c.round % 3] == 0 ? 1u : 0u) : 1u;
         st->pr_residual_l1 = c.residual_l1;
+        st->pull_rounds = c.pull_rounds;
     }
 
     // ------------------------------------------------------------ mutation
@@ -1097,9 +1164,37 @@ struct cig_engine {
     }
 
     // ------------------------------------------------------------ PR batch
-    void pr_amend(float sign) {
+    void pull_plan(const PrArgs &a, const uint32_t *gate) {
+        k_pr_pull_plan_reset<<<1, 64, 0, stream>>>(ctl.p);
+        k_pr_pull_plan<<<grid_small, kBlock, 0, stream>>>(a, gate);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int QPL>
+    void pull_pass(const PrArgs &a, uint32_t seq) {
+        k_pr_pull<QPL><<<grid_pull, kBlock, 0, stream>>>(a, seq);
+        k_pr_pull_long<<<grid_small, kBlock, 0, stream>>>(a, seq);
+    }
+
+    // The touched sources' contribution sign*alpha*x/deg along their current
+    // rows, added to the residuals of round `round`'s pending frontier: by
+    // scatter (gated by amend_push) or, with pull = true, by a sweep of the
+    // incoming rows (gated by amend_pull; builds the pull plan first).
+    void pr_amend(float sign, bool pull) {
+        if (pull) {
+            PrArgs a = pr_args(cfg.pr_max_rounds);
+            fill32(reinterpret_cast<uint32_t *>(delta.p), n, 0u);
+            fill32(reinterpret_cast<uint32_t *>(amask.p), 2 * ((uint64_t(n) + 63) / 64), 0u);
+            k_pr_amend_delta<<<grid_small, 256, 0, stream>>>(a, touched.p, sign);
+            pull_plan(a, sign < 0 ? &ctl.p->amend_pull : nullptr);
+            switch (qpl) {
+            case 1: pull_pass<1>(a, 0); break;
+            case 4: pull_pass<4>(a, 0); break;
+            default: pull_pass<2>(a, 0); break;
+            }
+        }
         PrScatterOp sc{rows.p, res.p, stamp.p, delta.p, &ctl.p->pr_invalid, cfg.pr_epsilon};
-        PrAmendOp op{rows.p, rank.p, sign, sc};
+        PrAmendOp op{rows.p, rank.p, sign, n, sc};
         // appends to the pending frontier (queue of the next round)
         read_ctl();
         const uint32_t r = hctl.p->round;
@@ -1108,7 +1203,7 @@ struct cig_engine {
         uint32_t zero = 0;
         HIPCHK(hipMemcpyAsync(&ctl.p->scount, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         k_expand_once<1><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), touched.p, &ctl.p->touched,
-                                                            0, sink, ts, nullptr);
+                                                            0, sink, ts, pull ? &ctl.p->amend_push : nullptr);
         k_expand_tiles<1><<<grid_small, kBlock, 0, stream>>>(op, ecfg(kStageAmend), t0.p, &ctl.p->scount, sink);
         HIPCHK(hipGetLastError());
     }
@@ -1120,15 +1215,23 @@ struct cig_engine {
         begin_call(st);
         MutArgs m = mut_args(d_dels, nd, d_adds, na);
         mutate_group(m);
-        pr_amend(-1.0f);  // old rows, old degrees
+        const bool pull = rev && !coop;
+        if (pull) {
+            ensure_pull();
+            const PrArgs a = pr_args(cfg.pr_max_rounds);
+            // touched counts outgoing and incoming rows: twice the sources
+            const uint32_t min_touched = a.pull_min > 0x7FFFFFFFu ? kNone : 2u * a.pull_min;
+            k_pr_amend_form<<<1, 64, 0, stream>>>(ctl.p, min_touched);
+        }
+        pr_amend(-1.0f, pull);  // old rows, old degrees
         mutate_apply(m);
         recover_overflow(m);
-        pr_amend(+1.0f);  // new rows, new degrees
+        pr_amend(+1.0f, pull);  // new rows, new degrees (the plan of the new rows stays for the solve)
         HIPCHK(hipEventRecord(ev[1], stream));
         HIPCHK(hipEventRecord(ev[2], stream));
         HIPCHK(hipEventRecord(ev[3], stream));
         HIPCHK(hipEventRecord(ev[5], stream));
-        pr_converge();
+        pr_converge(pull);
         HIPCHK(hipEventRecord(ev[4], stream));
         pr_residual_norm();
         mutate_finish(m);
@@ -1353,6 +1456,7 @@ struct cig_engine {
         sum.scan_kernel_ms += x.scan_kernel_ms;
         sum.round_kernel_ms += x.round_kernel_ms;
         sum.round_launches += x.round_launches;
+        sum.pull_rounds += x.pull_rounds;
     }
 
     // The next `k` resident batches as one group.

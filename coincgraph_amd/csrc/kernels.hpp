@@ -45,6 +45,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
     c->dtile_cnt = c->ht_cursor = 0;
     c->pr_invalid = 0;
     c->group_cursor = 0;
+    c->pull_rounds = 0;
     c->edges_scanned = c->host_edges = c->vexp = 0;
     c->deletes_applied = c->missing = c->invalid = c->adds_applied = c->relocated = 0;
     c->touched_sources = 0;
@@ -1298,6 +1299,16 @@ __global__ void k_advance_round(Ctl *c, uint32_t by) {
 }
 
 // ---------------------------------------------------------- PageRank rounds
+// Pull plan of one solve: incoming rows at least kPullSplit slots long are cut
+// into tiles of kPullTile slots; a row of several tiles also gets a PullRow
+// whose tiles' partial sums k_pr_pull_long adds up.
+constexpr uint32_t kPullSplit = 256;
+constexpr uint32_t kPullTile = 2048;
+
+struct PullRow {
+    uint32_t v, first, nt, pad;
+};
+
 struct PrArgs {
     Ctl *ctl;
     const Row *rows;
@@ -1308,7 +1319,20 @@ struct PrArgs {
     ExpandCfg cfg;
     float eps;
     uint32_t max_rounds;
+    // pull form (incoming rows present): rounds whose frontier holds at
+    // least pull_min vertices gather; pull_min = kNone disables it
+    const Row *in_rows;
+    uint32_t n, pull_min;
+    Tile *ptiles;
+    PullRow *prow;
+    float *tpart;
+    // bit v set iff delta[v] != 0; pull passes of rounds with fewer than
+    // mask_max frontier vertices (and amendments) test it before gathering
+    unsigned long long *amask;
+    uint32_t mask_max;
 };
+
+__device__ __forceinline__ bool pr_dense(const PrArgs &a, uint32_t nq) { return nq != 0 && nq >= a.pull_min; }
 
 // Runtime::Converge (pr_residual.cuh:171-187) as one persistent launch:
 // consume (rank += r with Kahan compensation, r = 0, contribution
@@ -1401,9 +1425,40 @@ __global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a, uint64_t *prog,
         post_progress(prog, tag, nq != 0);
     }
     if (nq == 0) return;
+    Counters ctr;
+    if (pr_dense(a, nq)) {
+        // pull form: the queued flags are the frontier; every vertex gets its
+        // contribution (0 when not consumed) in delta[v]
+        if (blockIdx.x == 0 && threadIdx.x == 0) c->pull_rounds += 1;
+        // a wave takes 64 consecutive vertices (one mask word) per step
+        for (uint32_t base = gthread() - lane_id(); base < a.n; base += gthreads()) {
+            const uint32_t v = base + lane_id();
+            float d = 0.0f;
+            if (v < a.n && a.queued[v]) {
+                a.queued[v] = 0u;
+                const float rv = a.res[v];
+                if (!isfinite(rv) || !isfinite(a.rank[v])) atomicExch(&c->pr_invalid, 1u);
+                if (fabsf(rv) > a.eps) {
+                    a.res[v] = 0.0f;
+                    const float x = a.rank[v];
+                    const float y = rv - a.comp[v];
+                    const float t = x + y;
+                    a.comp[v] = (t - x) - y;
+                    a.rank[v] = t;
+                    const uint32_t deg = a.rows[v].deg;
+                    ctr.vexp += 1;
+                    if (deg) d = 0.85f * rv / float(deg);
+                }
+            }
+            const unsigned long long bits = __ballot(d != 0.0f);
+            if (v < a.n) a.delta[v] = d;
+            if (lane_id() == 0) a.amask[base >> 6] = bits;
+        }
+        flush_counters(ctr, a.cfg.stats, a.cfg.stage);
+        return;
+    }
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
     Tile *tiles = (r & 1) ? a.t1 : a.t0;
-    Counters ctr;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nq; base += gthreads()) {
         const uint32_t i = base + threadIdx.x;
         uint32_t v = 0, nt = 0;
@@ -1458,7 +1513,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a, uint32_t seq) {
     Ctl *c = a.ctl;
     const uint32_t r = c->round + seq - 1;
     const uint32_t nq = c->qcount[r % 3];
-    if (nq == 0) return;
+    if (nq == 0 || pr_dense(a, nq)) return;
     const uint32_t nt = c->tcount[r % 3];
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
     Tile *tiles = (r & 1) ? a.t1 : a.t0;
@@ -1472,6 +1527,323 @@ __global__ __launch_bounds__(kBlock) void k_pr_scatter(PrArgs a, uint32_t seq) {
     expand_tiles<QPL>(op, a.cfg, tiles, nt, out);
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     out_flush(out);
+}
+
+// ------------------------------------------------ PageRank rounds, pull form
+// The scatter of a round adds alpha*r(u)/deg(u) to r(w) for every edge (u,w)
+// of a consumed u. When the frontier is dense the same sums are gathered
+// along the incoming rows (rows n..2n-1, sources of each vertex's in-edges):
+// w adds sum over its in-neighbours u of delta[u], which the consume phase
+// wrote for every vertex (0 where nothing was consumed). Each r(w) then has
+// a single writer, so the residual updates are plain loads and stores, not
+// the scatter's per-edge atomics, and a vertex joins the next frontier when
+// its residual crosses epsilon, as in the scatter. Only the summation order
+// differs from the scatter's (whose atomic order is itself unspecified).
+
+// Builds the pull plan of the current incoming rows (they do not change
+// during a solve); `gate` (when given) must be non-zero for it to run.
+__global__ void k_pr_pull_plan(PrArgs a, const uint32_t *gate) {
+    __shared__ uint32_t scratch[2 * (kMaxWavesPerBlock + 1)];
+    Ctl *c = a.ctl;
+    if (gate && *gate == 0) return;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < a.n; base += gthreads()) {
+        const uint32_t v = base + threadIdx.x;
+        Row row{};
+        if (v < a.n) row = a.in_rows[v];
+        const uint32_t nt = row.deg >= kPullSplit ? (row.deg + kPullTile - 1) / kPullTile : 0u;
+        const uint32_t x[2] = {nt, nt > 1 ? 1u : 0u};
+        uint32_t *const cnt[2] = {&c->pl_tiles, &c->pl_multi};
+        uint32_t at[2];
+        block_reserve_n<2>(x, cnt, at, scratch);
+        const uint32_t mi = nt > 1 ? at[1] : kNone;
+        if (nt > 1) a.prow[mi] = PullRow{v, at[0], nt, 0u};
+        for (uint32_t j = 0; j < nt; ++j) {
+            Tile t;
+            t.start = row.off + uint64_t(j) * kPullTile;
+            t.v = v;
+            t.len = min(kPullTile, row.deg - j * kPullTile);
+            t.payload = mi;
+            t.pad = 0;
+            a.ptiles[at[0] + j] = t;
+        }
+    }
+}
+
+__global__ void k_pr_pull_plan_reset(Ctl *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->pl_tiles = c->pl_multi = 0;
+}
+
+__device__ __forceinline__ float wave_sum_f(float x) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+// Adds `sum` to r(w) (w's only writer this round) and reports whether w
+// joins the next frontier.
+__device__ __forceinline__ bool pull_apply(const PrArgs &a, uint32_t w, float sum) {
+    if (sum == 0.0f) return false;
+    const float old = a.res[w];
+    const float now = old + sum;
+    a.res[w] = now;
+    if (!isfinite(now)) atomicExch(&a.ctl->pr_invalid, 1u);
+    if (fabsf(old) <= a.eps && fabsf(now) > a.eps && a.queued[w] == 0u) {
+        a.queued[w] = 1u;
+        return true;
+    }
+    return false;
+}
+
+struct PullLds {
+    uint32_t pref[kWave];
+    uint32_t deg[kWave];
+    uint32_t offlo[kWave];
+    uint32_t offhi[kWave];
+    float acc[kWave];
+};
+
+// Sum of delta[] over the live slots of QPL quads whose loads are in flight
+// together; dead slots (past the row's end or past the last quad) add 0.
+// With a mask, only sources whose mask bit is set are gathered (the mask is
+// 1/32 of delta's size, so a sparse pass touches far fewer cache lines).
+template <int QPL>
+__device__ __forceinline__ void pull_gather(const float *delta, const unsigned long long *mask,
+                                            const uint32_t (&d)[QPL][4], const uint32_t (&live)[QPL],
+                                            float (&s)[QPL]) {
+    float x[QPL][4];
+    if (mask) {
+        unsigned long long w[QPL][4];
+#pragma unroll
+        for (int q = 0; q < QPL; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[q][k] = mask[d[q][k] >> 6];
+#pragma unroll
+        for (int q = 0; q < QPL; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x[q][k] = (w[q][k] >> (d[q][k] & 63u)) & 1ull ? delta[d[q][k]] : 0.0f;
+    } else {
+#pragma unroll
+        for (int q = 0; q < QPL; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x[q][k] = delta[d[q][k]];
+    }
+#pragma unroll
+    for (int q = 0; q < QPL; ++q) {
+        s[q] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s[q] += uint32_t(k) < live[q] ? x[q][k] : 0.0f;
+    }
+}
+
+// The 64 vertices [v0, v0+64): rows shorter than kPullSplit are summed here,
+// lanes walking the wave's concatenated rows quad by quad and adding each
+// quad's sum to its row's LDS accumulator.
+template <int QPL, class Out>
+__device__ void pull_chunk(const PrArgs &a, const unsigned long long *mask, uint32_t v0, PullLds &L,
+                           const Out &out, Counters &ctr) {
+    const uint32_t lane = lane_id();
+    const uint32_t v = v0 + lane;
+    uint32_t deg = 0;
+    uint64_t off = 0;
+    if (v < a.n) {
+        const Row row = a.in_rows[v];
+        if (row.deg < kPullSplit) {
+            deg = row.deg;
+            off = row.off;
+            ctr.vexp += 1;
+        }
+    }
+    ctr.scanned += deg;
+    if (off & kHostTier) ctr.host += deg;
+    const uint32_t nq = (deg + 3u) >> 2;
+    const uint32_t incl = wave_incl_scan(nq);
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    float sum = 0.0f;
+    if (total) {
+        L.pref[lane] = incl - nq;
+        L.deg[lane] = deg;
+        L.offlo[lane] = uint32_t(off);
+        L.offhi[lane] = uint32_t(off >> 32);
+        L.acc[lane] = 0.0f;
+        wave_lds_sync();
+        for (uint32_t k0 = 0; k0 < total; k0 += kWave * QPL) {
+            uint32_t d[QPL][4], own[QPL], live[QPL];
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) {
+                const uint32_t k = k0 + uint32_t(q) * kWave + lane;
+                const uint32_t kk = k < total ? k : total - 1;
+                own[q] = owner_of(L.pref, kk);
+                const uint64_t o = (uint64_t(L.offhi[own[q]]) << 32) | L.offlo[own[q]];
+                live[q] = load_quad(a.cfg.pools.row(o), kk - L.pref[own[q]], L.deg[own[q]], d[q]);
+                if (k >= total) live[q] = 0;
+            }
+            float s[QPL];
+            pull_gather<QPL>(a.delta, mask, d, live, s);
+#pragma unroll
+            for (int q = 0; q < QPL; ++q)
+                if (live[q]) atomicAdd(&L.acc[own[q]], s[q]);
+        }
+        wave_lds_sync();
+        sum = L.acc[lane];
+        wave_lds_sync();  // LDS is reused by the next chunk of this wave
+    }
+    const bool f[1] = {deg != 0 && pull_apply(a, v, sum)};
+    const uint32_t val[1] = {v};
+    out_push<1>(out, f, val);
+}
+
+// One tile of a long incoming row: a wave sum; a one-tile row is finished
+// here, the tiles of a longer row leave partial sums for k_pr_pull_long.
+template <int QPL, class Out>
+__device__ void pull_tile(const PrArgs &a, const unsigned long long *mask, uint32_t ti, const Out &out,
+                          Counters &ctr) {
+    const Tile t = a.ptiles[ti];
+    const uint32_t *p = a.cfg.pools.row(t.start);
+    const uint32_t lane = lane_id();
+    const uint32_t nq = (t.len + 3u) >> 2;
+    float acc = 0.0f;
+    for (uint32_t k0 = 0; k0 < nq; k0 += kWave * QPL) {
+        uint32_t d[QPL][4], live[QPL];
+#pragma unroll
+        for (int q = 0; q < QPL; ++q) {
+            const uint32_t k = k0 + uint32_t(q) * kWave + lane;
+            live[q] = load_quad(p, k < nq ? k : nq - 1, t.len, d[q]);
+            if (k >= nq) live[q] = 0;
+        }
+        float s[QPL];
+        pull_gather<QPL>(a.delta, mask, d, live, s);
+#pragma unroll
+        for (int q = 0; q < QPL; ++q) acc += s[q];
+    }
+    acc = wave_sum_f(acc);
+    if (lane == 0) {
+        ctr.scanned += t.len;
+        if (t.start & kHostTier) ctr.host += t.len;
+    }
+    bool f[1] = {false};
+    if (t.payload == kNone) {
+        if (lane == 0) {
+            ctr.vexp += 1;
+            f[0] = pull_apply(a, t.v, acc);
+        }
+    } else if (lane == 0) {
+        a.tpart[ti] = acc;
+    }
+    const uint32_t val[1] = {t.v};
+    out_push<1>(out, f, val);
+}
+
+// Frontier staging of the pull kernels (they make no tiles): small, so that
+// many workgroups fit on a CU and keep many gathers in flight.
+using PullStage = StageLds<1024, 2>;
+
+// Where a pull pass appends and whether it runs. seq >= 1: the scatter phase
+// of round round + seq - 1 when that round is dense, appending to the next
+// round's frontier. seq = 0: a batch amendment (gated by amend_pull), whose
+// contributions are the touched sources' +-alpha*x/deg, appending to the
+// pending frontier of round `round`.
+struct PullPass {
+    bool run;
+    uint32_t stage;
+    Sink sink;
+    const unsigned long long *mask;
+};
+
+__device__ __forceinline__ PullPass pull_pass(const PrArgs &a, uint32_t seq) {
+    Ctl *c = a.ctl;
+    if (seq == 0) {
+        const uint32_t r = c->round;
+        return PullPass{c->amend_pull != 0, kStageAmend, Sink{(r & 1) ? a.q1 : a.q0, &c->qcount[r % 3]}, a.amask};
+    }
+    const uint32_t r = c->round + seq - 1;
+    const uint32_t nq = c->qcount[r % 3];
+    return PullPass{pr_dense(a, nq), kStageRepair, Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
+                    nq < a.mask_max ? a.amask : nullptr};
+}
+
+// Pull pass `seq` (see pull_pass): short rows by wave chunks, then the plan's tiles.
+template <int QPL>
+__global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a, uint32_t seq) {
+    __shared__ PullLds lds[kWavesPerBlock];
+    __shared__ PullStage stage;
+    Ctl *c = a.ctl;
+    const PullPass pass = pull_pass(a, seq);
+    if (!pass.run) return;
+    const StagedOut<1024, 2> out{pass.sink, TileSink{nullptr, &c->scount}, &stage};
+    out_init(out);
+    Counters ctr;
+    const uint32_t wpb = blockDim.x / kWave;
+    const uint32_t gwave = blockIdx.x * wpb + wave_in_block(), nwaves = gridDim.x * wpb;
+    const uint32_t nchunks = (a.n + kWave - 1) / kWave;
+    for (uint32_t ch = gwave; ch < nchunks; ch += nwaves)
+        pull_chunk<QPL>(a, pass.mask, ch * kWave, lds[wave_in_block()], out, ctr);
+    const uint32_t nt = c->pl_tiles;
+    for (uint32_t t = gwave; t < nt; t += nwaves) pull_tile<QPL>(a, pass.mask, t, out, ctr);
+    flush_counters(ctr, a.cfg.stats, pass.stage);
+    out_flush(out);
+}
+
+// Rows of several tiles: a wave adds up the tiles' partial sums.
+__global__ __launch_bounds__(kBlock) void k_pr_pull_long(PrArgs a, uint32_t seq) {
+    __shared__ PullStage stage;
+    Ctl *c = a.ctl;
+    const uint32_t nm = c->pl_multi;
+    const PullPass pass = pull_pass(a, seq);
+    if (nm == 0 || !pass.run) return;
+    const StagedOut<1024, 2> out{pass.sink, TileSink{nullptr, &c->scount}, &stage};
+    out_init(out);
+    const uint32_t wpb = blockDim.x / kWave;
+    const uint32_t gwave = blockIdx.x * wpb + wave_in_block(), nwaves = gridDim.x * wpb;
+    const uint32_t lane = lane_id();
+    uint64_t rows_done = 0;
+    for (uint32_t i = gwave; i < nm; i += nwaves) {
+        const PullRow pr = a.prow[i];
+        float s = 0.0f;
+        for (uint32_t j = lane; j < pr.nt; j += kWave) s += a.tpart[pr.first + j];
+        s = wave_sum_f(s);
+        bool f[1] = {false};
+        if (lane == 0) {
+            rows_done += 1;
+            f[0] = pull_apply(a, pr.v, s);
+        }
+        const uint32_t val[1] = {pr.v};
+        out_push<1>(out, f, val);
+    }
+    Counters ctr;
+    ctr.vexp = rows_done;
+    flush_counters(ctr, a.cfg.stats, pass.stage);
+    out_flush(out);
+}
+
+// Batch amendment form (Engine::UpdatePageRank, framework.cuh:1660-1704):
+// r += alpha (P_new^T - P_old^T) x over the touched sources, either scattered
+// along their old and new rows (few touched rows) or gathered by two sweeps
+// of the incoming rows, before and after the mutation (many touched rows: a
+// hub-heavy batch touches most edges, and the scatter's atomics would cost
+// several sweeps). `touched` counts outgoing and incoming rows.
+__global__ void k_pr_amend_form(Ctl *c, uint32_t min_touched) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t pull = c->touched >= min_touched ? 1u : 0u;
+    c->amend_pull = pull;
+    c->amend_push = 1u - pull;
+}
+
+// Pull amendment contributions: delta[u] = sign*alpha*x[u]/deg(u) along u's
+// current row for every touched source u, with its mask bit (delta and the
+// mask are zero elsewhere).
+__global__ void k_pr_amend_delta(PrArgs a, const uint32_t *touched, float sign) {
+    Ctl *c = a.ctl;
+    if (!c->amend_pull) return;
+    const uint32_t cnt = c->touched;
+    for (uint32_t i = gthread(); i < cnt; i += gthreads()) {
+        const uint32_t u = touched[i];
+        if (u >= a.n) continue;  // an incoming row
+        const uint32_t deg = a.rows[u].deg;
+        const float x = a.rank[u];
+        const float d = deg && x != 0.0f ? sign * 0.85f * x / float(deg) : 0.0f;
+        a.delta[u] = d;
+        if (d != 0.0f) atomicOr(&a.amask[u >> 6], 1ull << (u & 63u));
+    }
 }
 
 // --------------------------------------------------- multi-source queries

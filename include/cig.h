@@ -74,11 +74,13 @@ enum {
                                       instead of chained round kernels */
     CIG_CFG_NO_HOTNESS = 1u << 1,  /* do not count row activity (never counted
                                       without a cache budget, cache_bytes = 0) */
-    CIG_CFG_SCAN_REPAIR = 1u << 2, /* BFS/SSSP/CC: keep no incoming-row index and
-                                      repair deletions by scanning every row
-                                      (half the edge memory, slower repair) */
-    CIG_CFG_KERNEL_TIMING = 1u << 3 /* bracket every propagation round kernel with
-                                       HIP events (cig_batch_stats.round_kernel_ms) */
+    CIG_CFG_SCAN_REPAIR = 1u << 2, /* keep no incoming-row index: BFS/SSSP/CC repair
+                                      deletions by scanning every row, PR scatters
+                                      every round (half the edge memory) */
+    CIG_CFG_KERNEL_TIMING = 1u << 3, /* bracket every propagation round kernel with
+                                        HIP events (cig_batch_stats.round_kernel_ms) */
+    CIG_CFG_PR_PUSH_ONLY = 1u << 4 /* PR: keep no incoming-row index; every round
+                                      scatters with atomics (the reference's form) */
 };
 
 /* Engine configuration. Flags of the reference samples map as:
@@ -104,7 +106,11 @@ typedef struct cig_config {
     float pr_epsilon;         /* absolute residual threshold (hybrid_pr.cu:23) */
     uint32_t tile_len;        /* slots per tile of a split row; 0 = default */
     uint32_t round_blocks_per_cu; /* workgroups per CU of round kernels; 0 = default */
-    uint32_t reserved[5];
+    uint32_t pr_pull_divisor; /* PR rounds whose frontier holds at least n / pr_pull_divisor
+                                 vertices gather along the incoming rows instead of scattering
+                                 (no atomics); 0 = default (4096); >= n: every round */
+    uint32_t pull_blocks_per_cu; /* workgroups per CU of the PR pull kernels; 0 = default (16) */
+    uint32_t reserved[3];
 } cig_config;
 
 /* Per-call statistics. Times are device (HIP event) milliseconds of the
@@ -143,7 +149,7 @@ typedef struct cig_batch_stats {
     double round_kernel_ms;   /* CIG_CFG_KERNEL_TIMING: summed durations of the propagation
                                  round kernels (k_relax_round / k_pr_scatter) */
     uint64_t round_launches;  /* propagation round kernels launched (incl. no-op lookahead) */
-    uint64_t reserved[1];
+    uint64_t pull_rounds;     /* PR rounds run in pull form */
 } cig_batch_stats;
 
 typedef struct cig_graph_info {

@@ -211,22 +211,45 @@ def test_pool_overflow_recovery():
         check_topology(e, og, "overflow final")
 
 
+# PageRank round forms: "adaptive" (default: dense rounds gather along the
+# incoming rows, sparse ones scatter), "push" (every round scatters),
+# "pull" (every non-empty round gathers), "pull-host" (gathers with most
+# incoming rows in the pinned host tier). Scale 14's vertex 0 has ~5.6k
+# in-edges (several pull tiles); scale 10 has one-tile and short rows.
+PR_FORMS = {
+    "adaptive": dict(),
+    "push": dict(flags=cg.CFG_PR_PUSH_ONLY),
+    "pull": dict(pr_pull_divisor=0xFFFFFFFF),
+    "pull-host": dict(pr_pull_divisor=0xFFFFFFFF, cache_bytes=None),
+}
+
+
+@pytest.mark.parametrize("form", list(PR_FORMS))
 @pytest.mark.parametrize("scale", [10, 14])
-def test_pagerank_static_and_batches(scale):
-    _announce(f"pagerank s{scale}")
+def test_pagerank_static_and_batches(scale, form):
+    _announce(f"pagerank s{scale} {form}")
     eps = 1e-7
     g = cg.rmat_graph(scale, 16, seed=12)
     s = cg.rmat_stream(g, scale, 4, 300, 300, seed=13)
     og = oracle_graph(g)
-    with cg.Engine("pr", pr_epsilon=eps, pr_max_rounds=2000) as e:
+    kw = dict(PR_FORMS[form])
+    if "cache_bytes" in kw:
+        kw["cache_bytes"] = g.m * 4 * 2 // 3  # under a third of both row sets
+    with cg.Engine("pr", pr_epsilon=eps, pr_max_rounds=2000, **kw) as e:
         e.load_graph(g)
         e.init_graph()
         st = e.start()
+        if form.startswith("pull"):
+            assert st["pull_rounds"] == st["rounds"] > 0
+        if form == "pull-host":
+            assert e.info()["host_pool_used"] > 0
         for b in range(-1, s.nbatches):
             if b >= 0:
                 dels, adds = s.batch(b)
                 st = e.apply_batch(dels, adds)
                 oracle_apply(og, dels, adds)
+            if form == "push":
+                assert st["pull_rounds"] == 0
             assert st["converged"] == 1, f"batch {b}: not converged in 2000 rounds"
             ranks, resid = e.values(), e.residuals()
             ref, rounds = og.pagerank()
