@@ -19,11 +19,16 @@ recompute, single thread, a bounded sample of the same batches).
 
 --config 1 / 3 / 4 run the other incremental workloads at their own sizes
 (BFS S16, CC on a symmetrised S24, PageRank S25) with the same report.
+--config 5 is the serving workload: R-MAT scale 27 edge factor 32 under a
+16 GiB HBM edge budget (the rest of the rows zero-copy in pinned host
+memory), Poisson request arrivals, update requests of log-uniform size,
+interleaved 64-source BFS/SSSP queries, all through the serving loop.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -52,6 +57,11 @@ CONFIGS = {
     4: dict(algo="pr", scale=25, ef=16, sym=False, batch=500_000, steps=10, group=10,
             what="incremental PageRank (alpha 0.85, epsilon 1e-7), R-MAT scale 25 edge factor 16, "
                  "batches of 500000 deletes + 500000 inserts"),
+    5: dict(algo="sssp", scale=27, ef=32, sym=False, batch=0, steps=0, group=32, serve=True,
+            what="serving: incremental SSSP on R-MAT scale 27 edge factor 32 with a 16 GiB HBM edge budget "
+                 "(other rows zero-copy in pinned host memory), Poisson arrivals, update requests of "
+                 "log-uniform size in [1k, 1M] (half deletes, half inserts), interleaved 64-source BFS/SSSP "
+                 "queries, continuous batching, admission by free HBM"),
 }
 
 
@@ -156,6 +166,11 @@ def main(argv=None):
     ap.add_argument("--autotune", action="store_true", help="pick the launch shape first (cig_autotune)")
     ap.add_argument("--pr-push", action="store_true",
                     help="PageRank: scatter every round with atomics (keeps no incoming rows)")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving: arrival window (s)")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--query-every", type=int, default=100,
+                    help="serving: every k-th request is a 64-source query (0 = none)")
+    ap.add_argument("--budget-gb", type=float, default=16.0, help="serving: HBM edge budget (GiB)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -164,6 +179,8 @@ def main(argv=None):
         return 2
 
     c = CONFIGS[args.config]
+    if c.get("serve"):
+        return serve_bench(args, c)
     algo = c["algo"]
     scale = args.scale or c["scale"]
     ef = args.edge_factor or c["ef"]
@@ -333,6 +350,149 @@ def main(argv=None):
         "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
         "engine": {k: info[k] for k in ("block_size", "edges_per_lane", "split_degree", "cu_count",
                                         "hbm_pool_slots")},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def serve_bench(args, c):
+    """Open-loop serving run: requests arrive by a Poisson process and go
+    through the engine's serving loop (cig_serve_*), which coalesces queued
+    updates into groups and admits queries by free HBM. Latency = done -
+    submit as the server clocks it; updates/s = update records applied over
+    the span from the first submission to the last completion."""
+    import coincgraph_amd as cg
+
+    scale = args.scale or c["scale"]
+    ef = args.edge_factor or c["ef"]
+    rng = np.random.default_rng(args.seed + 5)
+    plan, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / args.rate))
+        if t > args.serve_seconds:
+            break
+        i = len(plan)
+        if args.query_every and i % args.query_every == args.query_every - 1:
+            plan.append(("query", t, "bfs" if (i // args.query_every) % 2 == 0 else "sssp"))
+        else:
+            k = int(round(math.exp(rng.uniform(math.log(1e3), math.log(1e6)))))
+            plan.append(("update", t, k // 2, k - k // 2))
+    nd = sum(p[2] for p in plan if p[0] == "update")
+    na = sum(p[3] for p in plan if p[0] == "update")
+    nq = sum(1 for p in plan if p[0] == "query")
+    log(f"plan: {len(plan)} requests over {args.serve_seconds:.0f} s ({nq} queries), {nd}+{na} update records")
+
+    t0 = time.perf_counter()
+    g = cg.rmat_graph(scale, ef, seed=args.seed)
+    log(f"graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    stream = cg.rmat_stream(g, scale, 1, nd, na, seed=args.seed + 1)
+    dels_all, adds_all = stream.batch(0)
+    log(f"update records ({time.perf_counter() - t0:.1f} s)")
+    budget = int(args.budget_gb * 2**30)
+    eng = cg.Engine("sssp", weight_mode=cg.WEIGHT_HASH64, weight_seed=args.weight_seed, cache_bytes=budget)
+    t0 = time.perf_counter()
+    eng.load_graph(g)
+    n, m0 = g.n, g.m
+    del g
+    info = eng.info()
+    log(f"load ({time.perf_counter() - t0:.1f} s): {info}")
+    eng.init_graph(0)
+    st = eng.start()
+    log(f"initial fixed point: {st['total_ms']:.1f} ms, rounds={st['rounds']}")
+    probe_bytes = 1 << 30
+    probe_ms = cg.host_read_probe(probe_bytes, iters=5)
+    probe_gbs = probe_bytes / (probe_ms * 1e-3) / 1e9
+    log(f"host-link probe: {probe_gbs:.1f} GB/s zero-copy read")
+    qout = np.empty(64 * n, dtype=np.uint32)  # shared by every query (contents unused)
+    sources = (np.arange(64, dtype=np.uint64) * 2097143 % n).astype(np.uint32)
+
+    srv = cg.Server(eng, max_group=c["group"], max_query_sources=64, hbm_reserve=1 << 30)
+    tickets = []
+    d0 = a0 = 0
+    start = time.perf_counter()
+    for i, p in enumerate(plan):
+        wait = p[1] - (time.perf_counter() - start)
+        if wait > 0:
+            time.sleep(wait)
+        if p[0] == "update":
+            tk = srv.submit_update(dels_all[d0:d0 + p[2]], adds_all[a0:a0 + p[3]])
+            d0 += p[2]
+            a0 += p[3]
+            tickets.append(("update", tk, p[2] + p[3]))
+        else:
+            tk = srv.submit_query(p[2], sources, out=qout)
+            tickets.append((p[2], tk, 64))
+        if (i + 1) % 100 == 0:
+            log(f"submitted {i + 1}/{len(plan)}")
+    done = {}
+    last = time.perf_counter()
+    while len(done) < len(tickets):
+        for kind, tk, size in tickets:
+            if tk not in done:
+                r = srv.poll(tk)
+                if r is not None:
+                    r.pop("result", None)
+                    done[tk] = (kind, size, r)
+        if time.perf_counter() - last > 10:
+            last = time.perf_counter()
+            log(f"completed {len(done)}/{len(tickets)}")
+        time.sleep(0.01)
+    srv.stop()
+    end_info = eng.info()
+    eng.close()
+
+    res = list(done.values())
+    for kind, size, r in res:
+        if r["status"] != 0:
+            log(f"request failed: {kind} status {r['status']}")
+            return 1
+    upd = [r for kind, _, r in res if kind == "update"]
+    lat_u = [r["done_ms"] - r["submit_ms"] for r in upd]
+    span_ms = max(r["done_ms"] for _, _, r in res) - min(r["submit_ms"] for _, _, r in res)
+    recs = sum(size for kind, size, _ in res if kind == "update")
+    # every request of a coalesced group carries the group's statistics
+    host_edges = sum(r["stats"]["host_edges_scanned"] / max(1, r["group_size"]) for _, _, r in res)
+    dev_ms = sum(r["stats"]["total_ms"] / max(1, r["group_size"]) for _, _, r in res)
+    queries = {}
+    for algo in ("bfs", "sssp"):
+        lq = [r["done_ms"] - r["submit_ms"] for kind, _, r in res if kind == algo]
+        if lq:
+            queries[algo] = {"count": len(lq), "p50_ms": round(percentile(lq, 50), 1),
+                             "p99_ms": round(percentile(lq, 99), 1),
+                             "device_ms_mean": round(float(np.mean([r["stats"]["total_ms"] for kind, _, r in res
+                                                                     if kind == algo])), 1)}
+    groups = [r["group_size"] for r in upd]
+    cold = 1.0 - end_info["cached_edges"] / max(1, end_info["edges"])
+    out = {
+        "metric": "serving p50/p99 update latency + updates/s, 1 GPU",
+        "value": round(recs / (span_ms * 1e-3), 1),
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": len(plan),
+        "warmup": 0,
+        "ms_per_step": round(span_ms / max(1, len(plan)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": c["what"], "config": 5, "vertices": n, "edges": m0,
+                   "budget_bytes": budget, "rate_per_s": args.rate, "window_s": args.serve_seconds,
+                   "query_every": args.query_every, "query_sources": 64},
+        "latency_ms": {"p50": round(percentile(lat_u, 50), 3), "p99": round(percentile(lat_u, 99), 3),
+                       "mean": round(float(np.mean(lat_u)), 3), "max": round(float(np.max(lat_u)), 3)},
+        "updates": {"requests": len(upd), "records": recs, "mean_group": round(float(np.mean(groups)), 2)},
+        "queries": queries,
+        "host_link": {"cold_edge_fraction": round(cold, 4),
+                      "probe_gbs": round(probe_gbs, 1),
+                      "request_gbs": round(host_edges * 4 / (dev_ms * 1e-3) / 1e9, 2) if dev_ms else None,
+                      "host_edges_scanned": int(host_edges)},
+        "roofline": None,
+        "cpu_baseline": None,
+        "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
+        "engine": {k: info[k] for k in ("hbm_pool_slots", "host_pool_slots", "cached_rows", "cached_edges",
+                                        "device_bytes", "host_pinned_bytes")},
     }
     print(json.dumps(out), flush=True)
     return 0

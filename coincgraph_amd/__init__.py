@@ -198,6 +198,7 @@ def lib():
         "cig_edge_weight": (C.c_uint32, [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
         "cig_device_count": (C.c_int, []),
         "cig_copy_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+        "cig_host_read_probe": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
         "cig_autotune": (C.c_int, [eng, C.c_uint32, C.POINTER(TuneResult)]),
         "cig_serve_config_default": (C.c_int, [C.POINTER(ServeConfig)]),
         "cig_serve_start": (C.c_int, [eng, C.POINTER(ServeConfig), C.POINTER(P)]),
@@ -255,6 +256,13 @@ def copy_probe(bytes_: int, iters: int = 20, device: int = 0) -> float:
     """Mean milliseconds of a plain copy kernel moving `bytes_` bytes."""
     ms = C.c_double()
     _check(lib().cig_copy_probe(device, bytes_, iters, C.byref(ms)))
+    return ms.value
+
+
+def host_read_probe(bytes_: int, iters: int = 10, device: int = 0) -> float:
+    """Mean milliseconds of a kernel reading `bytes_` bytes of pinned host memory zero-copy."""
+    ms = C.c_double()
+    _check(lib().cig_host_read_probe(device, bytes_, iters, C.byref(ms)))
     return ms.value
 
 
@@ -557,9 +565,14 @@ class Server:
                                        _ptr(a) if a.size else None, a.size, C.byref(t)))
         return t.value
 
-    def submit_query(self, algo: str, sources) -> int:
+    def submit_query(self, algo: str, sources, out=None) -> int:
+        """`out` (uint32, at least len(sources)*n elements) receives the
+        distances; it is allocated here when not given."""
         src = np.ascontiguousarray(sources, dtype=np.uint32)
-        out = np.zeros((len(src), self._eng.n), dtype=np.uint32)
+        if out is None:
+            out = np.zeros((len(src), self._eng.n), dtype=np.uint32)
+        elif out.dtype != np.uint32 or not out.flags.c_contiguous or out.size < len(src) * self._eng.n:
+            raise ValueError("out must be a contiguous uint32 array of at least len(sources)*n elements")
         t = C.c_uint64()
         _check(lib().cig_submit_query(self._s, ALGOS[algo], _ptr(src), len(src), _ptr(out), out.nbytes,
                                       C.byref(t)))

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -31,6 +32,7 @@
 
 #include "../../include/cig.h"
 #include "common.h"
+#include "host_par.h"
 #include "kernels.hpp"
 
 using namespace cig;
@@ -260,17 +262,108 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
     }
 
+    // Incoming rows in CSR form: the sources of each vertex's in-edges,
+    // ascending. Threads take source ranges of about m/T edges and bucket
+    // (dst, src) pairs by destination block; each block is then counting-
+    // sorted by destination, which keeps the sources in ascending order.
+    void build_incoming(const uint64_t *off, const uint32_t *dst, unsigned T, std::vector<uint64_t> &ioff,
+                        std::vector<uint32_t> &isrc) {
+        const uint64_t m = off[n];
+        ioff.assign(size_t(n) + 1, 0);
+        isrc.resize(m);
+        unsigned bb = 0;  // destination block = 2^bb vertices, at most 4096 blocks
+        while ((uint64_t(n) >> bb) > 4096) ++bb;
+        const uint64_t B = ((uint64_t(n) - 1) >> bb) + 1;
+        std::vector<uint32_t> sb(T + 1, n);
+        for (unsigned t = 0; t < T; ++t)
+            sb[t] = uint32_t(std::lower_bound(off, off + n, m * t / T) - off);
+        std::vector<uint64_t> cnt(size_t(T) * B, 0);
+        std::vector<std::thread> pool;
+        auto each_thread = [&](auto f) {
+            pool.clear();
+            for (unsigned t = 0; t < T; ++t) pool.emplace_back([&, t] { f(t); });
+            for (auto &th : pool) th.join();
+        };
+        each_thread([&](unsigned t) {
+            uint64_t *c = cnt.data() + size_t(t) * B;
+            for (uint64_t e = off[sb[t]]; e < off[sb[t + 1]]; ++e) c[dst[e] >> bb]++;
+        });
+        std::vector<uint64_t> bstart(B + 1, 0);
+        uint64_t run = 0;
+        for (uint64_t b = 0; b < B; ++b) {
+            bstart[b] = run;
+            for (unsigned t = 0; t < T; ++t) {
+                const uint64_t x = cnt[size_t(t) * B + b];
+                cnt[size_t(t) * B + b] = run;
+                run += x;
+            }
+        }
+        bstart[B] = m;
+        std::vector<uint64_t> pairs(m);
+        each_thread([&](unsigned t) {
+            uint64_t *c = cnt.data() + size_t(t) * B;
+            for (uint32_t s = sb[t]; s < sb[t + 1]; ++s)
+                for (uint64_t e = off[s]; e < off[s + 1]; ++e) pairs[c[dst[e] >> bb]++] = uint64_t(dst[e]) << 32 | s;
+        });
+        std::atomic<uint64_t> next{0};
+        each_thread([&](unsigned) {
+            std::vector<uint64_t> local(size_t(1) << bb);
+            for (uint64_t b; (b = next.fetch_add(1)) < B;) {
+                const uint64_t base = b << bb, nb = std::min<uint64_t>(uint64_t(1) << bb, n - base);
+                std::fill(local.begin(), local.begin() + int64_t(nb), 0);
+                for (uint64_t i = bstart[b]; i < bstart[b + 1]; ++i) local[(pairs[i] >> 32) - base]++;
+                uint64_t r = bstart[b];
+                for (uint64_t k = 0; k < nb; ++k) {
+                    ioff[base + k] = r;
+                    const uint64_t x = local[k];
+                    local[k] = r;
+                    r += x;
+                }
+                for (uint64_t i = bstart[b]; i < bstart[b + 1]; ++i)
+                    isrc[local[(pairs[i] >> 32) - base]++] = uint32_t(pairs[i]);
+            }
+        });
+        ioff[n] = m;
+    }
+
+    // Budget placement order: decreasing capacity (a power of two), outgoing
+    // before incoming rows of the same capacity, row id ascending. Long rows
+    // of either direction come first: besides traversal reads, a batch's
+    // deletes scan every long row they touch, and the hub rows are touched
+    // by nearly every batch.
+    static std::vector<uint32_t> placement_order(const std::vector<Row> &hrows, uint32_t n) {
+        auto key = [&](uint32_t v) -> uint32_t {
+            const uint32_t cap = hrows[v].cap;
+            const uint32_t lg = cap ? 31u - uint32_t(__builtin_clz(cap)) : 0u;
+            return 2u * (cap ? 32u - lg : 33u) + (v < n ? 0u : 1u);
+        };
+        const uint32_t nrows = uint32_t(hrows.size());
+        std::vector<uint64_t> start(69, 0);  // keys 2..67
+        for (uint32_t v = 0; v < nrows; ++v) start[key(v) + 1]++;
+        for (int k = 0; k < 68; ++k) start[k + 1] += start[k];
+        std::vector<uint32_t> order(nrows);
+        for (uint32_t v = 0; v < nrows; ++v) order[start[key(v)]++] = v;
+        return order;
+    }
+
     uint32_t round_bound() const { return uint32_t(std::min<uint64_t>(2ull * n + 64, 0x7FFFFFFFull)); }
 
     // ------------------------------------------------------------ load
     void load(uint32_t nv, const uint64_t *off, const uint32_t *dst) {
         if (nv == 0) throw HipError(CIG_EINVAL, "graph must have at least one vertex");
-        for (uint32_t v = 0; v < nv; ++v)
-            if (off[v + 1] < off[v] || off[v + 1] - off[v] > 0xFFFFFFFFull)
-                throw HipError(CIG_EINVAL, "offsets must be non-decreasing, row length < 2^32");
+        const unsigned T = pick_threads(0);
+        std::atomic<int> bad{0};
+        parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+            for (uint64_t v = lo; v < hi; ++v)
+                if (off[v + 1] < off[v] || off[v + 1] - off[v] > 0xFFFFFFFFull) bad = 1;
+        });
+        if (bad) throw HipError(CIG_EINVAL, "offsets must be non-decreasing, row length < 2^32");
         const uint64_t m = off[nv];
-        for (uint64_t e = 0; e < m; ++e)
-            if (dst[e] >= nv) throw HipError(CIG_EINVAL, "destination id out of range");
+        parallel_for(m, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+            for (uint64_t e = lo; e < hi; ++e)
+                if (dst[e] >= nv) bad = 1;
+        });
+        if (bad) throw HipError(CIG_EINVAL, "destination id out of range");
         n = nv;
         budget_slots = cfg.cache_bytes / sizeof(uint32_t);
         // incoming rows (the device reverse index) for deletion repair
@@ -280,38 +373,30 @@ struct cig_engine {
         nrows = rev ? 2 * n : n;
         std::vector<uint64_t> ioff;
         std::vector<uint32_t> isrc;
-        if (rev) {  // counting sort by destination; sources ascending per row
-            ioff.assign(size_t(n) + 1, 0);
-            for (uint64_t e = 0; e < m; ++e) ++ioff[dst[e] + 1];
-            for (uint32_t v = 0; v < n; ++v) ioff[v + 1] += ioff[v];
-            isrc.resize(m);
-            std::vector<uint64_t> fillp(ioff.begin(), ioff.end() - 1);
-            for (uint32_t u = 0; u < n; ++u)
-                for (uint64_t e = off[u]; e < off[u + 1]; ++e) isrc[fillp[dst[e]]++] = u;
-        }
+        if (rev) build_incoming(off, dst, T, ioff, isrc);
         auto row_data = [&](uint32_t v) -> const uint32_t * {
             return v < n ? dst + off[v] : isrc.data() + ioff[v - n];
         };
 
         std::vector<Row> hrows(nrows);
-        uint64_t total_cap = 0;
-        for (uint32_t v = 0; v < nrows; ++v) {
-            hrows[v].deg = uint32_t(v < n ? off[v + 1] - off[v] : ioff[v - n + 1] - ioff[v - n]);
-            hrows[v].cap = uint32_t(next_pow2_cap(hrows[v].deg));
-            total_cap += hrows[v].cap;
-        }
-        // tier placement: longest outgoing rows first while the budget lasts
-        // (90% fill), then incoming rows (read by the deletion repair and by
-        // PageRank's pull rounds)
+        std::vector<uint64_t> part(T, 0);
+        parallel_for(nrows, T, [&](uint64_t lo, uint64_t hi, unsigned t) {
+            uint64_t sum = 0;
+            for (uint64_t v = lo; v < hi; ++v) {
+                hrows[v].deg = uint32_t(v < n ? off[v + 1] - off[v] : ioff[v - n + 1] - ioff[v - n]);
+                hrows[v].cap = uint32_t(next_pow2_cap(hrows[v].deg));
+                sum += hrows[v].cap;
+            }
+            part[t] = sum;
+        });
+        const uint64_t total_cap = std::accumulate(part.begin(), part.end(), uint64_t(0));
+        // tier placement: rows by decreasing capacity while the budget lasts
+        // (90% fill); incoming rows are read by the deletion repair, by
+        // PageRank's pull rounds and by the deletes' long-row scans
         std::vector<uint8_t> in_hbm(nrows, 1);
         uint64_t hbm_need = total_cap, host_need = 0;
         if (budget_slots && total_cap > budget_slots * 9 / 10) {
-            std::vector<uint32_t> order(nrows);
-            std::iota(order.begin(), order.end(), 0u);
-            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-                if ((a < n) != (b < n)) return a < n;
-                return hrows[a].deg > hrows[b].deg;
-            });
+            const std::vector<uint32_t> order = placement_order(hrows, n);
             uint64_t used = 0;
             const uint64_t limit = budget_slots * 9 / 10;
             for (uint32_t v : order) {
@@ -327,28 +412,33 @@ struct cig_engine {
         hbm_pool.alloc(hbm_slots);
         host_pool.alloc(host_slots);
         host_pool_dev = host_pool.device_ptr();
-        std::vector<uint32_t> stage;
-        stage.reserve(hbm_need);
+        // rows keep row-id order within each tier; the copies run in parallel
         uint64_t htail = 0, ctail = 0;
         for (uint32_t v = 0; v < nrows; ++v) {
             Row &r = hrows[v];
-            const uint32_t *src = row_data(v);
             if (in_hbm[v]) {
                 r.off = htail;
-                stage.insert(stage.end(), src, src + r.deg);
-                stage.resize(htail + r.cap, 0u);
                 htail += r.cap;
             } else {
                 r.off = ctail | kHostTier;
-                std::memcpy(host_pool.p + ctail, src, size_t(r.deg) * sizeof(uint32_t));
                 ctail += r.cap;
             }
         }
+        std::vector<uint32_t> stage(htail, 0u);
+        parallel_for(nrows, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+            for (uint64_t v = lo; v < hi; ++v) {
+                const Row &r = hrows[v];
+                if (!r.deg) continue;
+                uint32_t *to = (r.off & kHostTier) ? host_pool.p + (r.off & ~kHostTier) : stage.data() + r.off;
+                std::memcpy(to, row_data(uint32_t(v)), size_t(r.deg) * sizeof(uint32_t));
+            }
+        });
         if (!stage.empty())
             HIPCHK(hipMemcpyAsync(hbm_pool.p, stage.data(), stage.size() * sizeof(uint32_t),
                                   hipMemcpyHostToDevice, stream));
         rows.alloc(nrows);
         HIPCHK(hipMemcpyAsync(rows.p, hrows.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice, stream));
+        sync();  // the staging vectors go out of scope below
 
         // per-vertex arrays
         q0.alloc(n);
@@ -2150,6 +2240,40 @@ cig_status cig_copy_probe(int32_t device, uint64_t bytes, uint32_t iters, double
         k_copy_probe<<<grid, 256>>>(a.p, b.p, n16);  // warm
         HIPCHK(hipEventRecord(e0, nullptr));
         for (uint32_t i = 0; i < iters; ++i) k_copy_probe<<<grid, 256>>>(a.p, b.p, n16);
+        HIPCHK(hipEventRecord(e1, nullptr));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = double(t) / iters;
+        return CIG_OK;
+    } catch (const HipError &x) {
+        g_create_error = x.what();
+        return x.code;
+    }
+}
+
+cig_status cig_host_read_probe(int32_t device, uint64_t bytes, uint32_t iters, double *ms) {
+    if (!ms || bytes < 16 || !iters) return CIG_EINVAL;
+    try {
+        HIPCHK(hipSetDevice(device));
+        const uint64_t n16 = bytes / 16;
+        HBuf<uint4> h;
+        h.alloc(n16);
+        std::memset(h.p, 0x5A, n16 * 16);
+        DBuf<uint32_t> sink;
+        sink.alloc(1);
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        const uint32_t grid = uint32_t(prop.multiProcessorCount) * 16;
+        const uint4 *src = h.device_ptr();
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        k_host_read_probe<<<grid, 256>>>(src, n16, sink.p);  // warm
+        HIPCHK(hipEventRecord(e0, nullptr));
+        for (uint32_t i = 0; i < iters; ++i) k_host_read_probe<<<grid, 256>>>(src, n16, sink.p);
         HIPCHK(hipEventRecord(e1, nullptr));
         HIPCHK(hipEventSynchronize(e1));
         float t = 0;

@@ -19,6 +19,7 @@
 
 #include "../../include/cig.h"
 #include "common.h"
+#include "host_par.h"
 
 namespace cig {
 
@@ -30,24 +31,6 @@ uint64_t splitmix64(uint64_t x) {
 }
 
 namespace {
-
-unsigned pick_threads(uint32_t requested) {
-    if (requested) return requested;
-    unsigned hw = std::thread::hardware_concurrency();
-    return std::max(1u, std::min(hw ? hw : 1u, 16u));
-}
-
-template <class F>
-void parallel_for(uint64_t n, unsigned threads, F f) {
-    if (threads <= 1 || n < 4096) { f(0, n, 0u); return; }
-    std::vector<std::thread> pool;
-    uint64_t chunk = (n + threads - 1) / threads;
-    for (unsigned t = 0; t < threads; ++t) {
-        uint64_t b = std::min<uint64_t>(n, t * chunk), e = std::min<uint64_t>(n, b + chunk);
-        pool.emplace_back([=] { f(b, e, t); });
-    }
-    for (auto &th : pool) th.join();
-}
 
 // One R-MAT sample: `scale` quadrant choices from 24-bit uniforms drawn from
 // a counter-based stream keyed by (seed, sample index).

@@ -1990,6 +1990,18 @@ __global__ void k_copy_probe(const uint4 *src, uint4 *dst, uint64_t n16) {
     for (uint64_t i = gthread(); i < n16; i += gthreads()) dst[i] = src[i];
 }
 
+// Zero-copy read of pinned host memory (16 B per lane per step): the
+// host-link reference point for rows in the host tier. The XOR of what each
+// thread read goes to `sink` so that the loads stay.
+__global__ void k_host_read_probe(const uint4 *src, uint64_t n16, uint32_t *sink) {
+    uint32_t x = 0;
+    for (uint64_t i = gthread(); i < n16; i += gthreads()) {
+        const uint4 v = src[i];
+        x ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (x == 0x9E3779B9u) sink[0] = x;  // practically never; keeps the loads
+}
+
 // FP64 residual norm sum |r| (the only FP64 arithmetic of the PR path).
 __global__ void k_pr_l1(const float *res, uint32_t n, StatShard *stats) {
     double s = 0;

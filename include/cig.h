@@ -383,6 +383,10 @@ int cig_device_count(void);
  * HIP-event time. The HBM reference point for achievable bandwidth and for
  * calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE. No reference counterpart. */
 cig_status cig_copy_probe(int32_t device, uint64_t bytes, uint32_t iters, double *ms_per_launch);
+/* Diagnostics: times a kernel reading `bytes` bytes of pinned, mapped host
+ * memory zero-copy (the access path of rows in the host tier), over `iters`
+ * launches. The host-link reference point. No reference counterpart. */
+cig_status cig_host_read_probe(int32_t device, uint64_t bytes, uint32_t iters, double *ms_per_launch);
 
 #ifdef __cplusplus
 }
