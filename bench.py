@@ -168,7 +168,7 @@ def main(argv=None):
                     help="PageRank: scatter every round with atomics (keeps no incoming rows)")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving: arrival window (s)")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate (requests/s)")
-    ap.add_argument("--query-every", type=int, default=100,
+    ap.add_argument("--query-every", type=int, default=200,
                     help="serving: every k-th request is a 64-source query (0 = none)")
     ap.add_argument("--budget-gb", type=float, default=16.0, help="serving: HBM edge budget (GiB)")
     args = ap.parse_args(argv)

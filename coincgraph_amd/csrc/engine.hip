@@ -160,7 +160,9 @@ struct cig_engine {
     const uint8_t *cur_skip = nullptr;  // cancel flags of the group being applied
     // multi-source queries
     DBuf<uint32_t> ms_dist, ms_stamp, ms_src, ms_out;
-    DBuf<unsigned long long> ms_act, ms_snap;
+    DBuf<unsigned long long> ms_act, ms_snap, ms_seen;
+    HBuf<uint8_t> bounce[2];  // pinned bounce buffers of large device-to-host results
+    hipEvent_t bounce_ev[2] = {};
     DBuf<uint2> hcnts;
     DBuf<DelTile> dtiles;
     uint64_t rec_cap = 0;
@@ -1646,9 +1648,10 @@ struct cig_engine {
     void ms_run(MsArgs &a) {
         advance_round(run_chain(
             [&](uint64_t tag, uint32_t seq) {
+                // whole-graph frontiers: the one-shot grid, not the round grid
                 k_ms_consume<<<grid_small, 256, 0, stream>>>(a, prog_dev, tag, seq);
-                k_ms_expand<W, QPL><<<grid_round, kBlock, 0, stream>>>(a, seq);
-                k_ms_tiles<W, QPL><<<grid_round, kBlock, 0, stream>>>(a, seq);
+                k_ms_expand<W, QPL><<<grid_small, kBlock, 0, stream>>>(a, seq);
+                k_ms_tiles<W, QPL><<<grid_small, kBlock, 0, stream>>>(a, seq);
             },
             round_bound(), "multi-source query"));
     }
@@ -1678,12 +1681,15 @@ struct cig_engine {
             ms_stamp.alloc(n);
             fill32(ms_stamp.p, n, 0);
         }
+        const bool bfs = algo == CIG_ALGO_BFS;
+        if (bfs && ms_seen.count < n) ms_seen.alloc(n);
         if (!ms_src.count) ms_src.alloc(64);
         if (ms_out.count < uint64_t(n) * k) ms_out.alloc(uint64_t(n) * k);
         begin_call(st);
         fill32(ms_dist.p, uint64_t(n) * stride, kInf);
         fill32(reinterpret_cast<uint32_t *>(ms_act.p), uint64_t(n) * 2, 0);
         fill32(reinterpret_cast<uint32_t *>(ms_snap.p), uint64_t(n) * 2, 0);
+        if (bfs) fill32(reinterpret_cast<uint32_t *>(ms_seen.p), uint64_t(n) * 2, 0);
         HIPCHK(hipMemcpyAsync(ms_src.p, sources, k * 4, hipMemcpyHostToDevice, stream));
         MsArgs a{};
         a.ctl = ctl.p;
@@ -1691,6 +1697,7 @@ struct cig_engine {
         a.dist = ms_dist.p;
         a.act = ms_act.p;
         a.snap = ms_snap.p;
+        a.seen = bfs ? ms_seen.p : nullptr;
         a.stamp = ms_stamp.p;
         a.q0 = q0.p;
         a.q1 = q1.p;
@@ -1712,8 +1719,43 @@ struct cig_engine {
         HIPCHK(hipEventRecord(ev[4], stream));
         k_ms_gather<<<grid_small, 256, 0, stream>>>(ms_dist.p, n, stride, k, ms_out.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out, ms_out.p, uint64_t(n) * k * 4, hipMemcpyDeviceToHost, stream));
+        copy_out(out, ms_out.p, uint64_t(n) * k * 4);
         end_call(st, 4);
+    }
+
+    // Device-to-host copy into caller (pageable) memory. Large results go
+    // through two pinned bounce buffers: the device copies chunk i while host
+    // threads move chunk i-1 into `out`. Returns with the copy complete.
+    void copy_out(void *out, const void *dev, uint64_t bytes) {
+        constexpr uint64_t kChunk = uint64_t(256) << 20;
+        if (bytes <= kChunk) {
+            HIPCHK(hipMemcpyAsync(out, dev, bytes, hipMemcpyDeviceToHost, stream));
+            sync();
+            return;
+        }
+        if (!bounce[0].p) {
+            for (int b = 0; b < 2; ++b) {
+                bounce[b].alloc(kChunk);
+                HIPCHK(hipEventCreateWithFlags(&bounce_ev[b], hipEventDisableTiming));
+            }
+        }
+        const unsigned T = pick_threads(0);
+        auto host_copy = [&](uint64_t chunk) {
+            const uint64_t off = chunk * kChunk, len = std::min(kChunk, bytes - off);
+            HIPCHK(hipEventSynchronize(bounce_ev[chunk & 1]));
+            uint8_t *to = static_cast<uint8_t *>(out) + off;
+            const uint8_t *from = bounce[chunk & 1].p;
+            parallel_for(len, T, [&](uint64_t lo, uint64_t hi, unsigned) { std::memcpy(to + lo, from + lo, hi - lo); });
+        };
+        const uint64_t chunks = (bytes + kChunk - 1) / kChunk;
+        for (uint64_t i = 0; i < chunks; ++i) {
+            const uint64_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+            HIPCHK(hipMemcpyAsync(bounce[i & 1].p, static_cast<const uint8_t *>(dev) + off, len,
+                                  hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipEventRecord(bounce_ev[i & 1], stream));
+            if (i > 0) host_copy(i - 1);  // frees buffer (i-1)&1 for chunk i+1
+        }
+        host_copy(chunks - 1);
     }
 
     // ------------------------------------------------------------ cache
@@ -1865,6 +1907,8 @@ struct cig_engine {
                 if (e) (void)hipEventDestroy(e);
             for (auto &e : rev_begin) (void)hipEventDestroy(e);
             for (auto &e : rev_end) (void)hipEventDestroy(e);
+            for (auto &e : bounce_ev)
+                if (e) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(stream);
         }
     }
@@ -2069,8 +2113,11 @@ struct cig_server {
             } else {
                 try {
                     const uint32_t k = [&] { uint32_t t = 0; for (auto &r : batch) t += uint32_t(r->sources.size()); return t; }();
-                    const uint64_t need = uint64_t(e->n) * (((k + 3) & ~3u) + k) * 4 + uint64_t(e->n) * 20;
-                    if (need + cfg.hbm_reserve > free_hbm() + e->ms_dist.bytes() + e->ms_out.bytes() + e->ms_act.bytes() * 3) {
+                    // distances, the source-major copy, act/snap/seen masks and stamps
+                    const uint64_t need = uint64_t(e->n) * (((k + 3) & ~3u) + k) * 4 + uint64_t(e->n) * 28;
+                    const uint64_t held = e->ms_dist.bytes() + e->ms_out.bytes() + e->ms_act.bytes() +
+                                          e->ms_snap.bytes() + e->ms_seen.bytes() + e->ms_stamp.bytes();
+                    if (need + cfg.hbm_reserve > free_hbm() + held) {
                         for (auto &r : batch) fail(r, CIG_ENOMEM);
                         continue;
                     }

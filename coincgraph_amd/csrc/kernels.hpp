@@ -1861,6 +1861,7 @@ struct MsArgs {
     const Row *rows;
     uint32_t *dist;
     unsigned long long *act, *snap;
+    unsigned long long *seen;  // BFS: sources that reached each vertex
     uint32_t *stamp;
     uint32_t *q0, *q1;
     Tile *t0, *t1;
@@ -1889,20 +1890,101 @@ struct MsOp {
         return snap[t.v] != 0ull;
     }
     __device__ uint64_t peek(uint32_t) const { return 0; }
+    // Sources are taken eight at a time: their 16 distance loads are in
+    // flight together and the minimums are non-returning atomics. A source
+    // counts as improved when its candidate beat the value read; if another
+    // lane lowered it further meanwhile, w is merely expanded once more for
+    // that source with its (smaller) current distance, which changes nothing.
     __device__ bool edge(uint32_t u, uint32_t, uint32_t w, uint64_t) const {
         const uint32_t wt = weight<W>(u, w, seed);
         const uint32_t *du = dist + uint64_t(u) * stride;
         uint32_t *dw = dist + uint64_t(w) * stride;
         unsigned long long gained = 0;
-        for (unsigned long long m = snap[u]; m; m &= m - 1) {
-            const uint32_t s = uint32_t(__ffsll(m) - 1);
-            const uint64_t c = uint64_t(du[s]) + wt;
-            if (c >= dw[s]) continue;  // also rejects c >= kInf
-            if (atomicMin(&dw[s], uint32_t(c)) > uint32_t(c)) gained |= 1ull << s;
+        for (unsigned long long m = snap[u]; m;) {
+            constexpr int kB = 8;
+            uint32_t s[kB], dv[kB], wv[kB];
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                s[k] = m ? uint32_t(__ffsll(m) - 1) : 0u;
+                cnt += m ? 1 : 0;
+                m &= m - 1;
+            }
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                dv[k] = k < cnt ? du[s[k]] : kInf;
+                wv[k] = k < cnt ? dw[s[k]] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const uint64_t c = uint64_t(dv[k]) + wt;
+                if (k < cnt && c < wv[k]) {  // also rejects c >= kInf
+                    __hip_atomic_fetch_min(&dw[s[k]], uint32_t(c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    gained |= 1ull << s[k];
+                }
+            }
         }
         if (!gained) return false;
         atomicOr(&act[w], gained);
         return claim(stamp, w, tag);
+    }
+};
+
+// BFS from up to 64 sources, bit-parallel: the rounds are level-synchronous
+// (a round expands exactly the bits its vertices gained in the previous
+// round, all at the same depth), so a destination's depth for source s is
+// fixed the first time bit s reaches it. One 64-bit atomicOr on seen[w] per
+// edge finds the newly reached sources; each (w, s) depth is written once by
+// the lane that set the bit. Same depths as the per-source relaxation.
+struct MsBfsOp {
+    const Row *rows;
+    uint32_t *dist;
+    unsigned long long *act, *seen;
+    const unsigned long long *snap;
+    uint32_t *stamp;
+    uint32_t stride, tag;
+    // payload: the depth of the row owner's expanding sources, plus one
+    __device__ uint32_t depth_after(uint32_t v) const {
+        const uint32_t s = uint32_t(__ffsll(snap[v]) - 1);
+        return dist[uint64_t(v) * stride + s] + 1u;
+    }
+    __device__ bool source(uint32_t v, uint32_t, Row &row, uint32_t &pay) const {
+        if (snap[v] == 0ull) return false;
+        row = rows[v];
+        pay = depth_after(v);
+        return row.deg != 0;
+    }
+    __device__ bool tile_source(const Tile &t, uint32_t &pay) const {
+        if (snap[t.v] == 0ull) return false;
+        pay = depth_after(t.v);
+        return true;
+    }
+    // seen bits only grow, so a stale read can only under-report them
+    __device__ uint64_t peek(uint32_t w) const { return seen[w]; }
+    __device__ bool edge(uint32_t u, uint32_t depth, uint32_t w, uint64_t cur) const {
+        const unsigned long long bits = snap[u] & ~(unsigned long long)cur;
+        if (!bits) return false;
+        const unsigned long long gained = bits & ~atomicOr(&seen[w], bits);
+        if (!gained) return false;
+        uint32_t *dw = dist + uint64_t(w) * stride;
+        for (unsigned long long m = gained; m; m &= m - 1) dw[__ffsll(m) - 1] = depth;
+        atomicOr(&act[w], gained);
+        return claim(stamp, w, tag);
+    }
+};
+
+template <int W>
+struct MsOpOf {
+    using type = MsOp<W>;
+    __device__ static MsOp<W> make(const MsArgs &a, uint32_t tag) {
+        return MsOp<W>{a.rows, a.dist, a.act, a.snap, a.stamp, a.stride, tag, a.seed};
+    }
+};
+template <>
+struct MsOpOf<2> {
+    using type = MsBfsOp;
+    __device__ static MsBfsOp make(const MsArgs &a, uint32_t tag) {
+        return MsBfsOp{a.rows, a.dist, a.act, a.seen, a.snap, a.stamp, a.stride, tag};
     }
 };
 
@@ -1912,6 +1994,7 @@ __global__ void k_ms_seed(MsArgs a, const uint32_t *sources, uint32_t k) {
         const uint32_t v = sources[s];
         a.dist[uint64_t(v) * a.stride + s] = 0;
         atomicOr(&a.act[v], 1ull << s);
+        if (a.seen) atomicOr(&a.seen[v], 1ull << s);
         if (claim(a.stamp, v, r)) {
             const uint32_t at = atomicAdd(&a.ctl->qcount[r % 3], 1u);
             ((r & 1) ? a.q1 : a.q0)[at] = v;
@@ -1948,7 +2031,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_expand(MsArgs a, uint32_t seq) {
     const uint32_t r = c->round + seq - 1;
     const uint32_t nq = c->qcount[r % 3];
     if (nq == 0) return;
-    MsOp<W> op{a.rows, a.dist, a.act, a.snap, a.stamp, a.stride, r + 1, a.seed};
+    const auto op = MsOpOf<W>::make(a, r + 1);
     Counters ctr;
     const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
     const StagedOut<2048, 128> out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
@@ -1967,7 +2050,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_tiles(MsArgs a, uint32_t seq) {
     const uint32_t r = c->round + seq - 1;
     const uint32_t nt = c->scount;
     if (nt == 0) return;
-    MsOp<W> op{a.rows, a.dist, a.act, a.snap, a.stamp, a.stride, r + 1, a.seed};
+    const auto op = MsOpOf<W>::make(a, r + 1);
     const StagedOut<2048, 128> out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
                                    TileSink{nullptr, &c->scount}, &stage};
     out_init(out);
