@@ -67,3 +67,27 @@ def test_query_rejects_bad_arguments():
             e.query("cc", [0])
         with pytest.raises(cg.CigError):
             e.query("bfs", [g.n])
+
+
+@pytest.mark.parametrize("algo", ["bfs", "sssp"])
+def test_large_result_copy_matches(algo):
+    """A result above 256 MiB leaves the device through pinned bounce buffers
+    in chunks; it must equal the direct copy of smaller queries from the same
+    sources (64 sources x 2^21 vertices x 4 B = two 256-MiB chunks)."""
+    print(f"[gpu-case] large query result {algo}", flush=True)
+    g = cg.rmat_graph(21, 2, seed=53)
+    with cg.Engine("sssp", weight_mode=cg.WEIGHT_HASH64, weight_seed=9) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        sources = (np.arange(64, dtype=np.uint64) * 16411 % g.n).astype(np.uint32)
+        sources[0] = 0
+        big = e.query(algo, sources)
+        assert big.nbytes > 256 << 20
+        for lo in (0, 32, 60):
+            small = e.query(algo, sources[lo:lo + 4])
+            assert np.array_equal(big[lo:lo + 4], small), f"sources {lo}..{lo + 3}"
+        if algo == "sssp":
+            assert np.array_equal(big[0], e.values())
+        ref = _expect(oracle_graph(g), algo, sources[1], cg.WEIGHT_HASH64, 9)
+        assert np.array_equal(big[1], ref)
