@@ -1665,14 +1665,34 @@ struct cig_engine {
         }
     }
 
+    // One caller buffer of a (possibly merged) query: the distances of its
+    // `nsrc` consecutive sources.
+    struct QueryOut {
+        uint32_t *p;
+        uint32_t nsrc;
+    };
+
     void query(uint32_t algo, const uint32_t *sources, uint32_t k, uint32_t *out, uint64_t bytes,
+               cig_batch_stats *st) {
+        if (!out || bytes < uint64_t(n) * k * 4) throw HipError(CIG_EINVAL, "output buffer too small");
+        const QueryOut one{out, k};
+        query(algo, sources, k, &one, 1, st);
+    }
+
+    // The results go straight to each caller's buffer (no merged host copy).
+    void query(uint32_t algo, const uint32_t *sources, uint32_t k, const QueryOut *outs, size_t nouts,
                cig_batch_stats *st) {
         if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
         if (algo != CIG_ALGO_BFS && algo != CIG_ALGO_SSSP) throw HipError(CIG_EINVAL, "queries are BFS or SSSP");
         if (k == 0 || k > 64 || !sources) throw HipError(CIG_EINVAL, "1..64 sources");
         for (uint32_t s = 0; s < k; ++s)
             if (sources[s] >= n) throw HipError(CIG_EINVAL, "source out of range");
-        if (!out || bytes < uint64_t(n) * k * 4) throw HipError(CIG_EINVAL, "output buffer too small");
+        uint64_t placed = 0;
+        for (size_t i = 0; i < nouts; ++i) {
+            if (!outs[i].p) throw HipError(CIG_EINVAL, "null output buffer");
+            placed += outs[i].nsrc;
+        }
+        if (placed != k) throw HipError(CIG_EINVAL, "output buffers must cover every source");
         const uint32_t stride = (k + 3u) & ~3u;
         if (ms_dist.count < uint64_t(n) * stride) ms_dist.alloc(uint64_t(n) * stride);
         if (ms_act.count < n) {
@@ -1719,7 +1739,11 @@ struct cig_engine {
         HIPCHK(hipEventRecord(ev[4], stream));
         k_ms_gather<<<grid_small, 256, 0, stream>>>(ms_dist.p, n, stride, k, ms_out.p);
         HIPCHK(hipGetLastError());
-        copy_out(out, ms_out.p, uint64_t(n) * k * 4);
+        uint64_t at = 0;
+        for (size_t i = 0; i < nouts; ++i) {
+            copy_out(outs[i].p, ms_out.p + at, uint64_t(n) * outs[i].nsrc * 4);
+            at += uint64_t(n) * outs[i].nsrc;
+        }
         end_call(st, 4);
     }
 
@@ -2044,14 +2068,9 @@ struct cig_server {
         cig_batch_stats stats{};
         cig_status st = CIG_OK;
         try {
-            std::vector<uint32_t> merged(size_t(e->n) * src.size());
-            e->query(batch[0]->algo, src.data(), uint32_t(src.size()), merged.data(), merged.size() * 4, &stats);
-            size_t at = 0;
-            for (auto &r : batch) {
-                const size_t words = size_t(e->n) * r->sources.size();
-                std::memcpy(r->out, merged.data() + at, words * 4);
-                at += words;
-            }
+            std::vector<cig_engine::QueryOut> outs;
+            for (auto &r : batch) outs.push_back({r->out, uint32_t(r->sources.size())});
+            e->query(batch[0]->algo, src.data(), uint32_t(src.size()), outs.data(), outs.size(), &stats);
         } catch (const HipError &x) {
             st = x.code;
             e->err = x.what();
