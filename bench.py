@@ -442,6 +442,13 @@ def serve_bench(args, c):
             log(f"completed {len(done)}/{len(tickets)}")
         time.sleep(0.01)
     srv.stop()
+    # cache admission on the activity the serving run left (age, histogram,
+    # threshold, evict/load on the device), then once more with no new activity
+    refresh = []
+    for _ in range(2):
+        rs = eng.cache_refresh()
+        refresh.append({"ms": round(rs["total_ms"], 2), "rows_moved": rs["relocated_rows"]})
+    log(f"cache refresh: {refresh}")
     end_info = eng.info()
     eng.close()
 
@@ -491,6 +498,7 @@ def serve_bench(args, c):
                     "service_ms": {"p50": round(percentile([r["done_ms"] - r["start_ms"] for r in upd], 50), 3),
                                    "p99": round(percentile([r["done_ms"] - r["start_ms"] for r in upd], 99), 3)}},
         "queries": queries,
+        "cache_refresh": refresh,
         "host_link": {"cold_edge_fraction": round(cold, 4),
                       "probe_gbs": round(probe_gbs, 1),
                       "request_gbs": round(host_edges * 4 / (dev_ms * 1e-3) / 1e9, 2) if dev_ms else None,

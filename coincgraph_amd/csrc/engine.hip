@@ -146,6 +146,7 @@ struct cig_engine {
     DBuf<PullRow> prow;
     DBuf<float> tpart;
     DBuf<unsigned long long> amask;  // active-source mask of the pull passes
+    DBuf<unsigned long long> adm_hist, adm_acc;  // cache admission: capacity per score, sums
     uint64_t ptile_cap = 0;
     DBuf<uint64_t> cnt64, fill64, gofs;
     DBuf<uint32_t> touched, grouped, small_list, big_list, add_list, overflow;
@@ -446,9 +447,10 @@ struct cig_engine {
         q0.alloc(n);
         q1.alloc(n);
         stamp.alloc(n);
-        hot.alloc(n);
-        hist.alloc(n);
-        score.alloc(n);
+        // activity per row: expansions of outgoing rows, records of any row
+        hot.alloc(nrows);
+        hist.alloc(nrows);
+        score.alloc(nrows);
         // per-row mutation arrays (outgoing and incoming rows)
         cnt64.alloc(nrows);
         fill64.alloc(nrows);
@@ -474,8 +476,9 @@ struct cig_engine {
         }
         ensure_tiles();
         fill32(stamp.p, n, 0);
-        fill32(hot.p, n, 0);
-        fill32(hist.p, n, 0);
+        fill32(hot.p, nrows, 0);
+        fill32(hist.p, nrows, 0);
+        fill32(score.p, nrows, 0);
         fill32(hcnt.p, nrows, 0);
         fill32(reinterpret_cast<uint32_t *>(cnt64.p), uint64_t(nrows) * 2, 0);
         fill32(reinterpret_cast<uint32_t *>(fill64.p), uint64_t(nrows) * 2, 0);
@@ -853,6 +856,7 @@ struct cig_engine {
         m.pools = pools();
         m.n = n;
         m.rev = rev ? 1u : 0u;
+        m.hot = (cfg.flags & CIG_CFG_NO_HOTNESS) || !budget_slots ? nullptr : hot.p;
         m.dels = d;
         m.nd = nd;
         m.adds = a;
@@ -1007,8 +1011,8 @@ struct cig_engine {
     // Moves the coldest HBM rows (lowest activity score, then shortest) to the
     // host tier until `slots` are freed; updates hrows and the device rows.
     void demote_for_space(std::vector<Row> &hrows, uint64_t slots) {
-        std::vector<uint32_t> sc(nrows, 0u);  // incoming rows score 0: they go first
-        HIPCHK(hipMemcpy(sc.data(), score.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> sc(nrows, 0u);
+        HIPCHK(hipMemcpy(sc.data(), score.p, size_t(nrows) * sizeof(uint32_t), hipMemcpyDeviceToHost));
         std::vector<uint32_t> cand;
         for (uint32_t v = 0; v < nrows; ++v)
             if (!(hrows[v].off & kHostTier) && hrows[v].cap) cand.push_back(v);
@@ -1787,49 +1791,79 @@ struct cig_engine {
     // framework.cuh:605-729): rows by score descending (stable: ties by id),
     // admit the longest prefix whose capacity fits 90% of the budget, evict
     // the rest to the host tier, then load admitted cold rows.
+    // compute_hot_vertices + confirm_candidate_batch + evict/compact/LoadCache
+    // (framework.cuh:605-769, 1873-1956) on the device: age the activity
+    // window, histogram the capacities by score, pick the threshold on the
+    // host from 1024 sums, list the rows that change tier, move them.
     void cache_refresh(cig_batch_stats *st) {
         check_ready();
         begin_call(st);
-        k_age_hotness<<<grid_small, 256, 0, stream>>>(hot.p, hist.p, score.p, n);
+        k_age_hotness<<<grid_small, 256, 0, stream>>>(hot.p, hist.p, score.p, nrows);
         HIPCHK(hipGetLastError());
+        uint64_t moved = 0;
         if (budget_slots) {
+            if (adm_hist.count < kScoreBuckets) {
+                adm_hist.alloc(kScoreBuckets);
+                adm_acc.alloc(4);
+            }
+            fill32(reinterpret_cast<uint32_t *>(adm_hist.p), 2 * kScoreBuckets, 0u);
+            fill32(reinterpret_cast<uint32_t *>(adm_acc.p), 8, 0u);
+            k_adm_hist<<<uint32_t(cus) * 2, 256, 0, stream>>>(rows.p, score.p, nrows, adm_hist.p);
+            HIPCHK(hipGetLastError());
+            std::vector<unsigned long long> h(kScoreBuckets);
+            HIPCHK(hipMemcpyAsync(h.data(), adm_hist.p, kScoreBuckets * 8, hipMemcpyDeviceToHost, stream));
             sync();
-            // incoming rows rank after every outgoing row (score 0, larger ids)
-            std::vector<uint32_t> sc(nrows, 0u);
-            std::vector<Row> hrows(nrows);
-            HIPCHK(hipMemcpy(sc.data(), score.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
-            std::vector<uint32_t> order(nrows);
-            std::iota(order.begin(), order.end(), 0u);
-            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sc[a] > sc[b]; });
             const uint64_t limit = budget_slots * 9 / 10;
-            std::vector<uint8_t> admit(nrows, 0);
-            uint64_t used = 0;
-            for (uint32_t v : order) {
-                if (!hrows[v].cap) continue;
-                if (used + hrows[v].cap > limit) break;
-                admit[v] = 1;
-                used += hrows[v].cap;
-            }
-            std::vector<uint32_t> evict, load;
-            uint64_t evict_need = 0, load_need = 0;
-            for (uint32_t v = 0; v < nrows; ++v) {
-                const bool cached = !(hrows[v].off & kHostTier);
-                if (!hrows[v].cap) continue;
-                if (cached && !admit[v]) { evict.push_back(v); evict_need += hrows[v].cap; }
-                if (!cached && admit[v]) { load.push_back(v); load_need += hrows[v].cap; }
-            }
-            move_rows(hrows, evict, 1, evict_need);
-            if (!load.empty()) {
-                read_ctl();
-                if (hctl.p->pool_tail[0] + load_need > hctl.p->pool_cap[0]) {
-                    compact_pool(0, load_need);
-                    HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
+            uint64_t above = 0;
+            uint32_t thr = 0;
+            for (int s = int(kScoreBuckets) - 1; s >= 0; --s) {
+                if (above + h[s] > limit || s == 0) {
+                    thr = uint32_t(s);
+                    break;
                 }
-                move_rows(hrows, load, 0, load_need);
+                above += h[s];
             }
+            AdmArgs a{};
+            a.rows = rows.p;
+            a.score = score.p;
+            a.nrows = nrows;
+            a.thr = thr;
+            a.room = limit - above;
+            a.acc = adm_acc.p;
+            a.evict = small_list.p;  // mutation scratch, free between calls
+            a.load = big_list.p;
+            a.evict_cnt = &ctl.p->small_cnt;
+            a.load_cnt = &ctl.p->big_cnt;
+            k_adm_mark<<<grid_small, 256, 0, stream>>>(a, 0);
+            k_adm_mark<<<grid_small, 256, 0, stream>>>(a, 1);
+            HIPCHK(hipGetLastError());
+            unsigned long long acc[4];
+            HIPCHK(hipMemcpyAsync(acc, adm_acc.p, sizeof(acc), hipMemcpyDeviceToHost, stream));
+            read_ctl();
+            if (hctl.p->small_cnt) move_rows_device(small_list.p, &ctl.p->small_cnt, hctl.p->small_cnt, 1, acc[1]);
+            if (hctl.p->big_cnt) move_rows_device(big_list.p, &ctl.p->big_cnt, hctl.p->big_cnt, 0, acc[2]);
+            moved = uint64_t(hctl.p->small_cnt) + hctl.p->big_cnt;
         }
         end_call(st, 0);
+        if (st) st->relocated_rows = moved;  // rows that changed tier
+    }
+
+    // Moves the `count` rows listed on the device into `tier` (`need` slots),
+    // compacting that tier first when its tail has no room.
+    void move_rows_device(const uint32_t *list, const uint32_t *dcount, uint32_t count, int tier, uint64_t need) {
+        read_ctl();
+        if (hctl.p->pool_tail[tier] + need > hctl.p->pool_cap[tier]) {
+            compact_pool(tier, need);
+            read_ctl();
+            if (hctl.p->pool_tail[tier] + need > hctl.p->pool_cap[tier])
+                throw HipError(CIG_ENOMEM, "tier pool cannot hold moved rows");
+        }
+        DBuf<CopyJob> jobs;
+        jobs.alloc(count);
+        k_move_plan<<<grid_small, 256, 0, stream>>>(rows.p, list, dcount, uint32_t(tier), ctl.p, jobs.p);
+        k_move_rows<<<grid_small, 256, 0, stream>>>(jobs.p, dcount, pools());
+        HIPCHK(hipGetLastError());
+        sync();  // the job list is freed on return
     }
 
     // ------------------------------------------------------------ results

@@ -133,6 +133,7 @@ struct MutArgs {
     struct DelTile *dtiles;  // long-row delete scan tiles (DelTile below)
     struct AddPiece *pieces;  // copy pieces of the insert phase (AddPiece below)
     StatShard *stats;
+    uint32_t *hot;     // per row activity of the current window (null: not counted)
 };
 
 // Deletes find their slot by row length (BuildDeletionPlan,
@@ -851,11 +852,18 @@ __global__ void k_group_match(GroupArgs g) {
 }
 
 // End of a call: clears the per-row counters of the touched rows and, when
-// `aff` is given, the affected bitmap words of the affected list.
+// `aff` is given, the affected bitmap words of the affected list. With
+// activity counting on, every touched row (outgoing or incoming) adds its
+// records of the call to its window count: a delete scans the row, so rows
+// mutated often are worth keeping in HBM like rows expanded often.
 __global__ void k_clear_touched(MutArgs a, uint32_t *aff, const uint32_t *alist) {
     const uint32_t touched = ld_acquire(&a.ctl->touched);
     for (uint32_t t = gthread(); t < touched; t += gthreads()) {
         const uint32_t u = a.touched[t];
+        if (a.hot) {
+            const uint64_t c = a.cnt64[u];
+            a.hot[u] += uint32_t(min(uint64_t(255), (c & 0xFFFFFFFFull) + (c >> 32)));
+        }
         a.cnt64[u] = 0;
         a.fill64[u] = 0;
     }
@@ -2154,6 +2162,104 @@ __global__ void k_move_rows(const CopyJob *jobs, const uint32_t *njobs, Pools po
         const uint32_t *s = pools.row(jb.from);
         uint32_t *d = pools.row(jb.to);
         for (uint32_t q = lane; q < jb.len; q += kWave) d[q] = s[q];
+    }
+}
+
+// ------------------------------------------------------- cache admission
+// Device form of compute_hot_vertices + confirm_candidate_batch
+// (framework.cuh:605-729): rows are admitted to HBM by decreasing activity
+// score until 90% of the budget is used. Scores are sums of four 8-bit
+// window counts (<= 1020), so a per-score capacity histogram replaces the
+// reference's sort: the host walks it from the top to find the threshold
+// score, rows above it are admitted, rows at it share what room is left.
+constexpr uint32_t kScoreBuckets = 1024;
+
+__global__ void k_adm_hist(const Row *rows, const uint32_t *score, uint32_t nrows, unsigned long long *hist) {
+    __shared__ unsigned long long h[kScoreBuckets];
+    for (uint32_t i = threadIdx.x; i < kScoreBuckets; i += blockDim.x) h[i] = 0ull;
+    __syncthreads();
+    for (uint32_t v = gthread(); v < nrows; v += gthreads()) {
+        const uint32_t cap = rows[v].cap;
+        if (cap) atomicAdd(&h[min(score[v], kScoreBuckets - 1)], (unsigned long long)cap);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kScoreBuckets; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+struct AdmArgs {
+    const Row *rows;
+    const uint32_t *score;
+    uint32_t nrows;
+    uint32_t thr;              // rows scoring above thr are admitted, below it not
+    unsigned long long room;   // slots for rows scoring exactly thr
+    unsigned long long *acc;   // [0] room used, [1] evict slots, [2] load slots
+    uint32_t *evict, *load;    // rows leaving / entering HBM
+    uint32_t *evict_cnt, *load_cnt;
+};
+
+// Lists the rows that change tier. Rows scoring exactly thr take the room
+// in two passes, rows already in HBM first (pass 0), so that a threshold
+// bucket that fits moves nothing; the room is reserved per workgroup step.
+__global__ void k_adm_mark(AdmArgs a, uint32_t pass) {
+    __shared__ unsigned long long s64[kMaxWavesPerBlock + 1];
+    __shared__ uint32_t s32[kMaxWavesPerBlock + 1];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < a.nrows; base += gthreads()) {
+        const uint32_t v = base + threadIdx.x;
+        Row r{};
+        uint32_t sc = 0;
+        if (v < a.nrows) {
+            r = a.rows[v];
+            sc = min(a.score[v], kScoreBuckets - 1);
+        }
+        const bool cached = r.cap && !(r.off & kHostTier);
+        const bool cand = r.cap && sc == a.thr && (pass == 0 ? cached : !cached);
+        const unsigned long long at =
+            block_reserve<unsigned long long>(cand ? (unsigned long long)r.cap : 0ull, &a.acc[0], s64);
+        bool decided = false, admit = false;
+        if (r.cap && sc != a.thr && pass == 0) {
+            decided = true;
+            admit = sc > a.thr;
+        } else if (cand) {
+            decided = true;
+            admit = at + r.cap <= a.room;
+        }
+        const bool ev = decided && cached && !admit, ld = decided && !cached && admit;
+        block_reserve<unsigned long long>(ev ? (unsigned long long)r.cap : 0ull, &a.acc[1], s64);
+        block_reserve<unsigned long long>(ld ? (unsigned long long)r.cap : 0ull, &a.acc[2], s64);
+        block_append(ev, v, a.evict, a.evict_cnt, s32);
+        block_append(ld, v, a.load, a.load_cnt, s32);
+    }
+}
+
+// New blocks in `tier` for the listed rows (bump allocation from the tier's
+// tail, one reservation per workgroup step), the copy jobs, the rows' new
+// offsets; the vacated blocks become garbage of the other tier.
+__global__ void k_move_plan(Row *rows, const uint32_t *list, const uint32_t *count, uint32_t tier, Ctl *c,
+                            CopyJob *jobs) {
+    __shared__ unsigned long long s64[kMaxWavesPerBlock + 1];
+    const uint32_t cnt = ld_acquire(count);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += gthreads()) {
+        const uint32_t i = base + threadIdx.x;
+        Row r{};
+        uint32_t v = 0;
+        if (i < cnt) {
+            v = list[i];
+            r = rows[v];
+        }
+        const unsigned long long at = block_reserve<unsigned long long>(
+            (unsigned long long)r.cap, reinterpret_cast<unsigned long long *>(&c->pool_tail[tier]), s64);
+        block_reserve<unsigned long long>((unsigned long long)r.cap,
+                                          reinterpret_cast<unsigned long long *>(&c->garbage[1 - tier]), s64);
+        if (i < cnt) {
+            CopyJob j;
+            j.from = r.off;
+            j.to = at | (tier ? kHostTier : 0ull);
+            j.len = r.deg;
+            j.pad = 0;
+            jobs[i] = j;
+            rows[v].off = j.to;
+        }
     }
 }
 

@@ -133,7 +133,7 @@ typedef struct cig_batch_stats {
     uint64_t missing_deletes; /* delete records with no matching edge */
     uint64_t invalid_updates; /* records with an endpoint >= n */
     uint64_t adds_applied;
-    uint64_t relocated_rows;  /* rows moved to a larger block */
+    uint64_t relocated_rows;  /* rows moved to a larger block (cig_cache_refresh: rows that changed tier) */
     uint64_t pool_compactions;
     uint64_t touched_sources;
     uint32_t converged;       /* PR: 1 if no active residual remains */

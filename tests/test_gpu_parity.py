@@ -185,6 +185,14 @@ def test_zero_copy_tier_same_results(budget_frac):
             check_traversal(e, og, "sssp", 0, where=f"zc batch {b}")
             e.cache_refresh()
             check_traversal(e, og, "sssp", 0, where=f"zc refresh {b}")
+            if budget_frac:
+                # admission by activity: no host-tier row outscores an HBM row
+                tier = e.gather(cg.FIELD_TIER).astype(bool)
+                score = e.gather(cg.FIELD_HOTNESS)
+                live = e.gather(cg.FIELD_DEGREE) > 0
+                hbm, host = score[live & tier], score[live & ~tier]
+                if hbm.size and host.size:
+                    assert host.max() <= hbm.min(), (b, int(host.max()), int(hbm.min()))
         check_topology(e, og, "zc final")
 
 
