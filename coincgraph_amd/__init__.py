@@ -194,6 +194,11 @@ def lib():
                                        C.POINTER(C.c_uint64), C.POINTER(P),
                                        C.POINTER(C.c_uint64), C.POINTER(P),
                                        C.POINTER(C.c_uint32)]),
+        "cig_write_csr_bin": (C.c_int, [C.c_char_p, C.c_uint32, P, P]),
+        "cig_read_csr_bin": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(P), C.POINTER(P)]),
+        "cig_write_updates_bin": (C.c_int, [C.c_char_p, P, C.c_uint64, P, C.c_uint64, P, C.c_uint32]),
+        "cig_read_updates_bin": (C.c_int, [C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64), C.POINTER(P),
+                                           C.POINTER(C.c_uint64), C.POINTER(P), C.POINTER(C.c_uint32)]),
         "cig_host_free": (None, [P]),
         "cig_edge_weight": (C.c_uint32, [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
         "cig_device_count": (C.c_int, []),
@@ -336,6 +341,43 @@ def read_market(path: str) -> HostGraph:
     _check(lib().cig_read_market(path.encode(), C.byref(n), C.byref(off), C.byref(dst)))
     offsets = _take(off, n.value + 1, np.uint64)
     return HostGraph(n.value, offsets, _take(dst, int(offsets[-1]), np.uint32))
+
+
+def write_csr_bin(g: HostGraph, path: str) -> None:
+    """Binary graph file ("CIGG", see cig.h)."""
+    _check(lib().cig_write_csr_bin(path.encode(), g.n, _ptr(g.offsets), _ptr(g.dst) if g.m else None))
+
+
+def read_csr_bin(path: str) -> HostGraph:
+    n = C.c_uint32()
+    off = C.c_void_p()
+    dst = C.c_void_p()
+    _check(lib().cig_read_csr_bin(path.encode(), C.byref(n), C.byref(off), C.byref(dst)))
+    offsets = _take(off, n.value + 1, np.uint64)
+    return HostGraph(n.value, offsets, _take(dst, int(offsets[-1]), np.uint32))
+
+
+def write_updates_bin(s: "UpdateStream", path: str) -> None:
+    """Binary update stream ("CIGU", see cig.h)."""
+    sizes = np.ascontiguousarray(s.batch_sizes, dtype=np.uint32)
+    adds = np.ascontiguousarray(s.adds, dtype=EDGE_DTYPE)
+    dels = np.ascontiguousarray(s.dels, dtype=EDGE_DTYPE)
+    _check(lib().cig_write_updates_bin(path.encode(), _ptr(adds) if adds.size else None, adds.size,
+                                       _ptr(dels) if dels.size else None, dels.size,
+                                       _ptr(sizes) if sizes.size else None, sizes.shape[0]))
+
+
+def read_updates_bin(path: str) -> "UpdateStream":
+    adds = C.c_void_p()
+    dels = C.c_void_p()
+    sizes = C.c_void_p()
+    na = C.c_uint64()
+    nd = C.c_uint64()
+    nb = C.c_uint32()
+    _check(lib().cig_read_updates_bin(path.encode(), C.byref(adds), C.byref(na), C.byref(dels), C.byref(nd),
+                                      C.byref(sizes), C.byref(nb)))
+    return UpdateStream(_take(adds, na.value, EDGE_DTYPE), _take(dels, nd.value, EDGE_DTYPE),
+                        _take(sizes, 2 * nb.value, np.uint32).reshape(nb.value, 2))
 
 
 def read_updates(update_path: str, batch_path: str) -> UpdateStream:

@@ -502,6 +502,137 @@ extern "C" cig_status cig_read_updates(const char *update_path, const char *batc
     return CIG_OK;
 }
 
+// ------------------------------------------------------------ binary formats
+// Little-endian, a 32-byte header then the arrays exactly as the C-ABI
+// holds them, so that reading is one sequential read per array:
+//   graph  "CIGG" v1: n (u32), m (u64), offsets[n+1] (u64), dst[m] (u32)
+//   stream "CIGU" v1: nbatches (u32), n_add (u64), n_del (u64),
+//                     sizes[2*nbatches] (u32: adds, dels per batch),
+//                     adds[n_add], dels[n_del] (cig_edge, 12 B)
+namespace {
+
+constexpr uint32_t kGraphMagic = 0x47474943u;   // "CIGG"
+constexpr uint32_t kStreamMagic = 0x55474943u;  // "CIGU"
+constexpr uint32_t kBinVersion = 1;
+
+struct BinHeader {
+    uint32_t magic, version, count32, pad;
+    uint64_t a, b;
+};
+static_assert(sizeof(BinHeader) == 32, "binary header layout");
+
+struct File {
+    FILE *f = nullptr;
+    explicit File(const char *path, const char *mode) : f(std::fopen(path, mode)) {}
+    ~File() {
+        if (f) std::fclose(f);
+    }
+    bool write(const void *p, size_t bytes) { return !bytes || std::fwrite(p, 1, bytes, f) == bytes; }
+    bool read(void *p, size_t bytes) { return !bytes || std::fread(p, 1, bytes, f) == bytes; }
+    bool at_end() { return std::fgetc(f) == EOF; }
+};
+
+template <class T>
+T *read_array(File &in, uint64_t count, bool &ok) {
+    T *p = static_cast<T *>(std::malloc(std::max<uint64_t>(1, count) * sizeof(T)));
+    if (!p) return nullptr;
+    ok = ok && in.read(p, count * sizeof(T));
+    return p;
+}
+
+}  // namespace
+
+extern "C" cig_status cig_write_csr_bin(const char *path, uint32_t n, const uint64_t *offsets, const uint32_t *dst) {
+    if (!path || !offsets || (offsets[n] && !dst)) return CIG_EINVAL;
+    File out(path, "wb");
+    if (!out.f) return CIG_EIO;
+    const BinHeader h{kGraphMagic, kBinVersion, n, 0, offsets[n], 0};
+    const bool ok = out.write(&h, sizeof(h)) && out.write(offsets, (uint64_t(n) + 1) * 8) &&
+                    out.write(dst, offsets[n] * 4);
+    return ok ? CIG_OK : CIG_EIO;
+}
+
+extern "C" cig_status cig_read_csr_bin(const char *path, uint32_t *n_out, uint64_t **offsets_out,
+                                       uint32_t **dst_out) {
+    if (!path || !n_out || !offsets_out || !dst_out) return CIG_EINVAL;
+    File in(path, "rb");
+    if (!in.f) return CIG_EIO;
+    BinHeader h{};
+    if (!in.read(&h, sizeof(h)) || h.magic != kGraphMagic || h.version != kBinVersion) return CIG_EIO;
+    bool ok = true;
+    uint64_t *off = read_array<uint64_t>(in, uint64_t(h.count32) + 1, ok);
+    if (!off) return CIG_ENOMEM;
+    ok = ok && off[0] == 0 && off[h.count32] == h.a;
+    for (uint32_t v = 0; ok && v < h.count32; ++v) ok = off[v] <= off[v + 1];
+    uint32_t *dst = ok ? read_array<uint32_t>(in, h.a, ok) : nullptr;
+    if (ok && !dst) {
+        std::free(off);
+        return CIG_ENOMEM;
+    }
+    if (!ok || !in.at_end()) {
+        std::free(off);
+        std::free(dst);
+        return CIG_EIO;
+    }
+    *n_out = h.count32;
+    *offsets_out = off;
+    *dst_out = dst;
+    return CIG_OK;
+}
+
+extern "C" cig_status cig_write_updates_bin(const char *path, const cig_edge *adds, uint64_t n_add,
+                                            const cig_edge *dels, uint64_t n_del, const uint32_t *batch_sizes,
+                                            uint32_t nbatches) {
+    if (!path || (n_add && !adds) || (n_del && !dels) || (nbatches && !batch_sizes)) return CIG_EINVAL;
+    File out(path, "wb");
+    if (!out.f) return CIG_EIO;
+    const BinHeader h{kStreamMagic, kBinVersion, nbatches, 0, n_add, n_del};
+    const bool ok = out.write(&h, sizeof(h)) && out.write(batch_sizes, uint64_t(nbatches) * 8) &&
+                    out.write(adds, n_add * sizeof(cig_edge)) && out.write(dels, n_del * sizeof(cig_edge));
+    return ok ? CIG_OK : CIG_EIO;
+}
+
+extern "C" cig_status cig_read_updates_bin(const char *path, cig_edge **adds_out, uint64_t *n_add_out,
+                                           cig_edge **dels_out, uint64_t *n_del_out, uint32_t **batch_sizes_out,
+                                           uint32_t *nbatches_out) {
+    if (!path || !adds_out || !n_add_out || !dels_out || !n_del_out || !batch_sizes_out || !nbatches_out)
+        return CIG_EINVAL;
+    File in(path, "rb");
+    if (!in.f) return CIG_EIO;
+    BinHeader h{};
+    if (!in.read(&h, sizeof(h)) || h.magic != kStreamMagic || h.version != kBinVersion) return CIG_EIO;
+    bool ok = true;
+    uint32_t *sizes = read_array<uint32_t>(in, uint64_t(h.count32) * 2, ok);
+    cig_edge *adds = read_array<cig_edge>(in, h.a, ok);
+    cig_edge *dels = read_array<cig_edge>(in, h.b, ok);
+    if (!sizes || !adds || !dels) {
+        std::free(sizes);
+        std::free(adds);
+        std::free(dels);
+        return CIG_ENOMEM;
+    }
+    uint64_t ta = 0, td = 0;
+    for (uint32_t b = 0; ok && b < h.count32; ++b) {
+        ta += sizes[2 * b];
+        td += sizes[2 * b + 1];
+    }
+    // the batch sizes may not ask for more records than the file holds
+    // (the text reader's rule, Loader.h:28-99)
+    if (!ok || ta > h.a || td > h.b || !in.at_end()) {
+        std::free(sizes);
+        std::free(adds);
+        std::free(dels);
+        return CIG_EIO;
+    }
+    *adds_out = adds;
+    *n_add_out = h.a;
+    *dels_out = dels;
+    *n_del_out = h.b;
+    *batch_sizes_out = sizes;
+    *nbatches_out = h.count32;
+    return CIG_OK;
+}
+
 extern "C" void cig_host_free(void *p) { std::free(p); }
 
 extern "C" uint32_t cig_edge_weight(uint32_t weight_mode, uint64_t seed, uint32_t u, uint32_t v) {

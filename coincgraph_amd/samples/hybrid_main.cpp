@@ -53,7 +53,8 @@ void def(const std::string &name, const std::string &v, const std::string &help,
 void define_flags() {
     // app_skeleton.h:42-90
     def("graphfile", "", "A file with a graph (market_big: 'src dst [w]' per line)");
-    def("format", "market_big", "graph format (market_big is supported)");
+    def("format", "market_big", "graph format: market_big (text) or cig_bin (binary graph; a binary "
+                                "update stream .cigu carries its own batch sizes)");
     def("updatefile", "", "Update records: 'a|d src dst w' per line");
     def("update_size", "", "Batch sizes: 'adds dels' per line");
     def("cache", "4", "GPU edge cache budget in GiB (reference table for 0..4)");
@@ -242,10 +243,13 @@ bool run() {
     const std::string U = upper();
     if (fb("interactive") || !fs("cmdfile").empty())
         throw std::invalid_argument("interactive and command-file sessions are not supported");
-    if (fs("format") != "market_big") throw std::invalid_argument("only --format=market_big is supported");
+    const bool binary = fs("format") == "cig_bin";
+    if (!binary && fs("format") != "market_big")
+        throw std::invalid_argument("--format must be market_big or cig_bin");
     if (fs("graphfile").empty()) throw std::invalid_argument(U + " requires --graphfile");
-    if (fs("updatefile").empty() != fs("update_size").empty())
-        throw std::invalid_argument("Provide both updatefile and update_size, or neither");
+    if (binary ? !fs("update_size").empty() : fs("updatefile").empty() != fs("update_size").empty())
+        throw std::invalid_argument(binary ? "a binary update stream carries its batch sizes: no update_size"
+                                           : "Provide both updatefile and update_size, or neither");
     const long long max_batches = fi(kApp + "_max_batches");
     if (max_batches < 0) throw std::invalid_argument(kApp + "_max_batches must be nonnegative");
     const long long segments = fi("SEGMENT");
@@ -273,7 +277,8 @@ bool run() {
     uint32_t n = 0;
     uint64_t *off = nullptr;
     uint32_t *dst = nullptr;
-    if (cig_read_market(fs("graphfile").c_str(), &n, &off, &dst) != CIG_OK)
+    if ((binary ? cig_read_csr_bin(fs("graphfile").c_str(), &n, &off, &dst)
+                : cig_read_market(fs("graphfile").c_str(), &n, &off, &dst)) != CIG_OK)
         throw std::runtime_error("cannot read graph file " + fs("graphfile"));
     if (n == 0) { cig_host_free(off); cig_host_free(dst); throw std::invalid_argument(U + " requires a nonempty graph"); }
     cig_engine *e = nullptr;
@@ -307,8 +312,9 @@ bool run() {
         cig_edge *adds = nullptr, *dels = nullptr;
         uint64_t na = 0, nd = 0;
         uint32_t *sizes = nullptr;
-        if (cig_read_updates(fs("updatefile").c_str(), fs("update_size").c_str(), &adds, &na, &dels, &nd, &sizes,
-                             &nbatches) != CIG_OK)
+        if ((binary ? cig_read_updates_bin(fs("updatefile").c_str(), &adds, &na, &dels, &nd, &sizes, &nbatches)
+                    : cig_read_updates(fs("updatefile").c_str(), fs("update_size").c_str(), &adds, &na, &dels, &nd,
+                                       &sizes, &nbatches)) != CIG_OK)
             throw std::runtime_error("cannot read update files");
         CALL(e, cig_load_updates(e, adds, na, dels, nd, sizes, nbatches));
         cig_host_free(adds);

@@ -373,6 +373,22 @@ cig_status cig_read_updates(const char *update_path, const char *batch_path,
                             cig_edge **adds_out, uint64_t *n_add_out,
                             cig_edge **dels_out, uint64_t *n_del_out,
                             uint32_t **batch_sizes_out, uint32_t *nbatches_out);
+/* Binary forms of the same data (no reference counterpart; the reference
+ * reads text only): a 32-byte little-endian header, then the arrays exactly
+ * as the C-ABI holds them. Graph "CIGG": n, m, offsets[n+1] (u64), dst[m]
+ * (u32). Stream "CIGU": nbatches, n_add, n_del, sizes[2*nbatches] (adds,
+ * dels per batch), adds[n_add], dels[n_del] (cig_edge). Readers check the
+ * header, monotone offsets, the batch sizes against the record counts and
+ * the file length (CIG_EIO otherwise); their arrays are malloc'd (free with
+ * cig_host_free). */
+cig_status cig_write_csr_bin(const char *path, uint32_t n, const uint64_t *offsets, const uint32_t *dst);
+cig_status cig_read_csr_bin(const char *path, uint32_t *n_out, uint64_t **offsets_out, uint32_t **dst_out);
+cig_status cig_write_updates_bin(const char *path, const cig_edge *adds, uint64_t n_add,
+                                 const cig_edge *dels, uint64_t n_del, const uint32_t *batch_sizes,
+                                 uint32_t nbatches);
+cig_status cig_read_updates_bin(const char *path, cig_edge **adds_out, uint64_t *n_add_out,
+                                cig_edge **dels_out, uint64_t *n_del_out, uint32_t **batch_sizes_out,
+                                uint32_t *nbatches_out);
 void cig_host_free(void *p);
 /* Edge weight of (u,v) under a mode (the function the kernels use). */
 uint32_t cig_edge_weight(uint32_t weight_mode, uint64_t seed, uint32_t u, uint32_t v);

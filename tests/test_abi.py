@@ -134,6 +134,54 @@ def test_readers_reject_bad_files(tmp_path):
         cg.read_updates(str(u), str(bs))
 
 
+def test_binary_formats_round_trip(tmp_path):
+    """Binary graph and stream files hold exactly what the text readers and
+    the generators produce; the golden README fixture survives text -> bin."""
+    g = cg.read_market(os.path.join(GOLDEN, "readme", "graph.txt"))
+    s = cg.read_updates(os.path.join(GOLDEN, "readme", "updates.txt"),
+                        os.path.join(GOLDEN, "readme", "batches.txt"))
+    big = cg.rmat_graph(10, 8, seed=21)
+    bs = cg.rmat_stream(big, 10, 3, 40, 50, seed=22)
+    for name, graph, stream in (("readme", g, s), ("rmat", big, bs)):
+        gp, sp = str(tmp_path / f"{name}.cigg"), str(tmp_path / f"{name}.cigu")
+        cg.write_csr_bin(graph, gp)
+        cg.write_updates_bin(stream, sp)
+        g2, s2 = cg.read_csr_bin(gp), cg.read_updates_bin(sp)
+        assert g2.n == graph.n
+        assert np.array_equal(g2.offsets, graph.offsets) and np.array_equal(g2.dst, graph.dst)
+        assert np.array_equal(s2.batch_sizes, stream.batch_sizes)
+        assert np.array_equal(s2.adds, stream.adds) and np.array_equal(s2.dels, stream.dels)
+        # header (32 B) + arrays, nothing else
+        assert os.path.getsize(gp) == 32 + 8 * (graph.n + 1) + 4 * graph.m
+        assert os.path.getsize(sp) == 32 + 8 * stream.nbatches + 12 * (len(stream.adds) + len(stream.dels))
+
+
+def test_binary_readers_reject_bad_files(tmp_path):
+    g = cg.rmat_graph(8, 4, seed=23)
+    s = cg.rmat_stream(g, 8, 2, 10, 10, seed=24)
+    gp, sp = tmp_path / "g.cigg", tmp_path / "s.cigu"
+    cg.write_csr_bin(g, str(gp))
+    cg.write_updates_bin(s, str(sp))
+    raw_g, raw_s = gp.read_bytes(), sp.read_bytes()
+    cases = {
+        "g_trunc.cigg": raw_g[:-4],
+        "g_extra.cigg": raw_g + b"\0",
+        "g_magic.cigg": b"XXXX" + raw_g[4:],
+        "s_trunc.cigu": raw_s[:-1],
+        "s_magic.cigu": raw_g,  # a graph file is not a stream
+    }
+    for name, data in cases.items():
+        (tmp_path / name).write_bytes(data)
+        with pytest.raises(cg.CigError):
+            (cg.read_csr_bin if name.endswith(".cigg") else cg.read_updates_bin)(str(tmp_path / name))
+    # batch sizes asking for more records than the file holds
+    big_sizes = bytearray(raw_s)
+    big_sizes[32:36] = (10**6).to_bytes(4, "little")
+    (tmp_path / "s_sizes.cigu").write_bytes(bytes(big_sizes))
+    with pytest.raises(cg.CigError):
+        cg.read_updates_bin(str(tmp_path / "s_sizes.cigu"))
+
+
 def test_oracle_golden_fixture_known_answers():
     """The oracle reproduces the hand-derived README fixture answers."""
     d = os.path.join(GOLDEN, "readme")

@@ -123,3 +123,34 @@ def test_sample_rmat_stream(app, tmp_path):
     exp = {"bfs": lambda: og.bfs(2), "sssp": lambda: og.sssp(2), "cc": og.cc}[app]()
     assert got[:, 1].astype(np.int64).tolist() == exp.astype(np.int64).tolist()
     assert r.stdout.count("BATCH-CHECK") == 4 and "failed" not in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("app", ["bfs", "sssp"])
+def test_sample_binary_inputs(app, tmp_path):
+    """--format=cig_bin reads the binary graph and update stream; the run's
+    results equal the oracle's on the same records."""
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+    g = cg.rmat_graph(11, 8, seed=33)
+    s = cg.rmat_stream(g, 11, 3, 200, 200, seed=34)
+    gp, up, out = tmp_path / "g.cigg", tmp_path / "u.cigu", tmp_path / "o.txt"
+    cg.write_csr_bin(g, str(gp))
+    cg.write_updates_bin(s, str(up))
+    r = _run(app, f"--graphfile={gp}", "--format=cig_bin", f"--updatefile={up}", "--check=true",
+             f"--output={out}", f"--{app}_max_batches=3", "--source_node=1")
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert "Overall: Test passed" in r.stdout
+    og = oracle_graph(g)
+    for b in range(s.nbatches):
+        oracle_apply(og, *s.batch(b))
+    got = np.loadtxt(out, ndmin=2)
+    exp = og.bfs(1) if app == "bfs" else og.sssp(1)
+    assert got[:, 1].astype(np.int64).tolist() == exp.astype(np.int64).tolist()
+
+
+def test_sample_binary_flag_rules(tmp_path):
+    r = _run("sssp", "--graphfile=x.cigg", "--format=cig_bin", "--updatefile=u.cigu", "--update_size=b.txt")
+    assert r.returncode != 0 and "carries its batch sizes" in (r.stdout + r.stderr)
+    r = _run("sssp", "--graphfile=x", "--format=csv")
+    assert r.returncode != 0 and "market_big or cig_bin" in (r.stdout + r.stderr)
