@@ -66,7 +66,8 @@ class Config(C.Structure):
         ("split_degree", C.c_uint32), ("max_rounds", C.c_uint32),
         ("pr_max_rounds", C.c_uint32), ("pr_epsilon", C.c_float),
         ("tile_len", C.c_uint32), ("round_blocks_per_cu", C.c_uint32),
-        ("pr_pull_divisor", C.c_uint32), ("pull_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 3),
+        ("pr_pull_divisor", C.c_uint32), ("pull_blocks_per_cu", C.c_uint32),
+        ("kernel_timing_every", C.c_uint32), ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -401,7 +402,7 @@ class Engine:
     def __init__(self, algo="sssp", weight_mode=WEIGHT_REF128, weight_seed=0, cache_bytes=0,
                  device=0, pr_epsilon=1e-6, pr_max_rounds=100, flags=0, edges_per_lane=0,
                  split_degree=0, max_rounds=0, n_stream=3, segments=1, tile_len=0,
-                 round_blocks_per_cu=0, pr_pull_divisor=0, pull_blocks_per_cu=0):
+                 round_blocks_per_cu=0, pr_pull_divisor=0, pull_blocks_per_cu=0, kernel_timing_every=0):
         L = lib()
         cfg = Config()
         _check(L.cig_config_default(C.byref(cfg)))
@@ -422,6 +423,7 @@ class Engine:
         cfg.round_blocks_per_cu = round_blocks_per_cu
         cfg.pr_pull_divisor = pr_pull_divisor
         cfg.pull_blocks_per_cu = pull_blocks_per_cu
+        cfg.kernel_timing_every = kernel_timing_every
         self.config = cfg
         self._e = C.c_void_p()
         st = L.cig_create(C.byref(cfg), C.byref(self._e))

@@ -354,6 +354,7 @@ struct cig_engine {
     // ------------------------------------------------------------ load
     void load(uint32_t nv, const uint64_t *off, const uint32_t *dst) {
         if (nv == 0) throw HipError(CIG_EINVAL, "graph must have at least one vertex");
+        finish_pending = false;  // the buffers it names are about to be replaced
         const unsigned T = pick_threads(0);
         std::atomic<int> bad{0};
         parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned) {
@@ -588,17 +589,21 @@ struct cig_engine {
         }
     }
 
-    uint64_t round_launches = 0;
+    uint64_t round_launches = 0;  // timed round launches of the current call
+    uint64_t calls = 0;           // calls begun (selects the timed ones)
+    bool timing_now = false;      // this call's round kernels are timed
 
     // Launches one propagation round kernel, bracketed by an event pair when
-    // kernel timing is on.
+    // kernel timing is on for this call.
     template <class F>
     void timed_round(F launch) {
-        ++round_launches;
-        const bool timed = !rev_begin.empty() && round_timed < kRoundEvents;
+        const bool timed = timing_now && round_timed < kRoundEvents;
         if (timed) HIPCHK(hipEventRecord(rev_begin[round_timed], stream));
         launch();
-        if (timed) HIPCHK(hipEventRecord(rev_end[round_timed++], stream));
+        if (timed) {
+            HIPCHK(hipEventRecord(rev_end[round_timed++], stream));
+            ++round_launches;
+        }
     }
 
     void advance_round(uint64_t by) {
@@ -782,9 +787,13 @@ struct cig_engine {
     // ------------------------------------------------------------ stats
     void begin_call(cig_batch_stats *st) {
         if (st) std::memset(st, 0, sizeof(*st));
+        flush_finish();
         scan_timed = false;
         round_timed = 0;
         round_launches = 0;
+        const uint32_t every = std::max<uint32_t>(1, cfg.kernel_timing_every);
+        timing_now = !rev_begin.empty() && calls % every == 0;
+        ++calls;
         reset_ctl(true);
         HIPCHK(hipEventRecord(ev[0], stream));
     }
@@ -905,6 +914,27 @@ struct cig_engine {
     void mutate_finish(const MutArgs &m, bool clear_affected = false) {
         k_clear_touched<<<grid_small, 256, 0, stream>>>(m, clear_affected ? aff.p : nullptr, alist.p);
         HIPCHK(hipGetLastError());
+    }
+
+    // A traversal batch ends with the host's read of the control block; its
+    // end-of-call clear (per-row counters, affected bits, activity counts) is
+    // enqueued at the start of the next call instead, ahead of that call's
+    // control reset, so that the stream does not idle through a host round
+    // trip. Everything that needs the counters zero begins with flush_finish.
+    bool finish_pending = false;
+    MutArgs pending_m{};
+    bool pending_aff = false;
+
+    void defer_finish(const MutArgs &m, bool clear_affected) {
+        pending_m = m;
+        pending_aff = clear_affected;
+        finish_pending = true;
+    }
+
+    void flush_finish() {
+        if (!finish_pending) return;
+        finish_pending = false;
+        mutate_finish(pending_m, pending_aff);
     }
 
     // Rows that found no room: reclaim garbage (compaction), demote to the
@@ -1216,7 +1246,7 @@ struct cig_engine {
             relax();
             end_call(st, 4);  // the control block still holds the whole call's totals
         }
-        mutate_finish(m, nd != 0);
+        defer_finish(m, nd != 0);
     }
 
     template <int W, bool S>
@@ -1235,6 +1265,7 @@ struct cig_engine {
     void insert_stage(const cig_edge *d_adds, uint32_t na) {
         // counters of the previous stage stay in the control block; only the
         // per-mutation lists are cleared
+        flush_finish();
         k_reset_mut<<<1, 64, 0, stream>>>(ctl.p);
         MutArgs m = mut_args(nullptr, 0, d_adds, na);
         mutate_group(m);

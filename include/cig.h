@@ -110,7 +110,10 @@ typedef struct cig_config {
                                  vertices gather along the incoming rows instead of scattering
                                  (no atomics); 0 = default (4096); >= n: every round */
     uint32_t pull_blocks_per_cu; /* workgroups per CU of the PR pull kernels; 0 = default (16) */
-    uint32_t reserved[3];
+    uint32_t kernel_timing_every; /* with CIG_CFG_KERNEL_TIMING: time the round kernels of one call
+                                     in this many (each event pair costs the stream ~10 us);
+                                     0 or 1 = every call */
+    uint32_t reserved[2];
 } cig_config;
 
 /* Per-call statistics. Times are device (HIP event) milliseconds of the
@@ -147,8 +150,9 @@ typedef struct cig_batch_stats {
     double scan_kernel_ms;    /* the main repair kernel alone (k_repair_pull, or k_repair_scan
                                  with CIG_CFG_SCAN_REPAIR), 0 if not run */
     double round_kernel_ms;   /* CIG_CFG_KERNEL_TIMING: summed durations of the propagation
-                                 round kernels (k_relax_round / k_pr_scatter) */
-    uint64_t round_launches;  /* propagation round kernels launched (incl. no-op lookahead) */
+                                 round kernels (k_relax_round / k_pr_pull or k_pr_scatter) */
+    uint64_t round_launches;  /* round kernels whose durations round_kernel_ms sums (incl. no-op
+                                 lookahead); 0 in a call that kernel_timing_every skips */
     uint64_t pull_rounds;     /* PR rounds run in pull form */
 } cig_batch_stats;
 

@@ -339,3 +339,28 @@ def test_golden_readme_example(algo):
             np.testing.assert_allclose(got, exp[:, 1], rtol=2e-6, atol=1e-6)
         else:
             assert got.tolist() == exp[:, 1].astype(np.int64).tolist()
+
+
+def test_sampled_round_timing():
+    """kernel_timing_every = 3 times the round kernels of one call in three;
+    the untimed calls report no launches, and every result stays exact."""
+    _announce("sampled round timing")
+    g = cg.rmat_graph(11, 16, seed=41)
+    s = cg.rmat_stream(g, 11, 9, 200, 200, seed=42)
+    og = oracle_graph(g)
+    with cg.Engine("sssp", flags=cg.CFG_KERNEL_TIMING, kernel_timing_every=3) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        timed = 0
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            st = e.apply_batch(dels, adds)
+            oracle_apply(og, dels, adds)
+            check_traversal(e, og, "sssp", 0, where=f"timed batch {b}")
+            if st["round_launches"]:
+                timed += 1
+                assert st["round_kernel_ms"] > 0
+            else:
+                assert st["round_kernel_ms"] == 0
+        assert timed == 3
