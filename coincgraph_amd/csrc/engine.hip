@@ -148,7 +148,8 @@ struct cig_engine {
     DBuf<unsigned long long> amask;  // active-source mask of the pull passes
     DBuf<unsigned long long> adm_hist, adm_acc;  // cache admission: capacity per score, sums
     uint64_t ptile_cap = 0;
-    DBuf<uint64_t> cnt64, fill64, gofs;
+    DBuf<uint64_t> cnt64, gofs;
+    DBuf<uint32_t> rslot;  // per record of a call: rank within its row's deletes / adds
     DBuf<uint32_t> touched, grouped, small_list, big_list, add_list, overflow;
     DBuf<AddPiece> pieces;              // insert-phase copy pieces
     DBuf<StatShard> stats;              // sharded statistics (see k_reduce_stats)
@@ -454,7 +455,6 @@ struct cig_engine {
         score.alloc(nrows);
         // per-row mutation arrays (outgoing and incoming rows)
         cnt64.alloc(nrows);
-        fill64.alloc(nrows);
         gofs.alloc(nrows);
         touched.alloc(nrows);
         small_list.alloc(nrows);
@@ -482,7 +482,6 @@ struct cig_engine {
         fill32(score.p, nrows, 0);
         fill32(hcnt.p, nrows, 0);
         fill32(reinterpret_cast<uint32_t *>(cnt64.p), uint64_t(nrows) * 2, 0);
-        fill32(reinterpret_cast<uint32_t *>(fill64.p), uint64_t(nrows) * 2, 0);
         if (!is_pr()) fill32(aff.p, (uint64_t(n) + 31) / 32, 0);
         // control block: pools
         std::memset(hctl.p, 0, sizeof(Ctl));
@@ -508,11 +507,17 @@ struct cig_engine {
         tile_cap = need;
     }
 
+    // Per-record arrays of a call: the grouped destinations and the ranks.
+    void ensure_grouped(uint64_t records) {
+        if (grouped.count < records) grouped.alloc(records);
+        if (rslot.count < records) rslot.alloc(records);
+    }
+
     void ensure_records(uint64_t nd, uint64_t na) {
         const uint64_t need = std::max(nd, na);
         if (rec_dels.count < need) rec_dels.alloc(need);
         if (rec_adds.count < need) rec_adds.alloc(need);
-        if (grouped.count < nd + na) grouped.alloc(nd + na);
+        ensure_grouped(nd + na);
     }
 
     // ------------------------------------------------------------ control
@@ -871,7 +876,7 @@ struct cig_engine {
         m.adds = a;
         m.na = na;
         m.cnt64 = cnt64.p;
-        m.fill64 = fill64.p;
+        m.rslot = rslot.p;
         m.gofs = gofs.p;
         m.touched = touched.p;
         m.grouped = grouped.p;
@@ -1418,7 +1423,7 @@ struct cig_engine {
                       cig_batch_stats *st) {
         const uint64_t mirror = rev ? 2 : 1;  // the incoming rows take every update once more
         if ((uint64_t(nd) + na) * mirror > 0xFFFFFFFFull) throw HipError(CIG_ELIMIT, "batch too large");
-        if (grouped.count < (uint64_t(nd) + na) * mirror) grouped.alloc((uint64_t(nd) + na) * mirror);
+        ensure_grouped((uint64_t(nd) + na) * mirror);
         ensure_delete_space(nd * mirror);
         ensure_pieces(nd * mirror, na * mirror);
         if (is_pr()) pr_batch(d, nd, a, na, st);
@@ -1444,7 +1449,7 @@ struct cig_engine {
         next_batch = 0;
         deletes_done = false;
         add_off = del_off = 0;
-        if (grouped.count < max_batch * (rev ? 2 : 1)) grouped.alloc(max_batch * (rev ? 2 : 1));
+        ensure_grouped(max_batch * (rev ? 2 : 1));
     }
 
     void apply_next(uint32_t flags, cig_batch_stats *st) {

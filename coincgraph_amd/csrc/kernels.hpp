@@ -115,7 +115,7 @@ struct MutArgs {
     const cig_edge *adds;
     uint32_t na;
     uint64_t *cnt64;   // per vertex: deletes (low) | adds (high) of this call
-    uint64_t *fill64;  // per vertex scatter cursors, same packing
+    uint32_t *rslot;   // per record: its rank among its row's deletes (or adds), from k_count
     uint64_t *gofs;    // per vertex offset of its group in `grouped`
     uint32_t *touched;
     uint32_t *grouped;
@@ -215,8 +215,10 @@ __device__ __forceinline__ MutRecord del_record(const MutArgs &a, uint32_t i) {
 }
 
 // GroupedUpdateBatch (effective_update_batch.h:35-87) on the device: count
-// records per row and collect the touched rows. A thread takes four records
-// a step, so the touched-list reservation is one atomic per 1024 records.
+// records per row and collect the touched rows. The count's atomic also
+// ranks the record among its row's deletes (or adds), which k_scatter uses
+// as its slot. A thread takes four records a step, so the touched-list
+// reservation is one atomic per 1024 records.
 __global__ void k_count(MutArgs a) {
     __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     constexpr uint32_t R = 4;
@@ -240,6 +242,7 @@ __global__ void k_count(MutArgs a) {
                     u[k] = r.row;
                     const uint64_t old = atomicAdd((unsigned long long *)&a.cnt64[r.row],
                                                    r.del ? 1ull : (1ull << 32));
+                    a.rslot[i] = r.del ? uint32_t(old) : uint32_t(old >> 32);
                     first[k] = old == 0;
                 }
             }
@@ -317,10 +320,8 @@ __global__ void k_scatter(MutArgs a) {
     for (uint32_t i = gthread(); i < total; i += gthreads()) {
         const MutRecord r = mut_record(a, i);
         if (!r.valid) continue;
-        const uint64_t pos = atomicAdd((unsigned long long *)&a.fill64[r.row],
-                                       r.del ? 1ull : (1ull << 32));
         const uint32_t nd = uint32_t(a.cnt64[r.row]);
-        const uint32_t slot = r.del ? uint32_t(pos) : nd + uint32_t(pos >> 32);
+        const uint32_t slot = r.del ? a.rslot[i] : nd + a.rslot[i];
         a.grouped[a.gofs[r.row] + slot] = r.dst;
         const uint32_t to = r.del ? a.htoff[r.row] : kNone;
         if (to != kNone) {
@@ -865,7 +866,6 @@ __global__ void k_clear_touched(MutArgs a, uint32_t *aff, const uint32_t *alist)
             a.hot[u] += uint32_t(min(uint64_t(255), (c & 0xFFFFFFFFull) + (c >> 32)));
         }
         a.cnt64[u] = 0;
-        a.fill64[u] = 0;
     }
     if (aff) {
         const uint32_t cnt = ld_acquire(&a.ctl->acount);
