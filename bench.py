@@ -262,10 +262,12 @@ def main(argv=None):
     updates = steps * 2 * batch * mult
     gteps = e_scan / wall / 1e9
 
-    stage_ms = {"propagate": sum(s["propagate_ms"] for s in stats),
-                "repair": sum(s["repair_ms"] for s in stats),
-                "mutation": sum(s["mutate_ms"] for s in stats),
-                "invalidation": sum(s["invalidate_ms"] for s in stats)}
+    # stage markers are recorded on the sampled batches only (kernel_timing_every)
+    staged = [s for s in stats if s["round_launches"]] or stats
+    stage_ms = {"propagate": sum(s["propagate_ms"] for s in staged) / len(staged),
+                "repair": sum(s["repair_ms"] for s in staged) / len(staged),
+                "mutation": sum(s["mutate_ms"] for s in staged) / len(staged),
+                "invalidation": sum(s["invalidate_ms"] for s in staged) / len(staged)}
     # Dominant kernel: the propagation round kernel (k_relax_round; for
     # PageRank k_pr_pull, the gather form of dense rounds, or k_pr_scatter
     # when no round gathered). Its duration comes from the HIP event pair the
@@ -334,8 +336,8 @@ def main(argv=None):
                        "mean": round(float(np.mean(lat)), 4), "max": round(max(lat), 4)},
         "updates_per_s": round(updates / wall, 1),
         "edges_scanned_per_batch": int(e_scan / steps),
-        "stage_ms_per_batch": {k: round(v / steps, 4) for k, v in stage_ms.items()},
-        "repair_kernel_ms_per_batch": round(sum(s["scan_kernel_ms"] for s in stats) / steps, 4),
+        "stage_ms_per_batch": {k: round(v, 4) for k, v in stage_ms.items()},
+        "repair_kernel_ms_per_batch": round(sum(s["scan_kernel_ms"] for s in staged) / len(staged), 4),
         "affected_per_batch": round(float(np.mean([s["affected"] for s in stats])), 1),
         "rounds_per_batch": round(float(np.mean([s["rounds"] for s in stats])), 1),
         **({"pull_rounds_per_batch": round(float(np.mean([s["pull_rounds"] for s in stats])), 1)} if pr else {}),

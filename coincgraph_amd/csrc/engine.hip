@@ -597,6 +597,14 @@ struct cig_engine {
     uint64_t round_launches = 0;  // timed round launches of the current call
     uint64_t calls = 0;           // calls begun (selects the timed ones)
     bool timing_now = false;      // this call's round kernels are timed
+    bool stages_now = true;       // this call records its stage markers
+
+    // Stage marker k (ev[k]) of a sampled call. Each event record costs the
+    // stream a few microseconds of idle, so calls that kernel_timing_every
+    // skips record only their begin/end pair.
+    void mark(int k) {
+        if (stages_now) HIPCHK(hipEventRecord(ev[k], stream));
+    }
 
     // Launches one propagation round kernel, bracketed by an event pair when
     // kernel timing is on for this call.
@@ -797,7 +805,8 @@ struct cig_engine {
         round_timed = 0;
         round_launches = 0;
         const uint32_t every = std::max<uint32_t>(1, cfg.kernel_timing_every);
-        timing_now = !rev_begin.empty() && calls % every == 0;
+        stages_now = calls % every == 0;
+        timing_now = !rev_begin.empty() && stages_now;
         ++calls;
         reset_ctl(true);
         HIPCHK(hipEventRecord(ev[0], stream));
@@ -827,7 +836,7 @@ struct cig_engine {
         if (!st) return;
         const Ctl &c = *hctl.p;
         st->total_ms = elapsed(0, 7);
-        if (marks >= 4) {  // ev: 0 start, 1 mutated, 2 invalidated, 3 repaired, 5 seeded, 4 propagated
+        if (marks >= 4 && stages_now) {  // ev: 0 start, 1 mutated, 2 invalidated, 3 repaired, 5 seeded, 4 propagated
             st->mutate_ms = elapsed(0, 1);
             st->invalidate_ms = elapsed(1, 2);
             st->repair_ms = elapsed(2, 3);
@@ -1149,10 +1158,10 @@ struct cig_engine {
         k_build_filter<<<grid_small, 256, 0, stream>>>(a);
         // 32 KB filter + 8 wave slots of LDS per workgroup: 3 workgroups per CU
         const uint32_t grid = uint32_t(cus) * 3;
-        HIPCHK(hipEventRecord(ev[8], stream));
+        mark(8);
         k_repair_scan<W, S, QPL><<<grid, kRepairBlock, 0, stream>>>(a);
-        HIPCHK(hipEventRecord(ev[9], stream));
-        scan_timed = true;
+        mark(9);
+        scan_timed = stages_now;
         k_repair_tiles<W, S, QPL><<<grid, kRepairBlock, 0, stream>>>(a);
         HIPCHK(hipGetLastError());
     }
@@ -1170,10 +1179,10 @@ struct cig_engine {
     void repair_pull_qpl(const InvalArgs &ia) {
         InvalArgs a = ia;
         a.cfg = ecfg(kStageRepair);
-        HIPCHK(hipEventRecord(ev[8], stream));
+        mark(8);
         k_repair_pull<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
-        HIPCHK(hipEventRecord(ev[9], stream));
-        scan_timed = true;
+        mark(9);
+        scan_timed = stages_now;
         k_repair_pull_tiles<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
         HIPCHK(hipGetLastError());
     }
@@ -1224,23 +1233,23 @@ struct cig_engine {
             // stage 1: deletes to their fixed point, then stage 2: adds
             delete_stage<W, S>(d_dels, nd);
             insert_stage<W, S>(d_adds, na);
-            HIPCHK(hipEventRecord(ev[4], stream));
+            mark(4);
             end_call(st, 0);
             return;
         }
         MutArgs m = mut_args(d_dels, nd, d_adds, na);
         mutate_group(m);
         mutate_apply(m);
-        HIPCHK(hipEventRecord(ev[1], stream));
+        mark(1);
         InvalArgs ia = inval_args(d_dels, nd, d_adds, na);
         if (nd) invalidate_and_reset(ia);
-        HIPCHK(hipEventRecord(ev[2], stream));
+        mark(2);
         if (nd) repair<W, S>(ia);
-        HIPCHK(hipEventRecord(ev[3], stream));
+        mark(3);
         if (na) insert_seed<W, S>(ia, nd != 0);
-        HIPCHK(hipEventRecord(ev[5], stream));
+        mark(5);
         relax();
-        HIPCHK(hipEventRecord(ev[4], stream));
+        mark(4);
         // One host read per batch: end_call's control-block read also tells
         // whether any insert found no room (then recover, re-seed those
         // inserts and converge again, and refresh the statistics).
@@ -1359,12 +1368,12 @@ struct cig_engine {
         mutate_apply(m);
         recover_overflow(m);
         pr_amend(+1.0f, pull);  // new rows, new degrees (the plan of the new rows stays for the solve)
-        HIPCHK(hipEventRecord(ev[1], stream));
-        HIPCHK(hipEventRecord(ev[2], stream));
-        HIPCHK(hipEventRecord(ev[3], stream));
-        HIPCHK(hipEventRecord(ev[5], stream));
+        mark(1);
+        mark(2);
+        mark(3);
+        mark(5);
         pr_converge(pull);
-        HIPCHK(hipEventRecord(ev[4], stream));
+        mark(4);
         pr_residual_norm();
         mutate_finish(m);
         end_call(st, 4);
@@ -1379,9 +1388,9 @@ struct cig_engine {
     void converge(cig_batch_stats *st) {
         check_ready();
         begin_call(st);
-        for (int k : {1, 2, 3, 5}) HIPCHK(hipEventRecord(ev[k], stream));
+        for (int k : {1, 2, 3, 5}) mark(k);
         if (is_pr()) pr_converge(); else relax();
-        HIPCHK(hipEventRecord(ev[4], stream));
+        mark(4);
         if (is_pr()) pr_residual_norm();
         end_call(st, 4);
     }
@@ -1769,14 +1778,14 @@ struct cig_engine {
         a.seed = cfg.weight_seed;
         k_ms_seed<<<1, 64, 0, stream>>>(a, ms_src.p, k);
         HIPCHK(hipGetLastError());
-        for (int m : {1, 2, 3, 5}) HIPCHK(hipEventRecord(ev[m], stream));
+        for (int m : {1, 2, 3, 5}) mark(m);
         const uint32_t wkind = algo == CIG_ALGO_BFS ? 2u : (cfg.weight_mode == CIG_WEIGHT_HASH64 ? 1u : 0u);
         switch (qpl) {
         case 1: ms_dispatch<1>(wkind, a); break;
         case 4: ms_dispatch<4>(wkind, a); break;
         default: ms_dispatch<2>(wkind, a); break;
         }
-        HIPCHK(hipEventRecord(ev[4], stream));
+        mark(4);
         k_ms_gather<<<grid_small, 256, 0, stream>>>(ms_dist.p, n, stride, k, ms_out.p);
         HIPCHK(hipGetLastError());
         uint64_t at = 0;

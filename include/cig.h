@@ -110,9 +110,10 @@ typedef struct cig_config {
                                  vertices gather along the incoming rows instead of scattering
                                  (no atomics); 0 = default (4096); >= n: every round */
     uint32_t pull_blocks_per_cu; /* workgroups per CU of the PR pull kernels; 0 = default (16) */
-    uint32_t kernel_timing_every; /* with CIG_CFG_KERNEL_TIMING: time the round kernels of one call
-                                     in this many (each event pair costs the stream ~10 us);
-                                     0 or 1 = every call */
+    uint32_t kernel_timing_every; /* time one call in this many: its stage markers (the *_ms stage
+                                     fields) and, with CIG_CFG_KERNEL_TIMING, its round kernels
+                                     (each event record idles the stream a few us); total_ms is
+                                     always timed; 0 or 1 = every call */
     uint32_t reserved[2];
 } cig_config;
 

@@ -358,9 +358,10 @@ def test_sampled_round_timing():
             st = e.apply_batch(dels, adds)
             oracle_apply(og, dels, adds)
             check_traversal(e, og, "sssp", 0, where=f"timed batch {b}")
+            assert st["total_ms"] > 0  # always timed
             if st["round_launches"]:
                 timed += 1
-                assert st["round_kernel_ms"] > 0
-            else:
-                assert st["round_kernel_ms"] == 0
+                assert st["round_kernel_ms"] > 0 and st["mutate_ms"] > 0 and st["propagate_ms"] > 0
+            else:  # the stage markers are skipped with the round timing
+                assert st["round_kernel_ms"] == 0 and st["mutate_ms"] == 0
         assert timed == 3
