@@ -150,7 +150,7 @@ struct cig_engine {
     uint64_t ptile_cap = 0;
     DBuf<uint64_t> cnt64, gofs;
     DBuf<uint32_t> rslot;  // per record of a call: rank within its row's deletes / adds
-    DBuf<uint32_t> touched, grouped, small_list, big_list, add_list, overflow;
+    DBuf<uint32_t> touched, grouped, small_list, mid_list, big_list, add_list, overflow;
     DBuf<AddPiece> pieces;              // insert-phase copy pieces
     DBuf<StatShard> stats;              // sharded statistics (see k_reduce_stats)
     DBuf<uint32_t> htoff, hkeys, hcnt;  // long-row delete tables
@@ -458,6 +458,7 @@ struct cig_engine {
         gofs.alloc(nrows);
         touched.alloc(nrows);
         small_list.alloc(nrows);
+        mid_list.alloc(nrows);
         big_list.alloc(nrows);
         add_list.alloc(nrows);
         overflow.alloc(nrows);
@@ -890,6 +891,7 @@ struct cig_engine {
         m.touched = touched.p;
         m.grouped = grouped.p;
         m.small_list = small_list.p;
+        m.mid_list = mid_list.p;
         m.big_list = big_list.p;
         m.add_list = add_list.p;
         m.overflow = overflow.p;
@@ -915,7 +917,7 @@ struct cig_engine {
     void mutate_apply(const MutArgs &m) {
         if (m.nd) {
             k_del_lane<<<grid_small, 256, 0, stream>>>(m);
-            k_del_wave<<<grid_small, kBlock, 0, stream>>>(m);
+            k_del_rows<<<grid_small, kBlock, 0, stream>>>(m);
             k_del_scan<<<grid_small, 256, 0, stream>>>(m);
             k_del_fix_rows<<<grid_small, 256, 0, stream>>>(m);
             k_del_fix<<<grid_small, 256, 0, stream>>>(m);

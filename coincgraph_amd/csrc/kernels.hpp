@@ -42,6 +42,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
     c->scount = 0;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
+    c->mid_cnt = 0;
     c->dtile_cnt = c->ht_cursor = 0;
     c->pr_invalid = 0;
     c->group_cursor = 0;
@@ -64,6 +65,7 @@ __global__ void k_reset_mut(Ctl *c) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
+    c->mid_cnt = 0;
     c->dtile_cnt = c->ht_cursor = 0;
     c->group_cursor = 0;
 }
@@ -120,6 +122,7 @@ struct MutArgs {
     uint32_t *touched;
     uint32_t *grouped;
     uint32_t *small_list;
+    uint32_t *mid_list;   // medium rows with deletes (k_del_rows)
     uint32_t *big_list;
     uint32_t *add_list;
     uint32_t *overflow;
@@ -140,8 +143,8 @@ struct MutArgs {
 // source_local_chunk_store.h:670-723, restated for the device):
 //  - tiny rows (<= 64 slots): one lane per delete record reads the whole row
 //    and claims a matching slot with a CAS (slot := kNone);
-//  - medium rows (<= 4096 slots, <= kDelWaveMax deletes): one wave per delete
-//    record, 1024 slots a step, same claim;
+//  - medium rows (<= 4096 slots, <= kDelWaveMax deletes): one wave per row
+//    against all of the row's deletes at once, 1024 slots a step;
 //  - other rows: tiles of kDelTile slots over the whole grid against the
 //    row's delete multiset in an open-addressing table of 2^bits >= 2*nd
 //    slots.
@@ -261,13 +264,13 @@ __global__ void k_count(MutArgs a) {
 }
 
 // Reserves each touched row's group and classifies its work; long rows
-// with deletes also reserve their table and scan tiles. All six cursors are
+// with deletes also reserve their table and scan tiles. All seven cursors are
 // reserved together, one round trip per workgroup step.
 __global__ void k_plan(MutArgs a) {
-    __shared__ uint32_t scratch[6 * (kMaxWavesPerBlock + 1)];
+    __shared__ uint32_t scratch[7 * (kMaxWavesPerBlock + 1)];
     const uint32_t touched = ld_acquire(&a.ctl->touched);
-    uint32_t *const ctr[6] = {&a.ctl->group_cursor, &a.ctl->ht_cursor, &a.ctl->dtile_cnt,
-                              &a.ctl->small_cnt, &a.ctl->big_cnt, &a.ctl->add_cnt};
+    uint32_t *const ctr[7] = {&a.ctl->group_cursor, &a.ctl->ht_cursor, &a.ctl->dtile_cnt,
+                              &a.ctl->small_cnt, &a.ctl->big_cnt, &a.ctl->add_cnt, &a.ctl->mid_cnt};
     for (uint32_t base = blockIdx.x * blockDim.x; base < touched; base += gthreads()) {
         const uint32_t t = base + threadIdx.x;
         uint32_t u = 0, nd = 0, na = 0, deg = 0;
@@ -283,11 +286,12 @@ __global__ void k_plan(MutArgs a) {
         }
         const bool table = cls == kDelClassLong;
         const bool thread_fix = nd != 0 && nd <= kDelThreadFix && !table;
-        const uint32_t x[6] = {nd + na, table ? (1u << del_table_bits(nd)) : 0u,
+        const uint32_t x[7] = {nd + na, table ? (1u << del_table_bits(nd)) : 0u,
                                table ? (deg + kDelTile - 1) / kDelTile : 0u, thread_fix ? 1u : 0u,
-                               (nd != 0 && !thread_fix) ? 1u : 0u, na != 0 ? 1u : 0u};
-        uint32_t at[6];
-        block_reserve_n<6>(x, ctr, at, scratch);
+                               (nd != 0 && !thread_fix) ? 1u : 0u, na != 0 ? 1u : 0u,
+                               cls == kDelClassMedium ? 1u : 0u};
+        uint32_t at[7];
+        block_reserve_n<7>(x, ctr, at, scratch);
         if (t < touched) {
             a.gofs[u] = at[0];
             a.htoff[u] = table ? at[1] : kNone;
@@ -309,6 +313,7 @@ __global__ void k_plan(MutArgs a) {
             if (x[3]) a.small_list[at[3]] = u;
             if (x[4]) a.big_list[at[4]] = u;
             if (x[5]) a.add_list[at[5]] = u;
+            if (x[6]) a.mid_list[at[6]] = u;
         }
     }
 }
@@ -371,61 +376,89 @@ __global__ void k_del_lane(MutArgs a) {
     }
 }
 
-// Medium rows: one 16-lane group per delete record (four records per wave),
-// 256 slots a step, stopping at the first slot the record claims.
-__global__ __launch_bounds__(kBlock) void k_del_wave(MutArgs a) {
+// Medium rows (64 < deg <= 4096 slots, at most kDelWaveMax deletes): one
+// 16-lane group per row (four rows per wave) reads the row once, 256 slots
+// a step, against all of the row's requested destinations, which the
+// group's lanes hold in registers (a destination requested twice is two
+// requests). The group owns the row, so a slot equal to a destination with
+// an open request is claimed with a plain store and its index goes to the
+// row's hole list; the group stops as soon as every request is claimed. (A
+// group per delete record would read the row once per request.)
+__global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
     constexpr uint32_t G = 16, GPW = kWave / G;
-    const uint32_t total = a.nd << a.rev;
+    static_assert(kDelWaveMax <= G, "a group holds its row's requests");
+    const uint32_t cnt = ld_acquire(&a.ctl->mid_cnt);
     const uint32_t lane = lane_id(), g = lane / G, gl = lane % G;
     const uint64_t gmask = ((1ull << G) - 1ull) << (g * G);
-    for (uint32_t base = gwave() * GPW; base < total; base += gwaves() * GPW) {
+    for (uint32_t base = gwave() * GPW; base < cnt; base += gwaves() * GPW) {
         const uint32_t i = base + g;
-        bool active = false;
-        MutRecord r{};
-        Row row{};
-        uint32_t *p = nullptr;
-        uint32_t nq = 0;
-        if (i < total) {
-            r = del_record(a, i);
-            active = r.valid && a.dclass[r.row] == kDelClassMedium;
-        }
-        if (active) {
-            row = a.rows[r.row];
+        uint32_t u = 0, nd = 0, nq = 0, deg = 0;
+        uint32_t *grp = nullptr, *p = nullptr;
+        if (i < cnt) {
+            u = a.mid_list[i];
+            nd = min(uint32_t(a.cnt64[u]), kDelWaveMax);
+            const Row row = a.rows[u];
+            deg = row.deg;
+            nq = (deg + 3u) >> 2;
+            grp = a.grouped + a.gofs[u];  // the requests, then the hole list
             p = a.pools.row(row.off);
-            nq = (row.deg + 3u) >> 2;
         }
-        for (uint32_t q0 = 0; __ballot(active && q0 < nq) != 0; q0 += G * 4) {
-            uint32_t m = 0;
-            if (active && q0 < nq) {
-                uint32_t d[4][4], live[4];
+        const uint32_t mine = gl < nd ? grp[gl] : kNone;
+        uint32_t key[kDelWaveMax];
+#pragma unroll
+        for (uint32_t j = 0; j < kDelWaveMax; ++j) key[j] = __shfl(mine, int(g * G + j), kWave);
+        uint32_t open = (1u << nd) - 1u;  // uniform within the group
+        uint32_t holes = 0;               // uniform within the group
+        for (uint32_t q0 = 0; __ballot(open != 0u && q0 < nq) != 0; q0 += G * 4) {
+            uint32_t d[4][4], cand = 0;  // bit 4k+s: slot (k, s) equals an open request
+            if (open != 0u && q0 < nq) {
 #pragma unroll
                 for (uint32_t k = 0; k < 4; ++k) {
                     const uint32_t q = q0 + k * G + gl;
-                    live[k] = load_quad(p, q < nq ? q : nq - 1, row.deg, d[k]);
-                    if (q >= nq) live[k] = 0;
+                    uint32_t live = load_quad(p, q < nq ? q : nq - 1, deg, d[k]);
+                    if (q >= nq) live = 0;
 #pragma unroll
-                    for (uint32_t s = 0; s < 4; ++s)
-                        if (s < live[k] && d[k][s] == r.dst) m |= 1u << (4 * k + s);
+                    for (uint32_t s = 0; s < 4; ++s) {
+                        bool hit = false;
+#pragma unroll
+                        for (uint32_t j = 0; j < kDelWaveMax; ++j)
+                            hit |= ((open >> j) & 1u) && key[j] == d[k][s];
+                        if (s < live && hit) cand |= 1u << (4 * k + s);
+                    }
                 }
             }
+            // each group claims its candidates one at a time, in lane order
             for (;;) {
-                const uint64_t any = __ballot(m != 0u);
+                const uint64_t any = __ballot(cand != 0u);
                 if (any == 0) break;
-                const uint64_t mine = any & gmask;  // this group's candidate lanes
-                uint32_t ok = 0;
-                if (mine && lane == uint32_t(__ffsll((unsigned long long)mine) - 1)) {
-                    const uint32_t b = uint32_t(__ffs(m) - 1);
-                    const uint32_t slot = 4u * (q0 + (b >> 2) * G + gl) + (b & 3u);
-                    ok = atomicCAS(&p[slot], r.dst, kNone) == r.dst ? 1u : 0u;
-                    if (ok) del_hole(a, r.row, slot);
-                    m &= m - 1u;
+                const uint64_t mineg = any & gmask;
+                const uint32_t leader = mineg ? uint32_t(__ffsll((unsigned long long)mineg) - 1) : g * G;
+                uint32_t claimed = kNone, slot = 0;
+                if (mineg && lane == leader) {
+                    const uint32_t b = uint32_t(__ffs(cand) - 1);
+                    cand &= cand - 1u;
+                    const uint32_t x = d[b >> 2][b & 3u];
+                    uint32_t m = 0;
+#pragma unroll
+                    for (uint32_t j = 0; j < kDelWaveMax; ++j) m |= (key[j] == x ? 1u : 0u) << j;
+                    m &= open;
+                    if (m) {
+                        claimed = uint32_t(__ffs(m) - 1);
+                        slot = 4u * (q0 + (b >> 2) * G + gl) + (b & 3u);
+                        p[slot] = kNone;
+                    }
                 }
-                if (__ballot(ok != 0u) & gmask) {  // the record found its slot
-                    active = false;
-                    m = 0;
+                claimed = __shfl(claimed, int(leader), kWave);
+                slot = __shfl(slot, int(leader), kWave);
+                if (claimed != kNone) {
+                    open &= ~(1u << claimed);
+                    if (gl == 0) grp[holes] = slot;  // the requests were read into registers above
+                    ++holes;
                 }
+                if (open == 0u) cand = 0;
             }
         }
+        if (i < cnt && gl == 0) a.hcnt[u] = holes;  // zero before the call; this group is its only writer
     }
 }
 
