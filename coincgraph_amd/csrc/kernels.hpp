@@ -364,12 +364,16 @@ __global__ void k_del_lane(MutArgs a) {
             live[q] = load_quad(p, q < nq ? q : nq - 1, row.deg, d[q]);
             if (q >= nq) live[q] = 0;
         }
+        // a row with one delete record has one claimant: a plain store
+        // (for host-tier rows a CAS is an atomic over the host link)
+        const bool sole = uint32_t(a.cnt64[r.row]) == 1u;
         bool done = false;
 #pragma unroll
         for (uint32_t q = 0; q < kDelTinyDeg / 4; ++q)
 #pragma unroll
             for (uint32_t s = 0; s < 4; ++s)
-                if (!done && s < live[q] && d[q][s] == r.dst && atomicCAS(&p[4 * q + s], r.dst, kNone) == r.dst) {
+                if (!done && s < live[q] && d[q][s] == r.dst &&
+                    (sole ? (p[4 * q + s] = kNone, true) : atomicCAS(&p[4 * q + s], r.dst, kNone) == r.dst)) {
                     del_hole(a, r.row, 4 * q + s);
                     done = true;
                 }
