@@ -479,14 +479,30 @@ __global__ void k_del_fix_rows(MutArgs a) {
             uint32_t *row = a.pools.row(r.off);
             const uint32_t *hl = a.grouped + a.gofs[u];
             const uint32_t newdeg = r.deg - k;
-            // as many holes lie below newdeg as live slots lie at or above it
-            uint32_t hi = 0;
-            for (uint32_t p = newdeg; p < r.deg; ++p) {
-                const uint32_t v = row[p];
-                if (v == kNone) continue;
-                while (hi < k && hl[hi] >= newdeg) ++hi;
-                if (hi == k) break;  // unreachable while the hole list is exact
-                row[hl[hi++]] = v;
+            // k <= kDelThreadFix: the k tail slots and the k holes are loaded
+            // together first (the stores below go to holes under newdeg, never
+            // to the tail), so a host-tier row costs one host-link round trip
+            uint32_t tv[kDelThreadFix], hv[kDelThreadFix];
+#pragma unroll
+            for (uint32_t j = 0; j < kDelThreadFix; ++j) {
+                tv[j] = j < k ? row[newdeg + j] : kNone;
+                hv[j] = j < k ? hl[j] : kNone;
+            }
+            // as many holes lie below newdeg as live slots lie at or above
+            // it: the j-th live tail slot fills the j-th hole below newdeg
+            uint32_t rt = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kDelThreadFix; ++j) {
+                const bool live = tv[j] != kNone;
+                uint32_t target = kNone, rh = 0;
+#pragma unroll
+                for (uint32_t i = 0; i < kDelThreadFix; ++i) {
+                    const bool below = hv[i] < newdeg;
+                    if (below && rh == rt) target = hv[i];
+                    rh += below ? 1u : 0u;
+                }
+                if (live && target != kNone) row[target] = tv[j];
+                rt += live ? 1u : 0u;
             }
             a.rows[u].deg = newdeg;
             a.hcnt[u] = 0;
