@@ -917,7 +917,8 @@ struct cig_engine {
     void mutate_apply(const MutArgs &m) {
         if (m.nd) {
             k_del_lane<<<grid_small, 256, 0, stream>>>(m);
-            k_del_rows<<<grid_small, kBlock, 0, stream>>>(m);
+            k_del_rows<4, false><<<grid_small, kBlock, 0, stream>>>(m);
+            k_del_rows<2, true><<<grid_small, kBlock, 0, stream>>>(m);
             k_del_scan<<<grid_small, 256, 0, stream>>>(m);
             k_del_fix_rows<<<grid_small, 256, 0, stream>>>(m);
             k_del_fix<<<grid_small, 256, 0, stream>>>(m);

@@ -42,7 +42,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
     c->scount = 0;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
-    c->mid_cnt = 0;
+    c->mid_cnt = c->mid_host_cnt = 0;
     c->dtile_cnt = c->ht_cursor = 0;
     c->pr_invalid = 0;
     c->group_cursor = 0;
@@ -65,7 +65,7 @@ __global__ void k_reset_mut(Ctl *c) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
-    c->mid_cnt = 0;
+    c->mid_cnt = c->mid_host_cnt = 0;
     c->dtile_cnt = c->ht_cursor = 0;
     c->group_cursor = 0;
 }
@@ -122,7 +122,8 @@ struct MutArgs {
     uint32_t *touched;
     uint32_t *grouped;
     uint32_t *small_list;
-    uint32_t *mid_list;   // medium rows with deletes (k_del_rows)
+    uint32_t *mid_list;   // medium rows with deletes (k_del_rows): HBM-tier rows from the
+                          // front, host-tier rows from the back (n << rev entries)
     uint32_t *big_list;
     uint32_t *add_list;
     uint32_t *overflow;
@@ -267,31 +268,36 @@ __global__ void k_count(MutArgs a) {
 // with deletes also reserve their table and scan tiles. All seven cursors are
 // reserved together, one round trip per workgroup step.
 __global__ void k_plan(MutArgs a) {
-    __shared__ uint32_t scratch[7 * (kMaxWavesPerBlock + 1)];
+    __shared__ uint32_t scratch[8 * (kMaxWavesPerBlock + 1)];
     const uint32_t touched = ld_acquire(&a.ctl->touched);
-    uint32_t *const ctr[7] = {&a.ctl->group_cursor, &a.ctl->ht_cursor, &a.ctl->dtile_cnt,
-                              &a.ctl->small_cnt, &a.ctl->big_cnt, &a.ctl->add_cnt, &a.ctl->mid_cnt};
+    uint32_t *const ctr[8] = {&a.ctl->group_cursor, &a.ctl->ht_cursor, &a.ctl->dtile_cnt,
+                              &a.ctl->small_cnt,    &a.ctl->big_cnt,    &a.ctl->add_cnt,
+                              &a.ctl->mid_cnt,      &a.ctl->mid_host_cnt};
     for (uint32_t base = blockIdx.x * blockDim.x; base < touched; base += gthreads()) {
         const uint32_t t = base + threadIdx.x;
         uint32_t u = 0, nd = 0, na = 0, deg = 0;
+        bool host = false;
         uint8_t cls = kDelClassNone;
         if (t < touched) {
             u = a.touched[t];
             const uint64_t c = a.cnt64[u];
             nd = uint32_t(c);
             na = uint32_t(c >> 32);
-            deg = a.rows[u].deg;
+            const Row row = a.rows[u];
+            deg = row.deg;
+            host = (row.off & kHostTier) != 0;
             cls = del_class(nd, deg);
             a.dclass[u] = cls;
         }
         const bool table = cls == kDelClassLong;
         const bool thread_fix = nd != 0 && nd <= kDelThreadFix && !table;
-        const uint32_t x[7] = {nd + na, table ? (1u << del_table_bits(nd)) : 0u,
+        const uint32_t x[8] = {nd + na, table ? (1u << del_table_bits(nd)) : 0u,
                                table ? (deg + kDelTile - 1) / kDelTile : 0u, thread_fix ? 1u : 0u,
                                (nd != 0 && !thread_fix) ? 1u : 0u, na != 0 ? 1u : 0u,
-                               cls == kDelClassMedium ? 1u : 0u};
-        uint32_t at[7];
-        block_reserve_n<7>(x, ctr, at, scratch);
+                               cls == kDelClassMedium && !host ? 1u : 0u,
+                               cls == kDelClassMedium && host ? 1u : 0u};
+        uint32_t at[8];
+        block_reserve_n<8>(x, ctr, at, scratch);
         if (t < touched) {
             a.gofs[u] = at[0];
             a.htoff[u] = table ? at[1] : kNone;
@@ -314,6 +320,7 @@ __global__ void k_plan(MutArgs a) {
             if (x[4]) a.big_list[at[4]] = u;
             if (x[5]) a.add_list[at[5]] = u;
             if (x[6]) a.mid_list[at[6]] = u;
+            if (x[7]) a.mid_list[(a.n << a.rev) - 1u - at[7]] = u;
         }
     }
 }
@@ -387,11 +394,16 @@ __global__ void k_del_lane(MutArgs a) {
 // requests). The group owns the row, so a slot equal to a destination with
 // an open request is claimed with a plain store and its index goes to the
 // row's hole list; the group stops as soon as every request is claimed. (A
-// group per delete record would read the row once per request.)
+// group per delete record would read the row once per request.) Q quads
+// per lane a step: 4 for HBM rows; host-tier rows take 2 (HOST), since
+// what a step reads past the last claim costs host-link bandwidth, which
+// bounds them.
+template <uint32_t Q, bool HOST>
 __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
     constexpr uint32_t G = 16, GPW = kWave / G;
     static_assert(kDelWaveMax <= G, "a group holds its row's requests");
-    const uint32_t cnt = ld_acquire(&a.ctl->mid_cnt);
+    static_assert(4 * Q <= 32, "candidate bits fit one word");
+    const uint32_t cnt = ld_acquire(HOST ? &a.ctl->mid_host_cnt : &a.ctl->mid_cnt);
     const uint32_t lane = lane_id(), g = lane / G, gl = lane % G;
     const uint64_t gmask = ((1ull << G) - 1ull) << (g * G);
     for (uint32_t base = gwave() * GPW; base < cnt; base += gwaves() * GPW) {
@@ -399,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
         uint32_t u = 0, nd = 0, nq = 0, deg = 0;
         uint32_t *grp = nullptr, *p = nullptr;
         if (i < cnt) {
-            u = a.mid_list[i];
+            u = a.mid_list[HOST ? (a.n << a.rev) - 1u - i : i];
             nd = min(uint32_t(a.cnt64[u]), kDelWaveMax);
             const Row row = a.rows[u];
             deg = row.deg;
@@ -413,11 +425,11 @@ __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
         for (uint32_t j = 0; j < kDelWaveMax; ++j) key[j] = __shfl(mine, int(g * G + j), kWave);
         uint32_t open = (1u << nd) - 1u;  // uniform within the group
         uint32_t holes = 0;               // uniform within the group
-        for (uint32_t q0 = 0; __ballot(open != 0u && q0 < nq) != 0; q0 += G * 4) {
-            uint32_t d[4][4], cand = 0;  // bit 4k+s: slot (k, s) equals an open request
+        for (uint32_t q0 = 0; __ballot(open != 0u && q0 < nq) != 0; q0 += G * Q) {
+            uint32_t d[Q][4], cand = 0;  // bit 4k+s: slot (k, s) equals an open request
             if (open != 0u && q0 < nq) {
 #pragma unroll
-                for (uint32_t k = 0; k < 4; ++k) {
+                for (uint32_t k = 0; k < Q; ++k) {
                     const uint32_t q = q0 + k * G + gl;
                     uint32_t live = load_quad(p, q < nq ? q : nq - 1, deg, d[k]);
                     if (q >= nq) live = 0;
