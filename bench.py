@@ -208,6 +208,8 @@ def main(argv=None):
     # every batch would inflate the latency it reports
     kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=args.weight_seed, flags=cg.CFG_KERNEL_TIMING,
               kernel_timing_every=TIMING_EVERY)
+    if c["sym"]:
+        kw["flags"] |= cg.CFG_SYMMETRIC  # undirected graph: outgoing rows double as incoming rows
     if pr:
         kw.update(pr_epsilon=1e-7, pr_max_rounds=1000)
         if args.pr_push:

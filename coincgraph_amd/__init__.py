@@ -38,6 +38,7 @@ CFG_COOPERATIVE, CFG_NO_HOTNESS = 1, 2
 CFG_SCAN_REPAIR = 1 << 2
 CFG_KERNEL_TIMING = 1 << 3
 CFG_PR_PUSH_ONLY = 1 << 4
+CFG_SYMMETRIC = 1 << 5
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1

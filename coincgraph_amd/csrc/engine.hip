@@ -132,6 +132,7 @@ struct cig_engine {
     uint32_t n = 0;
     uint32_t nrows = 0;  // n, or 2n with the incoming rows [n, 2n)
     bool rev = false;    // incoming-row index kept (pull repair)
+    bool sym = false;    // symmetric graph: the outgoing rows are the incoming rows
     bool loaded = false, initialized = false;
     uint32_t source = 0;
 
@@ -352,6 +353,26 @@ struct cig_engine {
 
     uint32_t round_bound() const { return uint32_t(std::min<uint64_t>(2ull * n + 64, 0x7FFFFFFFull)); }
 
+    // Incoming rows for the pull repair and PageRank's gathers: rows [n, 2n),
+    // or the outgoing rows themselves when the graph is symmetric.
+    bool pull_rows() const { return rev || sym; }
+    const Row *in_rows_p() const { return rev ? rows.p + n : rows.p; }
+
+    // CIG_CFG_SYMMETRIC: every vertex's in-degree must equal its out-degree
+    // (necessary for a symmetric multiset of edges; the caller asserts the rest).
+    static void check_symmetric_degrees(uint32_t nv, const uint64_t *off, const uint32_t *dst, unsigned T) {
+        std::vector<uint32_t> indeg(nv, 0u);
+        parallel_for(off[nv], T, [&](uint64_t lo, uint64_t hi, unsigned) {
+            for (uint64_t e = lo; e < hi; ++e) __atomic_fetch_add(&indeg[dst[e]], 1u, __ATOMIC_RELAXED);
+        });
+        std::atomic<int> bad{0};
+        parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+            for (uint64_t v = lo; v < hi; ++v)
+                if (uint64_t(indeg[v]) != off[v + 1] - off[v]) bad = 1;
+        });
+        if (bad) throw HipError(CIG_EINVAL, "CIG_CFG_SYMMETRIC: a vertex's in-degree differs from its out-degree");
+    }
+
     // ------------------------------------------------------------ load
     void load(uint32_t nv, const uint64_t *off, const uint32_t *dst) {
         if (nv == 0) throw HipError(CIG_EINVAL, "graph must have at least one vertex");
@@ -373,7 +394,10 @@ struct cig_engine {
         budget_slots = cfg.cache_bytes / sizeof(uint32_t);
         // incoming rows (the device reverse index) for deletion repair
         // (PR: the pull rounds' gather rows; the cooperative driver only scatters)
-        rev = !(cfg.flags & CIG_CFG_SCAN_REPAIR) && !(is_pr() && ((cfg.flags & CIG_CFG_PR_PUSH_ONLY) || coop));
+        const bool pull = !(cfg.flags & CIG_CFG_SCAN_REPAIR) && !(is_pr() && ((cfg.flags & CIG_CFG_PR_PUSH_ONLY) || coop));
+        sym = pull && (cfg.flags & CIG_CFG_SYMMETRIC);
+        rev = pull && !sym;
+        if (sym) check_symmetric_degrees(nv, off, dst, T);
         if (rev && n > 0x7FFFFFFFu) throw HipError(CIG_ELIMIT, "incoming index needs n < 2^31 (use CIG_CFG_SCAN_REPAIR)");
         nrows = rev ? 2 * n : n;
         std::vector<uint64_t> ioff;
@@ -703,10 +727,10 @@ struct cig_engine {
         a.max_rounds = max_rounds;
         a.n = n;
         a.pull_min = kNone;
-        if (rev && !coop) {
+        if (pull_rows() && !coop) {
             const uint32_t div = cfg.pr_pull_divisor ? cfg.pr_pull_divisor : kPullDivisorDefault;
             a.pull_min = std::max<uint32_t>(1u, n / div);
-            a.in_rows = rows.p + n;
+            a.in_rows = in_rows_p();
             a.ptiles = ptiles.p;
             a.prow = prow.p;
             a.tpart = tpart.p;
@@ -735,7 +759,7 @@ struct cig_engine {
     // plan_ready: the batch amendment already built the pull plan of the
     // current incoming rows.
     void pr_converge(bool plan_ready = false) {
-        if (rev && !coop) ensure_pull();
+        if (pull_rows() && !coop) ensure_pull();
         PrArgs a = pr_args(cfg.pr_max_rounds);
         if (coop) {
             auto kernel = k_pr_loop<1>;
@@ -1130,6 +1154,7 @@ struct cig_engine {
         InvalArgs x{};
         x.ctl = ctl.p;
         x.rows = rows.p;
+        x.in_rows = in_rows_p();
         x.state = state.p;
         x.stamp = stamp.p;
         x.aff = aff.p;
@@ -1194,7 +1219,7 @@ struct cig_engine {
     // or (without the incoming index) a push scan over every row.
     template <int W, bool S>
     void repair(const InvalArgs &ia) {
-        if (!rev) return repair_scan<W, S>(ia);
+        if (!pull_rows()) return repair_scan<W, S>(ia);
         switch (qpl) {
         case 1: repair_pull_qpl<W, S, 1>(ia); break;
         case 4: repair_pull_qpl<W, S, 4>(ia); break;
@@ -1359,12 +1384,13 @@ struct cig_engine {
         begin_call(st);
         MutArgs m = mut_args(d_dels, nd, d_adds, na);
         mutate_group(m);
-        const bool pull = rev && !coop;
+        const bool pull = pull_rows() && !coop;
         if (pull) {
             ensure_pull();
             const PrArgs a = pr_args(cfg.pr_max_rounds);
             // touched counts outgoing and incoming rows: twice the sources
-            const uint32_t min_touched = a.pull_min > 0x7FFFFFFFu ? kNone : 2u * a.pull_min;
+            // (a symmetric graph has no separate incoming rows)
+            const uint32_t min_touched = a.pull_min > 0x7FFFFFFFu ? kNone : (rev ? 2u : 1u) * a.pull_min;
             k_pr_amend_form<<<1, 64, 0, stream>>>(ctl.p, min_touched);
         }
         pr_amend(-1.0f, pull);  // old rows, old degrees
@@ -1952,10 +1978,10 @@ struct cig_engine {
     // incoming index (first = n).
     void gather_topology(uint64_t *offs, uint32_t *dst, uint64_t capacity, uint32_t first = 0) {
         if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
-        if (first && !rev) throw HipError(CIG_ESTATE, "this engine keeps no incoming-row index");
+        if (first && !pull_rows()) throw HipError(CIG_ESTATE, "this engine keeps no incoming-row index");
         sync();
         std::vector<Row> hrows(n);
-        HIPCHK(hipMemcpy(hrows.data(), rows.p + first, n * sizeof(Row), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hrows.data(), first ? in_rows_p() : rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
         read_ctl();
         const uint64_t used0 = hctl.p->pool_tail[0];
         std::vector<uint32_t> hp(std::min<uint64_t>(used0, hbm_slots));

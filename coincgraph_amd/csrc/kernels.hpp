@@ -942,6 +942,7 @@ __global__ void k_clear_touched(MutArgs a, uint32_t *aff, const uint32_t *alist)
 struct InvalArgs {
     Ctl *ctl;
     const Row *rows;
+    const Row *in_rows;  // incoming rows: rows + n, or rows itself for a symmetric graph
     uint64_t *state;
     uint32_t *stamp;
     uint32_t *aff;
@@ -1207,7 +1208,7 @@ __global__ __launch_bounds__(kRepairBlock) void k_repair_tiles(InvalArgs a) {
 // repair is identical.
 template <int W, bool kStrict>
 struct PullOp {
-    const Row *in_rows;  // rows + n
+    const Row *in_rows;  // incoming rows
     uint64_t *state;
     const uint32_t *aff;
     uint64_t seed;
@@ -1237,7 +1238,7 @@ __global__ __launch_bounds__(kBlock) void k_repair_pull(InvalArgs a) {
     Ctl *c = a.ctl;
     const uint32_t na = ld_acquire(&c->acount);
     if (na == 0) return;
-    PullOp<W, kStrict> op{a.rows + a.n, a.state, a.aff, a.seed};
+    PullOp<W, kStrict> op{a.in_rows, a.state, a.aff, a.seed};
     Counters ctr;
     const StagedOut<2, 128> out{Sink{nullptr, &c->scount}, TileSink{a.t0, &c->scount}, &stage};
     out_init(out);
@@ -1251,7 +1252,7 @@ __global__ __launch_bounds__(kBlock) void k_repair_pull_tiles(InvalArgs a) {
     Ctl *c = a.ctl;
     const uint32_t nt = ld_acquire(&c->scount);
     if (nt == 0) return;
-    PullOp<W, kStrict> op{a.rows + a.n, a.state, a.aff, a.seed};
+    PullOp<W, kStrict> op{a.in_rows, a.state, a.aff, a.seed};
     expand_tiles<QPL>(op, a.cfg, a.t0, nt, DirectOut{Sink{nullptr, &c->scount}, TileSink{nullptr, &c->scount}});
 }
 

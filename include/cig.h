@@ -79,8 +79,14 @@ enum {
                                       every round (half the edge memory) */
     CIG_CFG_KERNEL_TIMING = 1u << 3, /* bracket every propagation round kernel with
                                         HIP events (cig_batch_stats.round_kernel_ms) */
-    CIG_CFG_PR_PUSH_ONLY = 1u << 4 /* PR: keep no incoming-row index; every round
+    CIG_CFG_PR_PUSH_ONLY = 1u << 4, /* PR: keep no incoming-row index; every round
                                       scatters with atomics (the reference's form) */
+    CIG_CFG_SYMMETRIC = 1u << 5    /* the caller asserts an undirected graph: every edge (u,v)
+                                      has its reverse (v,u) as often, and every batch keeps it so
+                                      (the samples' CC input). The outgoing rows then serve as
+                                      the incoming rows: no mirrored updates, half the mutation
+                                      work and edge memory. cig_load_csr rejects a graph whose
+                                      in- and out-degrees differ. */
 };
 
 /* Engine configuration. Flags of the reference samples map as:

@@ -76,6 +76,61 @@ def test_incremental_batches(algo, wmode, staged, repair):
             check_incoming(e, og, "final")
 
 
+@pytest.mark.parametrize("algo,staged", [("cc", False), ("cc", True), ("sssp", False), ("bfs", False)])
+def test_symmetric_mode(algo, staged):
+    """CFG_SYMMETRIC on an undirected graph with symmetric batches: the
+    outgoing rows serve as the incoming rows (no mirrored updates), and the
+    state, topology and incoming rows equal the default engine's oracle."""
+    _announce(f"symmetric {algo} staged={staged}")
+    scale = 12
+    g = cg.rmat_graph(scale, 16, seed=21, symmetric=True)
+    s = cg.rmat_stream(g, scale, 6, 400, 400, seed=22, symmetric=True)
+    og = oracle_graph(g)
+    with cg.Engine(algo, weight_mode=cg.WEIGHT_HASH64, weight_seed=7, flags=cg.CFG_SYMMETRIC) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            st = e.apply_batch(dels, adds, staged=staged)
+            miss, inval = oracle_apply(og, dels, adds)
+            assert st["missing_deletes"] == miss and st["invalid_updates"] == inval
+            assert st["deletes_applied"] == len(dels) - miss
+            check_traversal(e, og, algo, 0, cg.WEIGHT_HASH64, 7, where=f"batch {b}")
+        check_topology(e, og, "final")
+        check_incoming(e, og, "final")
+
+
+def test_symmetric_mode_pagerank():
+    """PageRank on an undirected graph with CFG_SYMMETRIC gathers along the
+    outgoing rows; ranks match the FP64 oracle."""
+    _announce("symmetric pagerank")
+    eps = 1e-7
+    g = cg.rmat_graph(12, 16, seed=23, symmetric=True)
+    s = cg.rmat_stream(g, 12, 3, 300, 300, seed=24, symmetric=True)
+    og = oracle_graph(g)
+    with cg.Engine("pr", pr_epsilon=eps, pr_max_rounds=2000, flags=cg.CFG_SYMMETRIC) as e:
+        e.load_graph(g)
+        e.init_graph()
+        st = e.start()
+        assert st["pull_rounds"] > 0
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            st = e.apply_batch(dels, adds)
+            oracle_apply(og, dels, adds)
+            assert st["converged"] == 1
+            ref, _ = og.pagerank()
+            l1n, maxerr = pr_errors(e.values(), ref)
+            assert l1n <= 1e-6, f"batch {b}: normalised L1 {l1n:.3e} (max {maxerr:.3e})"
+
+
+def test_symmetric_mode_rejects_directed_graph():
+    g = cg.rmat_graph(10, 8, seed=25, symmetric=False)
+    with cg.Engine("cc", flags=cg.CFG_SYMMETRIC) as e:
+        with pytest.raises(cg.CigError):
+            e.load_graph(g)
+
+
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
 def test_delete_then_insert_calls(algo):
     """del_edge and add_edge as separate calls (the reference's two stages,
