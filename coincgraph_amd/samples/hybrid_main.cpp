@@ -100,6 +100,7 @@ void define_flags() {
     def("weight_seed", "0", "seed of the hash64 weights");
     def("fused", "false", "apply each batch as one fused repair (no delete-stage check)", true);
     def("cache_refresh", "true", "run the hotness cache maintenance after each batch", true);
+    def("symmetric", "false", "undirected graph and batches: outgoing rows double as incoming rows", true);
 }
 
 bool parse_flags(int argc, char **argv) {
@@ -264,6 +265,7 @@ bool run() {
     cfg.n_stream = uint32_t(fi("n_stream"));
     cfg.segments = uint32_t(segments);
     cfg.block_size = uint32_t(fi("block_size"));
+    if (fb("symmetric")) cfg.flags |= CIG_CFG_SYMMETRIC;
     if (kApp == "pr") {
         const double err = fd("error");
         const long long rounds = fi("pr_max_rounds");

@@ -92,17 +92,19 @@ def test_sample_readme_fixture(app, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pr"])
+@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pr", "cc-sym"])
 def test_sample_rmat_stream(app, tmp_path):
     if cg.device_count() < 1:
         pytest.skip("no HIP device")
+    extra = ["--symmetric=true"] if app.endswith("-sym") else []  # undirected-graph mode
+    app = app.split("-")[0]
     g = cg.rmat_graph(11, 8, seed=31, symmetric=app == "cc")
     s = cg.rmat_stream(g, 11, 4, 150, 150, seed=32, symmetric=app == "cc")
     gp, up, bp, out = (tmp_path / x for x in ("g.txt", "u.txt", "b.txt", "o.txt"))
     _write_graph(gp, g)
     _write_updates(up, bp, s)
     args = [f"--graphfile={gp}", f"--updatefile={up}", f"--update_size={bp}", "--check=true",
-            f"--output={out}", f"--{app}_max_batches=10", "--source_node=2"]
+            f"--output={out}", f"--{app}_max_batches=10", "--source_node=2"] + extra
     if app == "pr":
         # the sample keeps the reference's 1..100 round limit (hybrid_pr.cu:129-132);
         # 1e-6 converges within it on this graph
