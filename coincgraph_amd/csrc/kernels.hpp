@@ -435,10 +435,10 @@ __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
                     if (q >= nq) live = 0;
 #pragma unroll
                     for (uint32_t s = 0; s < 4; ++s) {
+                        // claimed and unused requests hold kNone, which no live slot holds
                         bool hit = false;
 #pragma unroll
-                        for (uint32_t j = 0; j < kDelWaveMax; ++j)
-                            hit |= ((open >> j) & 1u) && key[j] == d[k][s];
+                        for (uint32_t j = 0; j < kDelWaveMax; ++j) hit |= key[j] == d[k][s];
                         if (s < live && hit) cand |= 1u << (4 * k + s);
                     }
                 }
@@ -468,6 +468,9 @@ __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
                 slot = __shfl(slot, int(leader), kWave);
                 if (claimed != kNone) {
                     open &= ~(1u << claimed);
+#pragma unroll
+                    for (uint32_t j = 0; j < kDelWaveMax; ++j)
+                        if (j == claimed) key[j] = kNone;
                     if (gl == 0) grp[holes] = slot;  // the requests were read into registers above
                     ++holes;
                 }
