@@ -553,8 +553,15 @@ __device__ __forceinline__ uint32_t block_flag_scan(bool f, uint32_t *wsum, uint
 // destination is in the row's table claims one requested occurrence (first
 // claims win). The tile's row loads are issued before the table is staged
 // in LDS (tables of more than kDelLdsKeys slots are probed in place).
+// A 2048-bit filter of the staged table's keys: a slot whose bit is clear
+// matches no request and skips the probe (most slots of a long row match
+// nothing, and a wave runs each probe loop as long as its longest chain).
+constexpr uint32_t kDelBloomWords = 64;
+__device__ __forceinline__ uint32_t del_bloom_bit(uint32_t x) { return (x * 0x85EBCA6Bu) >> 21; }
+
 __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
     __shared__ uint32_t keys[kDelLdsKeys];
+    __shared__ uint32_t bloom[kDelBloomWords];
     const uint32_t nt = ld_acquire(&a.ctl->dtile_cnt);
     const uint32_t tid = threadIdx.x;
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
@@ -570,10 +577,22 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
             if (q >= nq) live[k] = 0;
         }
         const bool in_lds = mask < kDelLdsKeys;
-        __syncthreads();  // the previous tile is done with `keys`
-        if (in_lds)
+        __syncthreads();  // the previous tile is done with `keys` and `bloom`
+        if (in_lds) {
             for (uint32_t k = tid; k <= mask; k += blockDim.x) keys[k] = a.hkeys[to + k];
+            if (tid < kDelBloomWords) bloom[tid] = 0u;
+        }
         __syncthreads();
+        if (in_lds) {
+            for (uint32_t k = tid; k <= mask; k += blockDim.x) {
+                const uint32_t key = keys[k];
+                if (key != 0u) {
+                    const uint32_t b = del_bloom_bit(key - 1u);
+                    atomicOr(&bloom[b >> 5], 1u << (b & 31u));
+                }
+            }
+            __syncthreads();
+        }
         const uint32_t *gkeys = a.hkeys + to;
 #pragma unroll
         for (uint32_t k = 0; k < 4; ++k) {
@@ -581,6 +600,10 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
             for (uint32_t s = 0; s < 4; ++s) {
                 if (s >= live[k]) continue;
                 const uint32_t x = d[k][s];
+                if (in_lds) {
+                    const uint32_t b = del_bloom_bit(x);
+                    if (((bloom[b >> 5] >> (b & 31u)) & 1u) == 0u) continue;
+                }
                 uint32_t h = ht_hash(x, bits), hit = kNone;
                 if (in_lds) {
                     for (uint32_t probe = 0; probe <= mask; ++probe, h = (h + 1u) & mask) {
