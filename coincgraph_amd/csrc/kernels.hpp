@@ -553,11 +553,11 @@ __device__ __forceinline__ uint32_t block_flag_scan(bool f, uint32_t *wsum, uint
 // destination is in the row's table claims one requested occurrence (first
 // claims win). The tile's row loads are issued before the table is staged
 // in LDS (tables of more than kDelLdsKeys slots are probed in place).
-// A 2048-bit filter of the staged table's keys: a slot whose bit is clear
+// An 8192-bit filter of the staged table's keys: a slot whose bit is clear
 // matches no request and skips the probe (most slots of a long row match
 // nothing, and a wave runs each probe loop as long as its longest chain).
-constexpr uint32_t kDelBloomWords = 64;
-__device__ __forceinline__ uint32_t del_bloom_bit(uint32_t x) { return (x * 0x85EBCA6Bu) >> 21; }
+constexpr uint32_t kDelBloomWords = 256;
+__device__ __forceinline__ uint32_t del_bloom_bit(uint32_t x) { return (x * 0x85EBCA6Bu) >> 19; }
 
 __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
     __shared__ uint32_t keys[kDelLdsKeys];
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
         __syncthreads();  // the previous tile is done with `keys` and `bloom`
         if (in_lds) {
             for (uint32_t k = tid; k <= mask; k += blockDim.x) keys[k] = a.hkeys[to + k];
-            if (tid < kDelBloomWords) bloom[tid] = 0u;
+            for (uint32_t k = tid; k < kDelBloomWords; k += blockDim.x) bloom[k] = 0u;
         }
         __syncthreads();
         if (in_lds) {
