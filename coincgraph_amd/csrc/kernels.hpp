@@ -561,10 +561,15 @@ __device__ __forceinline__ uint32_t del_bloom_bit(uint32_t x) { return (x * 0x85
 
 __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
     __shared__ uint32_t keys[kDelLdsKeys];
-    __shared__ uint32_t bloom[kDelBloomWords];
+    // two filters, alternating by tile: a tile fills one while it clears the
+    // other for the next tile, so staging needs no barrier of its own
+    __shared__ uint32_t blooms[2][kDelBloomWords];
     const uint32_t nt = ld_acquire(&a.ctl->dtile_cnt);
     const uint32_t tid = threadIdx.x;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    for (uint32_t k = tid; k < kDelBloomWords; k += blockDim.x) blooms[0][k] = 0u;
+    uint32_t it = 0;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x, ++it) {
+        uint32_t *bloom = blooms[it & 1u], *next = blooms[(it + 1u) & 1u];
         const DelTile dt = a.dtiles[t];
         const uint32_t bits = dt.bits, mask = (1u << bits) - 1u, to = dt.to, len = dt.len;
         uint32_t *row = a.pools.row(dt.off);     // the tile start is a multiple of 4
@@ -577,22 +582,19 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
             if (q >= nq) live[k] = 0;
         }
         const bool in_lds = mask < kDelLdsKeys;
-        __syncthreads();  // the previous tile is done with `keys` and `bloom`
-        if (in_lds) {
-            for (uint32_t k = tid; k <= mask; k += blockDim.x) keys[k] = a.hkeys[to + k];
-            for (uint32_t k = tid; k < kDelBloomWords; k += blockDim.x) bloom[k] = 0u;
-        }
-        __syncthreads();
+        __syncthreads();  // the previous tile is done with `keys` and with `next`
+        for (uint32_t k = tid; k < kDelBloomWords; k += blockDim.x) next[k] = 0u;
         if (in_lds) {
             for (uint32_t k = tid; k <= mask; k += blockDim.x) {
-                const uint32_t key = keys[k];
+                const uint32_t key = a.hkeys[to + k];
+                keys[k] = key;
                 if (key != 0u) {
                     const uint32_t b = del_bloom_bit(key - 1u);
-                    atomicOr(&bloom[b >> 5], 1u << (b & 31u));
+                    atomicOr(&bloom[b >> 5], 1u << (b & 31u));  // cleared by the previous tile
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
         const uint32_t *gkeys = a.hkeys + to;
 #pragma unroll
         for (uint32_t k = 0; k < 4; ++k) {
