@@ -12,9 +12,11 @@ library or GPU raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import re
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -141,6 +143,25 @@ class GraphInfo(C.Structure):
 
 
 _lib = None
+# Engines and servers still open. They are closed by an interpreter exit hook,
+# which runs before the C runtime's exit handlers (and the HIP runtime's
+# teardown among them): no HIP call is left to a finaliser that runs after
+# the runtime is gone, and a server thread never outlives its engine.
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    objs = list(_LIVE)
+    for o in objs:
+        if isinstance(o, Server):
+            try:
+                o.stop()
+            except Exception:
+                pass
+    for o in objs:
+        if isinstance(o, Engine):
+            o.close()
 
 
 def header_functions(path: str = HEADER_PATH) -> list:
@@ -432,8 +453,14 @@ class Engine:
             raise CigError(st, L.cig_last_error(None).decode())
         self.algo = cfg.algo
         self.n = 0
+        self._server = None
+        _LIVE.add(self)
 
     def close(self):
+        srv = getattr(self, "_server", None)
+        srv = srv() if srv is not None else None
+        if srv is not None:
+            srv.stop()  # the server thread owns the engine until it stops
         if getattr(self, "_e", None) and self._e.value:
             lib().cig_destroy(self._e)
             self._e = C.c_void_p()
@@ -600,7 +627,9 @@ class Server:
         self._s = C.c_void_p()
         self._eng = engine
         engine._call(lib().cig_serve_start(engine._e, C.byref(cfg), C.byref(self._s)))
+        engine._server = weakref.ref(self)
         self._outs = {}
+        _LIVE.add(self)
 
     def submit_update(self, dels, adds) -> int:
         d = np.ascontiguousarray(dels, dtype=EDGE_DTYPE)
@@ -641,9 +670,12 @@ class Server:
         return self._collect(lib().cig_wait, ticket)
 
     def stop(self):
-        if self._s:
-            _check(lib().cig_serve_stop(self._s))
-            self._s = C.c_void_p()
+        if getattr(self, "_s", None) and self._s.value:
+            s, self._s = self._s, C.c_void_p()
+            _check(lib().cig_serve_stop(s))
+
+    def __del__(self):
+        self.stop()
 
     def __enter__(self):
         return self

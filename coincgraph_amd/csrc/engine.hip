@@ -119,6 +119,7 @@ constexpr uint32_t kPullBlocksPerCu = 16;
 struct cig_engine {
     cig_config cfg{};
     std::string err;
+    cig_server *server = nullptr;  // the serving loop that owns the engine while it runs
     int device = 0;
     int cus = 0;
     hipStream_t stream = nullptr;
@@ -2048,6 +2049,26 @@ struct cig_engine {
 
 namespace {
 
+// Status of the exception being handled (call inside a catch block); err
+// receives its message. Used where an exception must not leave a thread.
+cig_status current_exception_status(std::string &err) {
+    try {
+        throw;
+    } catch (const HipError &x) {
+        err = x.what();
+        return x.code;
+    } catch (const std::bad_alloc &) {
+        err = "host allocation failed";
+        return CIG_ENOMEM;
+    } catch (const std::exception &x) {
+        err = x.what();
+        return CIG_EINTERNAL;
+    } catch (...) {
+        err = "unknown exception";
+        return CIG_EINTERNAL;
+    }
+}
+
 template <class F>
 cig_status guarded(cig_engine *e, F f) {
     if (!e) return CIG_EINVAL;
@@ -2161,9 +2182,8 @@ struct cig_server {
             }
             e->check_ready();
             e->apply_group(sd.p, uint32_t(nd), sa.p, uint32_t(na), dcut, acut, 0, &stats);
-        } catch (const HipError &x) {
-            st = x.code;
-            e->err = x.what();
+        } catch (...) {
+            st = current_exception_status(e->err);
         }
         const uint64_t v0 = version + 1;
         if (st == CIG_OK) version += batch.size();
@@ -2179,9 +2199,8 @@ struct cig_server {
             std::vector<cig_engine::QueryOut> outs;
             for (auto &r : batch) outs.push_back({r->out, uint32_t(r->sources.size())});
             e->query(batch[0]->algo, src.data(), uint32_t(src.size()), outs.data(), outs.size(), &stats);
-        } catch (const HipError &x) {
-            st = x.code;
-            e->err = x.what();
+        } catch (...) {
+            st = current_exception_status(e->err);
         }
         finish(batch, st, stats, version, false);
     }
@@ -2232,8 +2251,9 @@ struct cig_server {
                         fail(batch[0], CIG_ENOMEM);
                         continue;
                     }
-                } catch (const HipError &x) {
-                    for (auto &r : batch) fail(r, x.code);
+                } catch (...) {
+                    const cig_status st = current_exception_status(e->err);
+                    for (auto &r : batch) fail(r, st);
                     continue;
                 }
                 serve_updates(batch);
@@ -2248,8 +2268,9 @@ struct cig_server {
                         for (auto &r : batch) fail(r, CIG_ENOMEM);
                         continue;
                     }
-                } catch (const HipError &x) {
-                    for (auto &r : batch) fail(r, x.code);
+                } catch (...) {
+                    const cig_status st = current_exception_status(e->err);
+                    for (auto &r : batch) fail(r, st);
                     continue;
                 }
                 serve_queries(batch);
@@ -2302,7 +2323,11 @@ cig_status cig_create(const cig_config *cfg, cig_engine **out) {
     return CIG_OK;
 }
 
-void cig_destroy(cig_engine *e) { delete e; }
+void cig_destroy(cig_engine *e) {
+    if (!e) return;
+    if (e->server) (void)cig_serve_stop(e->server);  // its thread uses the engine until it stops
+    delete e;
+}
 
 const char *cig_last_error(const cig_engine *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
 
@@ -2481,10 +2506,12 @@ cig_status cig_serve_start(cig_engine *e, const cig_serve_config *c, cig_server 
         if (c) cfg = *c;
         if (!cfg.max_group || !cfg.max_query_sources || cfg.max_query_sources > 64)
             throw HipError(CIG_EINVAL, "max_group >= 1 and max_query_sources in 1..64");
+        if (e->server) throw HipError(CIG_ESTATE, "a server already runs on this engine");
         auto *s = new cig_server;
         s->e = e;
         s->cfg = cfg;
         s->th = std::thread([s] { s->loop(); });
+        e->server = s;
         *out = s;
     });
 }
@@ -2541,13 +2568,17 @@ cig_status cig_submit_query(cig_server *s, uint32_t algo, const uint32_t *source
 static cig_status collect(cig_server *s, uint64_t ticket, cig_request_info *info, bool block) {
     if (!s || !info) return CIG_EINVAL;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->live.find(ticket);
-    if (it == s->live.end()) return CIG_EINVAL;
-    auto r = it->second;
+    std::shared_ptr<ServeRequest> r;
+    {
+        auto it = s->live.find(ticket);
+        if (it == s->live.end()) return CIG_EINVAL;
+        r = it->second;  // no iterator is kept across the wait: submissions may rehash `live`
+    }
     if (block) s->done_cv.wait(lk, [&] { return r->done; });
     if (!r->done) return CIG_EPENDING;
+    // Another waiter on the same ticket may have collected it meanwhile.
+    if (s->live.erase(ticket) == 0) return CIG_EINVAL;
     *info = r->info;
-    s->live.erase(it);
     return CIG_OK;
 }
 
@@ -2562,6 +2593,7 @@ cig_status cig_serve_stop(cig_server *s) {
         s->work_cv.notify_all();
     }
     if (s->th.joinable()) s->th.join();
+    if (s->e && s->e->server == s) s->e->server = nullptr;
     delete s;
     return CIG_OK;
 }

@@ -530,10 +530,24 @@ struct File {
     bool write(const void *p, size_t bytes) { return !bytes || std::fwrite(p, 1, bytes, f) == bytes; }
     bool read(void *p, size_t bytes) { return !bytes || std::fread(p, 1, bytes, f) == bytes; }
     bool at_end() { return std::fgetc(f) == EOF; }
+    // bytes between the current position and the end of the file
+    uint64_t remaining() {
+        const long pos = std::ftell(f);
+        if (pos < 0 || std::fseek(f, 0, SEEK_END) != 0) return 0;
+        const long end = std::ftell(f);
+        std::fseek(f, pos, SEEK_SET);
+        return end > pos ? uint64_t(end - pos) : 0;
+    }
 };
 
 template <class T>
 T *read_array(File &in, uint64_t count, bool &ok) {
+    // a header count is untrusted: count * sizeof(T) must not wrap, and the
+    // file must hold that many bytes after the current position
+    if (count > SIZE_MAX / sizeof(T) || count * sizeof(T) > in.remaining()) {
+        ok = false;
+        return nullptr;
+    }
     T *p = static_cast<T *>(std::malloc(std::max<uint64_t>(1, count) * sizeof(T)));
     if (!p) return nullptr;
     ok = ok && in.read(p, count * sizeof(T));
@@ -561,7 +575,7 @@ extern "C" cig_status cig_read_csr_bin(const char *path, uint32_t *n_out, uint64
     if (!in.read(&h, sizeof(h)) || h.magic != kGraphMagic || h.version != kBinVersion) return CIG_EIO;
     bool ok = true;
     uint64_t *off = read_array<uint64_t>(in, uint64_t(h.count32) + 1, ok);
-    if (!off) return CIG_ENOMEM;
+    if (!off) return ok ? CIG_ENOMEM : CIG_EIO;
     ok = ok && off[0] == 0 && off[h.count32] == h.a;
     for (uint32_t v = 0; ok && v < h.count32; ++v) ok = off[v] <= off[v + 1];
     uint32_t *dst = ok ? read_array<uint32_t>(in, h.a, ok) : nullptr;
@@ -609,7 +623,7 @@ extern "C" cig_status cig_read_updates_bin(const char *path, cig_edge **adds_out
         std::free(sizes);
         std::free(adds);
         std::free(dels);
-        return CIG_ENOMEM;
+        return ok ? CIG_ENOMEM : CIG_EIO;
     }
     uint64_t ta = 0, td = 0;
     for (uint32_t b = 0; ok && b < h.count32; ++b) {

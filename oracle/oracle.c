@@ -382,18 +382,29 @@ void og_pr_check(const og_graph *g, const float *ranks, const float *residual,
     free(actual);
 }
 
-void og_pr_error(const float *ranks, const double *ref, uint32_t n, double *l1,
-                 double *max_err, double *mass) {
-    double a = 0, m = 0, s = 0;
+/* hybrid_pr_host.cu:15-45: the residual invariant x + r = 0.15 + 0.85*P^T x
+ * and the largest residual, with no reference solve. */
+int og_pr_invariant(const og_graph *g, const float *ranks, const float *residual,
+                    double *defect, double *max_res, double *mass) {
+    uint32_t n = g->n;
+    double *x = (double *)calloc((size_t)n + 1, sizeof(double));
+    double *next = (double *)calloc((size_t)n + 1, sizeof(double));
+    int finite = 1;
+    double d = 0, mr = 0, m = 0;
+    for (uint32_t v = 0; v < n; ++v) x[v] = ranks[v];
+    pr_multiply(g, x, next);
     for (uint32_t v = 0; v < n; ++v) {
-        double e = fabs((double)ranks[v] - ref[v]);
-        a += e;
-        if (e > m) m = e;
-        s += fabs(ref[v]);
+        finite &= isfinite(ranks[v]) && isfinite(residual[v]);
+        d += fabs(next[v] - (double)ranks[v] - (double)residual[v]);
+        if (fabs((double)residual[v]) > mr) mr = fabs((double)residual[v]);
+        m += fabs((double)ranks[v]);
     }
-    *l1 = a;
-    *max_err = m;
-    *mass = s;
+    *defect = d;
+    *max_res = mr;
+    *mass = m;
+    free(x);
+    free(next);
+    return finite;
 }
 
 /* hybrid_sssp.cu:211-264 (CheckBellmanRelaxed). */
@@ -425,20 +436,75 @@ void og_check_bellman(const og_graph *g, const uint32_t *dist, uint32_t src, int
     free(tight);
 }
 
+/* Open-addressing set of (parent, vertex) pairs: CheckParentWitness asks,
+ * for every reached vertex, whether its parent's row holds it. Looking each
+ * pair up while scanning every row once costs O(E + n) instead of
+ * O(sum of the parents' degrees), which hub parents make quadratic. */
+typedef struct {
+    uint64_t *key; /* p << 32 | v, or ~0 for an empty slot */
+    uint8_t *hit;
+    uint64_t mask;
+} pair_set;
+
+static uint64_t pair_hash(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return k;
+}
+
+static int pair_set_init(pair_set *t, uint64_t count) {
+    uint64_t cap = 16;
+    while (cap < 2 * count) cap <<= 1;
+    t->mask = cap - 1;
+    t->key = (uint64_t *)malloc(cap * sizeof(uint64_t));
+    t->hit = (uint8_t *)calloc(cap, 1);
+    if (!t->key || !t->hit) return -1;
+    memset(t->key, 0xFF, cap * sizeof(uint64_t));
+    return 0;
+}
+
+/* slot of k; inserted when absent and insert != 0, else ~0 when absent */
+static uint64_t pair_set_slot(pair_set *t, uint64_t k, int insert) {
+    for (uint64_t i = pair_hash(k) & t->mask;; i = (i + 1) & t->mask) {
+        if (t->key[i] == k) return i;
+        if (t->key[i] == ~0ull) {
+            if (!insert) return ~0ull;
+            t->key[i] = k;
+            return i;
+        }
+    }
+}
+
 /* hybrid_sssp.cu:266-316 (CheckParentWitness). */
 void og_check_parent(const og_graph *g, const uint32_t *dist, const uint32_t *parent,
                      uint32_t src, int wmode, uint64_t seed, og_parent_result *out) {
     uint32_t n = g->n;
     memset(out, 0, sizeof(*out));
     out->first_vertex = out->first_parent = OG_INF;
+    pair_set set;
+    if (pair_set_init(&set, n) != 0) {
+        free(set.key);
+        free(set.hit);
+        out->invalid_vertices = n;
+        return;
+    }
+    for (uint32_t v = 0; v < n; ++v) {
+        if (v == src || dist[v] == OG_INF) continue;
+        uint32_t p = parent[v];
+        if (p < n && dist[p] != OG_INF) pair_set_slot(&set, (uint64_t)p << 32 | v, 1);
+    }
+    for (uint32_t s = 0; s < n; ++s)
+        for (uint32_t e = 0; e < g->deg[s]; ++e) {
+            uint64_t i = pair_set_slot(&set, (uint64_t)s << 32 | g->rows[s][e], 0);
+            if (i != ~0ull) set.hit[i] = 1;
+        }
     for (uint32_t v = 0; v < n; ++v) {
         if (v == src || dist[v] == OG_INF) continue;
         uint32_t p = parent[v];
         int valid = p < n && dist[p] != OG_INF;
         if (valid) {
-            int exists = 0;
-            for (uint32_t e = 0; e < g->deg[p]; ++e)
-                if (g->rows[p][e] == v) { exists = 1; break; }
+            int exists = set.hit[pair_set_slot(&set, (uint64_t)p << 32 | v, 0)];
             int matches = (uint64_t)dist[p] + og_weight(wmode, seed, p, v) == dist[v];
             if (!exists) out->missing_parent_edges++;
             else if (!matches) out->wrong_parent_distances++;
@@ -449,6 +515,8 @@ void og_check_parent(const og_graph *g, const uint32_t *dist, const uint32_t *pa
             out->invalid_vertices++;
         }
     }
+    free(set.key);
+    free(set.hit);
 }
 
 /* hybrid_sssp.cu:318-330 (DistanceChecksum); hybrid_cc.cu:189-201 is the

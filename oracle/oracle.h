@@ -108,9 +108,12 @@ uint64_t og_parent_checksum(const uint32_t *dist, const uint32_t *parent, uint32
 uint64_t og_cc_check(const og_graph *g, const uint32_t *labels);
 void og_pr_check(const og_graph *g, const float *ranks, const float *residual,
                  double epsilon, og_pr_check_result *out);
-/* Error of ranks against an FP64 reference: sum|x-x*|, max|x-x*|, sum|x*|. */
-void og_pr_error(const float *ranks, const double *ref, uint32_t n, double *l1,
-                 double *max_err, double *mass);
+/* The invariant half of PageRankCheck (hybrid_pr_host.cu:15-45) without the
+ * reference solve, for graphs too large for a Jacobi run per batch:
+ * defect = sum_v |0.15 + 0.85*sum_{u->v} x[u]/outdeg(u) - x[v] - r[v]|,
+ * max_res = max|r|, mass = sum|x|; returns 0 if any x or r is not finite. */
+int og_pr_invariant(const og_graph *g, const float *ranks, const float *residual,
+                    double *defect, double *max_res, double *mass);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,9 @@ def lib():
         L.og_pr_jacobi.restype = C.c_int
         L.og_pr_jacobi.argtypes = [C.POINTER(_Graph), P, C.c_double, C.c_int]
         L.og_pr_check.argtypes = [C.POINTER(_Graph), P, P, C.c_double, C.POINTER(PrCheckResult)]
+        L.og_pr_invariant.restype = C.c_int
+        L.og_pr_invariant.argtypes = [C.POINTER(_Graph), P, P, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.og_check_bellman.argtypes = [C.POINTER(_Graph), P, C.c_uint32, C.c_int, C.c_uint64,
                                        C.POINTER(BellmanResult)]
         L.og_check_parent.argtypes = [C.POINTER(_Graph), P, P, C.c_uint32, C.c_int, C.c_uint64,
@@ -211,6 +214,15 @@ class Graph:
         res = PrCheckResult()
         lib().og_pr_check(self._g, _p(x), _p(r), epsilon, C.byref(res))
         return res
+
+    def pr_invariant(self, ranks, residual):
+        """(finite, defect, max|r|, sum|x|): PageRankCheck's invariant
+        sum_v |0.15 + 0.85 sum x[u]/deg(u) - x[v] - r[v]| without a reference solve."""
+        x = np.ascontiguousarray(ranks, dtype=np.float32)
+        r = np.ascontiguousarray(residual, dtype=np.float32)
+        d, mr, m = C.c_double(), C.c_double(), C.c_double()
+        ok = lib().og_pr_invariant(self._g, _p(x), _p(r), C.byref(d), C.byref(mr), C.byref(m))
+        return bool(ok), d.value, mr.value, m.value
 
 
 def distance_checksum(dist) -> int:

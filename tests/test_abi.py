@@ -180,6 +180,19 @@ def test_binary_readers_reject_bad_files(tmp_path):
     (tmp_path / "s_sizes.cigu").write_bytes(bytes(big_sizes))
     with pytest.raises(cg.CigError):
         cg.read_updates_bin(str(tmp_path / "s_sizes.cigu"))
+    # header record counts whose byte size wraps 2^64 (2^61 * 8 = 0) or exceeds the
+    # file are rejected as I/O errors before anything is allocated
+    for off, count in ((16, 1 << 61), (16, (1 << 64) - 1), (24, 1 << 62)):
+        for raw, ext in ((raw_g, "cigg"), (raw_s, "cigu")):
+            if off == 24 and ext == "cigg":
+                continue  # the graph header's second count is unused
+            bad = bytearray(raw)
+            bad[off:off + 8] = count.to_bytes(8, "little")
+            p = tmp_path / f"count_{off}_{count}.{ext}"
+            p.write_bytes(bytes(bad))
+            with pytest.raises(cg.CigError) as ei:
+                (cg.read_csr_bin if ext == "cigg" else cg.read_updates_bin)(str(p))
+            assert ei.value.status == cg.EIO, (off, count, ext)
 
 
 def test_oracle_golden_fixture_known_answers():

@@ -263,3 +263,136 @@ def test_checksums_special_cases():
     # single entry: (basis ^ (0 + golden + 0 + 0)) * prime mod 2^64
     one = O.distance_checksum(np.array([0], dtype=np.uint32))
     assert one == ((basis ^ 0x9E3779B97F4A7C15) * 1099511628211) % (1 << 64)
+
+
+def test_parent_checksum_closed_forms():
+    """og_parent_checksum (hybrid_sssp.cu:604-619): basis 1099511628211,
+    prime 1469598103934665603, golden 0x517cc1b727220a95, parent terms
+    (p<<7) + (p>>3); unreachable vertices are skipped."""
+    basis, prime, M = 1099511628211, 1469598103934665603, 1 << 64
+    e = np.array([], dtype=np.uint32)
+    assert O.parent_checksum(e, e) == basis
+    inf2 = np.array([INF, INF], dtype=np.uint32)
+    assert O.parent_checksum(inf2, np.array([3, 9], dtype=np.uint32)) == basis
+
+    def one(v, p):
+        return ((basis ^ (v + 0x517CC1B727220A95 + (p << 7) + (p >> 3))) * prime) % M
+
+    # a single reachable vertex at index v: the INF entries before it are skipped
+    for v, p in ((0, 0), (0, 12345), (2, 4000000000), (1, INF)):
+        dist = np.full(v + 1, INF, dtype=np.uint32)
+        dist[v] = 7
+        par = np.zeros(v + 1, dtype=np.uint32)
+        par[v] = p
+        assert O.parent_checksum(dist, par) == one(v, p), (v, p)
+    # two entries chain: the second folds into the first's product
+    d = np.array([0, 4], dtype=np.uint32)
+    pr = np.array([INF, 0], dtype=np.uint32)
+    h = one(0, INF)
+    h = ((h ^ (1 + 0x517CC1B727220A95 + 0 + 0)) * prime) % M
+    assert O.parent_checksum(d, pr) == h
+    # order and the parent value matter; the distance value does not
+    assert O.parent_checksum(d, pr) != O.parent_checksum(d, pr[::-1].copy())
+    assert O.parent_checksum(d, pr) == O.parent_checksum(np.array([9, 1], dtype=np.uint32), pr)
+    pr2 = pr.copy()
+    pr2[1] = 1
+    assert O.parent_checksum(d, pr2) != O.parent_checksum(d, pr)
+
+
+def test_pr_error_helper_hand_vectors():
+    """The normalised L1 / max error used by every PageRank parity test:
+    sum|x - x*| / sum|x*| and max|x - x*|, on hand-computed vectors."""
+    from tests.helpers import pr_errors
+    ref = np.array([1.0, 2.0, -1.0, 0.5])
+    x = np.array([1.5, 2.0, -0.75, 0.5], dtype=np.float32)
+    l1n, mx = pr_errors(x, ref)
+    assert abs(l1n - 0.75 / 4.5) < 1e-15
+    assert mx == 0.5
+    l1n, mx = pr_errors(ref.astype(np.float32), ref)
+    assert l1n == 0.0 and mx == 0.0
+
+
+@pytest.mark.parametrize("n,m", [(40, 120), (300, 1500)])
+def test_pr_invariant_closed_forms(n, m):
+    """og_pr_invariant (PageRankCheck's invariant, hybrid_pr_host.cu:15-45):
+    x = 0 with r = 0.15 has zero defect; the exact solve (numpy) with r = 0
+    has ~zero defect; moving x[v] by delta adds delta * (1 + 0.85 * (non-self
+    out-edges of v) / deg(v)) for a vertex v with out-edges, delta alone for a
+    dangling one; a non-finite entry is flagged."""
+    rng = np.random.default_rng(7 * n + m)
+    src, dst = random_multigraph(rng, n, m)
+    g = O.Graph.from_edges(n, src, dst)
+    deg = np.bincount(src, minlength=n).astype(np.float64)
+    M = np.zeros((n, n))
+    for u, v in zip(src, dst):
+        M[v, u] += 0.85 / deg[u]
+    zero = np.zeros(n, dtype=np.float32)
+    ok, d, mr, mass = g.pr_invariant(zero, np.full(n, 0.15, dtype=np.float32))
+    # only the float rounding of 0.15 is left
+    assert ok and abs(d - n * abs(0.15 - float(np.float32(0.15)))) < 1e-12
+    assert mr == float(np.float32(0.15)) and mass == 0.0
+    exact = np.linalg.solve(np.eye(n) - M, np.full(n, 0.15))
+    x = exact.astype(np.float32)
+    r = (0.15 + M @ x.astype(np.float64) - x.astype(np.float64)).astype(np.float32)
+    ok, d0, _, mass = g.pr_invariant(x, r)
+    assert ok and d0 <= 1e-5 * n and abs(mass - np.abs(x.astype(np.float64)).sum()) < 1e-9
+    delta = 0.5
+    for v in (int(np.argmax(deg)), int(np.argmin(deg))):
+        xb = x.copy()
+        xb[v] += np.float32(delta)
+        _, d1, _, _ = g.pr_invariant(xb, r)
+        if deg[v] > 0:
+            loops = int(np.sum((src == v) & (dst == v)))
+            # x[v] itself moves by delta; 0.85*delta/deg(v) lands on each out-edge's
+            # destination, v included for each self-loop
+            exp = abs(delta * (0.85 * loops / deg[v] - 1.0)) + delta * 0.85 * (deg[v] - loops) / deg[v]
+        else:
+            exp = delta
+        assert abs((d1 - d0) - exp) <= 1e-4, (v, d1 - d0, exp)
+    nanx = x.copy()
+    nanx[0] = np.nan
+    assert not g.pr_invariant(nanx, r)[0]
+
+
+def test_check_parent_brute_force():
+    """og_check_parent (CheckParentWitness, hybrid_sssp.cu:266-316) against a
+    direct restatement over Python sets: for every reached v other than the
+    source, its parent p must be reached, hold v in its row and satisfy
+    dist[p] + w(p,v) == dist[v]; parents are drawn from true tight parents,
+    random vertices, INF and out-of-range ids."""
+    rng = np.random.default_rng(31)
+    for trial in range(30):
+        n = int(rng.integers(2, 50))
+        src, dst = random_multigraph(rng, n, int(rng.integers(1, 4 * n)))
+        g = O.Graph.from_edges(n, src, dst)
+        dist = g.sssp(0, O.W_REF128)
+        edges = set(zip(src.tolist(), dst.tolist()))
+        par = rng.integers(0, n, n).astype(np.uint32)
+        pick = rng.random(n)
+        par[pick < 0.1] = INF
+        par[(pick >= 0.1) & (pick < 0.15)] = n + 3
+        for v in range(n):  # most vertices get a true tight parent
+            if pick[v] >= 0.5 and dist[v] != INF:
+                for u, w in edges:
+                    if w == v and dist[u] != INF and int(dist[u]) + O.weight(O.W_REF128, 0, u, v) == dist[v]:
+                        par[v] = u
+                        break
+        inval = missing = wrong = 0
+        first = INF
+        for v in range(n):
+            if v == 0 or dist[v] == INF:
+                continue
+            p = int(par[v])
+            ok = p < n and dist[p] != INF
+            if ok:
+                exists = (p, v) in edges
+                matches = int(dist[p]) + O.weight(O.W_REF128, 0, p, v) == int(dist[v])
+                missing += not exists
+                wrong += exists and not matches
+                ok = exists and matches
+            if not ok:
+                first = v if inval == 0 else first
+                inval += 1
+        r = g.check_parent(dist, par, 0, O.W_REF128)
+        assert (r.invalid_vertices, r.missing_parent_edges, r.wrong_parent_distances, r.first_vertex) == \
+            (inval, missing, wrong, first), trial
