@@ -44,6 +44,7 @@ CFG_SYMMETRIC = 1 << 5
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
+NOALLOC_SCRATCH_BYTES = 256 << 20  # CIG_NOALLOC_SCRATCH_BYTES
 
 
 class CigError(RuntimeError):
@@ -200,6 +201,8 @@ def lib():
         "cig_apply_next_batch": (C.c_int, [eng, C.c_uint32, C.POINTER(BatchStats)]),
         "cig_apply_next_batches": (C.c_int, [eng, C.c_uint32, C.c_uint32, C.POINTER(BatchStats)]),
         "cig_query": (C.c_int, [eng, C.c_uint32, P, C.c_uint32, P, C.c_uint64, C.POINTER(BatchStats)]),
+        "cig_query_noalloc": (C.c_int, [eng, C.c_uint32, P, C.c_uint32, C.c_void_p, C.c_uint64,
+                                        C.POINTER(BatchStats)]),
         "cig_update_batches_left": (C.c_uint32, [eng]),
         "cig_cache_refresh": (C.c_int, [eng, C.POINTER(BatchStats)]),
         "cig_gather": (C.c_int, [eng, C.c_uint32, P, C.c_size_t]),
@@ -561,6 +564,15 @@ class Engine:
                                    C.byref(st)))
         self.last_query_stats = st.as_dict()
         return out
+
+    def query_noalloc(self, algo: str, sources, dist_device: int, dist_bytes: int) -> dict:
+        """cig_query_noalloc: the distances go to device memory at address
+        `dist_device` (nsrc * n uint32, source-major); nothing is allocated."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        st = BatchStats()
+        self._call(lib().cig_query_noalloc(self._e, ALGOS[algo], _ptr(src), len(src),
+                                           C.c_void_p(dist_device), dist_bytes, C.byref(st)))
+        return st.as_dict()
 
     def batches_left(self) -> int:
         return int(lib().cig_update_batches_left(self._e))

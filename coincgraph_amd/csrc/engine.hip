@@ -42,6 +42,16 @@ namespace {
 
 thread_local std::string g_create_error;
 
+// Scratch of cig_query_noalloc: one array per GPU, allocated when the code
+// object is loaded on that GPU and held until the process ends. Its name is a
+// host-side stand-in, never a device address: host code uses only the
+// per-device address that scratch_address() resolves.
+static __device__ __align__(256) uint8_t scratch[CIG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
 struct HipError : std::runtime_error {
     cig_status code;
     HipError(cig_status c, const std::string &m) : std::runtime_error(m), code(c) {}
@@ -54,6 +64,19 @@ struct HipError : std::runtime_error {
             throw HipError(_e == hipErrorOutOfMemory ? CIG_ENOMEM : CIG_EDEVICE,            \
                            std::string(#call) + ": " + hipGetErrorString(_e));              \
     } while (0)
+
+// This device's address of the scratch array, resolved once per device
+// with that device current (every engine's set-up calls this).
+uint8_t *scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) throw HipError(CIG_EINVAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<uint8_t *>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 template <class T>
 struct DBuf {
@@ -163,7 +186,8 @@ struct cig_engine {
     DBuf<uint8_t> gconf, gskip;
     const uint8_t *cur_skip = nullptr;  // cancel flags of the group being applied
     // multi-source queries
-    DBuf<uint32_t> ms_dist, ms_stamp, ms_src, ms_out;
+    DBuf<uint32_t> ms_dist, ms_stamp, ms_out;
+    uint8_t *scratch_dev = nullptr;  // this device's copy of the no-allocation scratch array
     DBuf<unsigned long long> ms_act, ms_snap, ms_seen;
     HBuf<uint8_t> bounce[2];  // pinned bounce buffers of large device-to-host results
     hipEvent_t bounce_ev[2] = {};
@@ -216,6 +240,7 @@ struct cig_engine {
         HIPCHK(hipGetDeviceProperties(&prop, cfg.device));
         cus = prop.multiProcessorCount;
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        scratch_dev = scratch_address(cfg.device);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         if (cfg.flags & CIG_CFG_KERNEL_TIMING) {
             rev_begin.resize(kRoundEvents);
@@ -1758,14 +1783,61 @@ struct cig_engine {
         query(algo, sources, k, &one, 1, st);
     }
 
-    // The results go straight to each caller's buffer (no merged host copy).
-    void query(uint32_t algo, const uint32_t *sources, uint32_t k, const QueryOut *outs, size_t nouts,
-               cig_batch_stats *st) {
+    // Working memory of one multi-source run (vertex-major distances, source
+    // masks, BFS reach masks, enqueue stamps).
+    struct QueryMem {
+        uint32_t *dist;
+        unsigned long long *act, *snap, *seen;
+        uint32_t *stamp;
+    };
+
+    void query_check(uint32_t algo, const uint32_t *sources, uint32_t k) const {
         if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
         if (algo != CIG_ALGO_BFS && algo != CIG_ALGO_SSSP) throw HipError(CIG_EINVAL, "queries are BFS or SSSP");
         if (k == 0 || k > 64 || !sources) throw HipError(CIG_EINVAL, "1..64 sources");
         for (uint32_t s = 0; s < k; ++s)
             if (sources[s] >= n) throw HipError(CIG_EINVAL, "source out of range");
+    }
+
+    // Seeds the sources and runs the rounds to the fixed point; the caller
+    // has begun the call and set mem's distances to kInf and its masks to 0
+    // (stamps may hold any tag below the current round, or 0).
+    void query_run(uint32_t algo, const uint32_t *sources, uint32_t k, const QueryMem &mem) {
+        const bool bfs = algo == CIG_ALGO_BFS;
+        MsArgs a{};
+        a.ctl = ctl.p;
+        a.rows = rows.p;
+        a.dist = mem.dist;
+        a.act = mem.act;
+        a.snap = mem.snap;
+        a.seen = bfs ? mem.seen : nullptr;
+        a.stamp = mem.stamp;
+        a.q0 = q0.p;
+        a.q1 = q1.p;
+        a.t0 = t0.p;
+        a.t1 = t1.p;
+        a.cfg = ecfg(kStagePropagate);
+        a.n = n;
+        a.stride = (k + 3u) & ~3u;
+        a.seed = cfg.weight_seed;
+        SrcList list{};
+        std::memcpy(list.v, sources, size_t(k) * 4);
+        k_ms_seed<<<1, 64, 0, stream>>>(a, list, k);
+        HIPCHK(hipGetLastError());
+        for (int m : {1, 2, 3, 5}) mark(m);
+        const uint32_t wkind = bfs ? 2u : (cfg.weight_mode == CIG_WEIGHT_HASH64 ? 1u : 0u);
+        switch (qpl) {
+        case 1: ms_dispatch<1>(wkind, a); break;
+        case 4: ms_dispatch<4>(wkind, a); break;
+        default: ms_dispatch<2>(wkind, a); break;
+        }
+        mark(4);
+    }
+
+    // The results go straight to each caller's buffer (no merged host copy).
+    void query(uint32_t algo, const uint32_t *sources, uint32_t k, const QueryOut *outs, size_t nouts,
+               cig_batch_stats *st) {
+        query_check(algo, sources, k);
         uint64_t placed = 0;
         for (size_t i = 0; i < nouts; ++i) {
             if (!outs[i].p) throw HipError(CIG_EINVAL, "null output buffer");
@@ -1782,40 +1854,13 @@ struct cig_engine {
         }
         const bool bfs = algo == CIG_ALGO_BFS;
         if (bfs && ms_seen.count < n) ms_seen.alloc(n);
-        if (!ms_src.count) ms_src.alloc(64);
         if (ms_out.count < uint64_t(n) * k) ms_out.alloc(uint64_t(n) * k);
         begin_call(st);
         fill32(ms_dist.p, uint64_t(n) * stride, kInf);
         fill32(reinterpret_cast<uint32_t *>(ms_act.p), uint64_t(n) * 2, 0);
         fill32(reinterpret_cast<uint32_t *>(ms_snap.p), uint64_t(n) * 2, 0);
         if (bfs) fill32(reinterpret_cast<uint32_t *>(ms_seen.p), uint64_t(n) * 2, 0);
-        HIPCHK(hipMemcpyAsync(ms_src.p, sources, k * 4, hipMemcpyHostToDevice, stream));
-        MsArgs a{};
-        a.ctl = ctl.p;
-        a.rows = rows.p;
-        a.dist = ms_dist.p;
-        a.act = ms_act.p;
-        a.snap = ms_snap.p;
-        a.seen = bfs ? ms_seen.p : nullptr;
-        a.stamp = ms_stamp.p;
-        a.q0 = q0.p;
-        a.q1 = q1.p;
-        a.t0 = t0.p;
-        a.t1 = t1.p;
-        a.cfg = ecfg(kStagePropagate);
-        a.n = n;
-        a.stride = stride;
-        a.seed = cfg.weight_seed;
-        k_ms_seed<<<1, 64, 0, stream>>>(a, ms_src.p, k);
-        HIPCHK(hipGetLastError());
-        for (int m : {1, 2, 3, 5}) mark(m);
-        const uint32_t wkind = algo == CIG_ALGO_BFS ? 2u : (cfg.weight_mode == CIG_WEIGHT_HASH64 ? 1u : 0u);
-        switch (qpl) {
-        case 1: ms_dispatch<1>(wkind, a); break;
-        case 4: ms_dispatch<4>(wkind, a); break;
-        default: ms_dispatch<2>(wkind, a); break;
-        }
-        mark(4);
+        query_run(algo, sources, k, QueryMem{ms_dist.p, ms_act.p, ms_snap.p, ms_seen.p, ms_stamp.p});
         k_ms_gather<<<grid_small, 256, 0, stream>>>(ms_dist.p, n, stride, k, ms_out.p);
         HIPCHK(hipGetLastError());
         uint64_t at = 0;
@@ -1823,6 +1868,53 @@ struct cig_engine {
             copy_out(outs[i].p, ms_out.p + at, uint64_t(n) * outs[i].nsrc * 4);
             at += uint64_t(n) * outs[i].nsrc;
         }
+        end_call(st, 4);
+    }
+
+    // Byte offsets of a run's working memory inside the no-allocation scratch
+    // array: distances, act, snap, (BFS) seen, stamps, each 256-B aligned.
+    static uint64_t noalloc_layout(uint32_t nv, uint32_t k, bool bfs, uint64_t off[5]) {
+        auto up = [](uint64_t x) { return (x + 255) & ~uint64_t(255); };
+        const uint64_t stride = (k + 3u) & ~3u;
+        off[0] = 0;
+        off[1] = up(off[0] + uint64_t(nv) * stride * 4);
+        off[2] = up(off[1] + uint64_t(nv) * 8);
+        off[3] = up(off[2] + uint64_t(nv) * 8);
+        off[4] = up(off[3] + (bfs ? uint64_t(nv) * 8 : 0));
+        return up(off[4] + uint64_t(nv) * 4);
+    }
+
+    // cig_query with every buffer of the run in the device's scratch array
+    // (see cig_query_noalloc): no allocation on any path. The distances go
+    // to the caller's device memory, source-major.
+    void query_noalloc(uint32_t algo, const uint32_t *sources, uint32_t k, uint32_t *dist_dev, uint64_t bytes,
+                       cig_batch_stats *st) {
+        query_check(algo, sources, k);
+        if (!dist_dev || bytes < uint64_t(n) * k * 4) throw HipError(CIG_EINVAL, "output buffer too small");
+        const bool bfs = algo == CIG_ALGO_BFS;
+        uint64_t off[5];
+        const uint64_t need = noalloc_layout(n, k, bfs, off);
+        if (need > CIG_NOALLOC_SCRATCH_BYTES)
+            throw HipError(CIG_ELIMIT, "query needs " + std::to_string(need) + " bytes of scratch; the array holds " +
+                                           std::to_string(uint64_t(CIG_NOALLOC_SCRATCH_BYTES)));
+        if (!scratch_dev) throw HipError(CIG_EINTERNAL, "scratch array address not resolved");
+        const uint32_t stride = (k + 3u) & ~3u;
+        QueryMem m{reinterpret_cast<uint32_t *>(scratch_dev + off[0]),
+                   reinterpret_cast<unsigned long long *>(scratch_dev + off[1]),
+                   reinterpret_cast<unsigned long long *>(scratch_dev + off[2]),
+                   bfs ? reinterpret_cast<unsigned long long *>(scratch_dev + off[3]) : nullptr,
+                   reinterpret_cast<uint32_t *>(scratch_dev + off[4])};
+        begin_call(st);
+        // the array keeps the last call's contents (of any engine on this GPU):
+        // everything the run reads before writing is cleared, stamps included
+        fill32(m.dist, uint64_t(n) * stride, kInf);
+        fill32(reinterpret_cast<uint32_t *>(m.act), uint64_t(n) * 2, 0);
+        fill32(reinterpret_cast<uint32_t *>(m.snap), uint64_t(n) * 2, 0);
+        if (bfs) fill32(reinterpret_cast<uint32_t *>(m.seen), uint64_t(n) * 2, 0);
+        fill32(m.stamp, n, 0);
+        query_run(algo, sources, k, m);
+        k_ms_gather<<<grid_small, 256, 0, stream>>>(m.dist, n, stride, k, dist_dev);
+        HIPCHK(hipGetLastError());
         end_call(st, 4);
     }
 
@@ -2398,6 +2490,11 @@ cig_status cig_autotune(cig_engine *e, uint32_t source, cig_tune_result *res) {
 cig_status cig_query(cig_engine *e, uint32_t algo, const uint32_t *sources, uint32_t nsrc, uint32_t *dist,
                      uint64_t dist_bytes, cig_batch_stats *st) {
     return guarded(e, [&] { e->query(algo, sources, nsrc, dist, dist_bytes, st); });
+}
+
+cig_status cig_query_noalloc(cig_engine *e, uint32_t algo, const uint32_t *sources, uint32_t nsrc,
+                             uint32_t *dist_device, uint64_t dist_bytes, cig_batch_stats *st) {
+    return guarded(e, [&] { e->query_noalloc(algo, sources, nsrc, dist_device, dist_bytes, st); });
 }
 
 cig_status cig_apply_next_batches(cig_engine *e, uint32_t count, uint32_t flags, cig_batch_stats *st) {

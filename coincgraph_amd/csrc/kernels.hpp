@@ -2090,10 +2090,16 @@ struct MsOpOf<2> {
     }
 };
 
-__global__ void k_ms_seed(MsArgs a, const uint32_t *sources, uint32_t k) {
+// The sources of a multi-source run, passed by value (256 B of kernel
+// arguments), so that no device copy of them is needed.
+struct SrcList {
+    uint32_t v[64];
+};
+
+__global__ void k_ms_seed(MsArgs a, SrcList sources, uint32_t k) {
     const uint32_t r = a.ctl->round;
     for (uint32_t s = gthread(); s < k; s += gthreads()) {
-        const uint32_t v = sources[s];
+        const uint32_t v = sources.v[s];
         a.dist[uint64_t(v) * a.stride + s] = 0;
         atomicOr(&a.act[v], 1ull << s);
         if (a.seen) atomicOr(&a.seen[v], 1ull << s);

@@ -332,6 +332,20 @@ cig_status cig_autotune(cig_engine *e, uint32_t source, cig_tune_result *result)
  * state is not touched. */
 cig_status cig_query(cig_engine *e, uint32_t algo, const uint32_t *sources, uint32_t nsrc, uint32_t *dist,
                      uint64_t dist_bytes, cig_batch_stats *stats);
+/* No counterpart in the reference. For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): cig_query without any allocation. The run's working memory
+ * (vertex-major distances, source masks, stamps: 4*round_up(nsrc,4) + 28
+ * bytes per vertex, 36 for BFS) lives in a fixed device array of
+ * CIG_NOALLOC_SCRATCH_BYTES that each GPU holds from library load to process
+ * exit; sources are passed by value; the distances are written to the
+ * caller's device memory dist_device (nsrc*n words, source-major, as
+ * cig_query). CIG_ELIMIT when the run needs more than the array. All calls on
+ * one GPU share its array: calls that overlap on one GPU (from any engine or
+ * thread) race and must not be issued. */
+#define CIG_NOALLOC_SCRATCH_BYTES (256ull << 20)
+cig_status cig_query_noalloc(cig_engine *e, uint32_t algo, const uint32_t *sources, uint32_t nsrc,
+                             uint32_t *dist_device, uint64_t dist_bytes, cig_batch_stats *stats);
 /* Batched form (no counterpart in the reference, whose driver applies one
  * batch per call): the next `count` resident batches as one group. Every
  * delete is matched to the latest earlier add of the same (u,v) in the
