@@ -52,9 +52,10 @@ CONFIGS = {
             what="incremental SSSP, R-MAT scale 22 edge factor 16 (a,b,c,d=0.57,0.19,0.19,0.05, deduped, "
                  "unpermuted), source 0, uint32 weights uniform in [1,64], batches of 100000 deletes + "
                  "100000 inserts, graph and stream resident in HBM"),
-    3: dict(algo="cc", scale=24, ef=16, sym=True, batch=1_000_000, steps=10, group=10,
-            what="incremental min-label CC, symmetrised R-MAT scale 24 edge factor 16 (both directions "
-                 "stored), batches of 1M symmetric delete pairs + 1M symmetric insert pairs"),
+    3: dict(algo="cc", scale=24, ef=8, sym=True, batch=1_000_000, steps=10, group=10, autotune=True,
+            what="incremental min-label CC, symmetrised R-MAT scale 24 (8 undirected samples per vertex, "
+                 "both directions stored: ~268M directed edges), batches of 1M symmetric delete pairs + 1M "
+                 "symmetric insert pairs, autotuned launch shape"),
     4: dict(algo="pr", scale=25, ef=16, sym=False, batch=500_000, steps=10, group=10,
             what="incremental PageRank (alpha 0.85, epsilon 1e-7), R-MAT scale 25 edge factor 16, "
                  "batches of 500000 deletes + 500000 inserts"),
@@ -68,6 +69,24 @@ CONFIGS = {
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_info():
+    """CPU model string, hardware threads and the threads this process may use."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu": model, "hardware_concurrency": os.cpu_count(), "nproc": usable}
 
 
 def percentile(xs, q):
@@ -146,6 +165,54 @@ def cpu_baseline(g, stream, batches, algo, wseed, budget_s):
     }
 
 
+def cpu_mutation_pool(g, stream, batches, budget_s):
+    """The reference's CPU batch maintenance as its worker pool runs it:
+    per-source chunk mutation with min(20, hardware threads) workers
+    (source_local_chunk_store.h:116-121, 218-391), restated in the oracle,
+    on the same batches, timed alone (no recompute)."""
+    from oracle import oracle as O
+    workers = min(20, os.cpu_count() or 1)
+    og = O.Graph(g.n, g.offsets, g.dst)
+    t_total, done, recs = 0.0, 0, 0
+    for b in range(batches):
+        dels, adds = stream.batch(b)
+        t0 = time.perf_counter()
+        og.apply_batch_mt(dels["u"], dels["v"], adds["u"], adds["v"], workers)
+        t_total += time.perf_counter() - t0
+        done += 1
+        recs += len(dels) + len(adds)
+        if t_total >= budget_s:
+            break
+    return {"workers": workers, "batches": done, "ms_per_batch": round(t_total / done * 1e3, 3),
+            "updates_per_s": round(recs / t_total, 1)}
+
+
+def multi_source_form(g, stream, batches, algo, wseed, single_p50):
+    """One engine keeping 64 sources current through every batch (cig_init
+    with 64 sources): per-batch latency and source-updates/s against 64
+    single-source engines at the single-source p50 (every batch's state is
+    produced for every source in both forms)."""
+    import coincgraph_amd as cg
+    k = 64
+    sources = (np.arange(k, dtype=np.uint64) * 65537 % g.n).astype(np.uint32)
+    sources[0] = 0
+    kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=wseed) if algo == "sssp" else {}
+    with cg.Engine(algo, **kw) as e:
+        e.load_graph(g)
+        e.init_graph(sources)
+        st0 = e.start()
+        e.load_updates(stream)
+        lat = []
+        for _ in range(batches):
+            lat.append(e.apply_next_batch()["total_ms"])
+    p50 = percentile(lat, 50)
+    per_batch_updates = 2 * int(stream.batch_sizes[0][0])
+    return {"sources": k, "batches": batches, "initial_ms": round(st0["total_ms"], 3),
+            "latency_ms": {"p50": round(p50, 4), "p99": round(percentile(lat, 99), 4), "samples": len(lat)},
+            "source_updates_per_s": round(k * per_batch_updates / (p50 * 1e-3), 1),
+            "speedup_vs_single_source_engines": round(k * single_p50 / p50, 2)}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +232,8 @@ def main(argv=None):
                     help="batched form: batches per group (cig_apply_next_batches)")
     ap.add_argument("--group-steps", type=int, default=None, help="batched form: timed groups (0 = skip)")
     ap.add_argument("--autotune", action="store_true", help="pick the launch shape first (cig_autotune)")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (config 3 runs it by default)")
+    ap.add_argument("--no-multi-source", action="store_true", help="skip the 64-source engine (configs 1, 2)")
     ap.add_argument("--pr-push", action="store_true",
                     help="PageRank: scatter every round with atomics (keeps no incoming rows)")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving: arrival window (s)")
@@ -189,6 +258,10 @@ def main(argv=None):
     steps = args.steps if args.steps is not None else c["steps"]
     group = args.group or c["group"]
     group_steps = args.group_steps if args.group_steps is not None else (10 if args.config <= 2 else 1)
+    # at least 100 latency samples where batches are cheap (configs 1 and 2),
+    # taken after the timed steps when --steps asks for fewer
+    lat_extra = max(0, 100 - steps) if args.config in (1, 2) else 0
+    ms_batches = 20 if args.config in (1, 2) and not args.no_multi_source else 0
     pr = algo == "pr"
     mult = 2 if c["sym"] else 1  # directed records per requested update
 
@@ -197,7 +270,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     g = cg.rmat_graph(scale, ef, seed=args.seed, symmetric=c["sym"])
     log(f"graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f} s)")
-    nb = args.warmup + steps + group * group_steps
+    nb = args.warmup + steps + lat_extra + group * group_steps
     t0 = time.perf_counter()
     stream = cg.rmat_stream(g, scale, nb, batch, batch, seed=args.seed + 1, symmetric=c["sym"])
     log(f"stream: {nb} batches of {batch}+{batch}{' pairs' if c['sym'] else ''} "
@@ -217,7 +290,7 @@ def main(argv=None):
     eng = cg.Engine(algo, **kw)
     eng.load_graph(g)
     tune = None
-    if args.autotune and not pr:
+    if (args.autotune or c.get("autotune")) and not args.no_autotune and not pr:
         tune = eng.autotune(0)
         log(f"autotune: {tune}")
     else:
@@ -238,6 +311,9 @@ def main(argv=None):
             log(f"timed batches done: {k + 1}/{steps}")
     wall = time.perf_counter() - t0
     info = eng.info()
+    lat_stats = list(stats)
+    for _ in range(lat_extra):  # latency samples only: not in the timed value
+        lat_stats.append(eng.apply_next_batch(staged=args.staged))
 
     # Batched form: the next batches applied `group` at a time (one mutation
     # and one fixed point per group; same final state as one by one).
@@ -251,6 +327,8 @@ def main(argv=None):
         glat = [x["total_ms"] for x in gstats]
         gup = group_steps * group * 2 * batch * mult
         grouped = {
+            "form": "coalesced groups: one mutation and one fixed point per group; the states between "
+                    "the batches of a group are not produced",
             "batches_per_group": group,
             "groups": group_steps,
             "updates_per_s": round(gup / gwall, 1),
@@ -259,7 +337,7 @@ def main(argv=None):
         }
         log(f"batched form: {grouped}")
 
-    lat = [s["total_ms"] for s in stats]
+    lat = [s["total_ms"] for s in lat_stats]
     e_scan = sum(s["edges_scanned"] for s in stats)
     updates = steps * 2 * batch * mult
     gteps = e_scan / wall / 1e9
@@ -310,9 +388,15 @@ def main(argv=None):
             return 1
 
     eng.close()
+    multi = None
+    if ms_batches and algo in ("sssp", "bfs"):
+        multi = multi_source_form(g, stream, ms_batches, algo, args.weight_seed, percentile(lat, 50))
+        log(f"multi-source form: {multi}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(g, stream, nb, algo, args.weight_seed, args.cpu_seconds)
+        cpu["mutation_pool"] = cpu_mutation_pool(g, stream, nb, args.cpu_seconds / 4)
+        log(f"cpu mutation pool: {cpu['mutation_pool']}")
 
     default_shape = (args.scale, args.edge_factor, args.batch) == (None, None, None)
     out = {
@@ -335,7 +419,7 @@ def main(argv=None):
             "batch_inserts": batch, "symmetric_pairs": c["sym"], "mode": "staged" if args.staged else "fused",
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4),
-                       "mean": round(float(np.mean(lat)), 4), "max": round(max(lat), 4)},
+                       "mean": round(float(np.mean(lat)), 4), "max": round(max(lat), 4), "samples": len(lat)},
         "updates_per_s": round(updates / wall, 1),
         "edges_scanned_per_batch": int(e_scan / steps),
         "stage_ms_per_batch": {k: round(v, 4) for k, v in stage_ms.items()},
@@ -344,6 +428,7 @@ def main(argv=None):
         "rounds_per_batch": round(float(np.mean([s["rounds"] for s in stats])), 1),
         **({"pull_rounds_per_batch": round(float(np.mean([s["pull_rounds"] for s in stats])), 1)} if pr else {}),
         "batched_form": grouped,
+        "multi_source_form": multi,
         "roofline": {
             "kernel": kernel,
             "bound": "hbm",
@@ -359,7 +444,7 @@ def main(argv=None):
         },
         "cpu_baseline": cpu,
         "autotune": tune,
-        "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
+        "host": host_info(),
         "engine": {k: info[k] for k in ("block_size", "edges_per_lane", "split_degree", "cu_count",
                                         "hbm_pool_slots")},
     }
@@ -517,7 +602,7 @@ def serve_bench(args, c):
                       "host_edges_scanned": int(host_edges)},
         "roofline": None,
         "cpu_baseline": None,
-        "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
+        "host": host_info(),
         "engine": {k: info[k] for k in ("hbm_pool_slots", "host_pool_slots", "cached_rows", "cached_edges",
                                         "device_bytes", "host_pinned_bytes")},
     }

@@ -41,6 +41,9 @@ CFG_SCAN_REPAIR = 1 << 2
 CFG_KERNEL_TIMING = 1 << 3
 CFG_PR_PUSH_ONLY = 1 << 4
 CFG_SYMMETRIC = 1 << 5
+CFG_ORDERED_REPAIR = 1 << 6
+CFG_CC_UNION_REPAIR = 1 << 7
+CFG_GRAPH_BATCHES = 1 << 8
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
@@ -71,7 +74,7 @@ class Config(C.Structure):
         ("pr_max_rounds", C.c_uint32), ("pr_epsilon", C.c_float),
         ("tile_len", C.c_uint32), ("round_blocks_per_cu", C.c_uint32),
         ("pr_pull_divisor", C.c_uint32), ("pull_blocks_per_cu", C.c_uint32),
-        ("kernel_timing_every", C.c_uint32), ("reserved", C.c_uint32 * 2),
+        ("kernel_timing_every", C.c_uint32), ("repair_delta", C.c_uint32), ("reserved", C.c_uint32 * 1),
     ]
 
 
@@ -113,7 +116,8 @@ class TuneResult(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("max_group", C.c_uint32), ("max_query_sources", C.c_uint32),
-                ("hbm_reserve", C.c_uint64), ("reserved", C.c_uint64 * 4)]
+                ("hbm_reserve", C.c_uint64), ("refresh_every", C.c_uint32),
+                ("max_query_wait_ms", C.c_uint32), ("reserved", C.c_uint64 * 3)]
 
 
 class RequestInfo(C.Structure):
@@ -427,7 +431,8 @@ class Engine:
     def __init__(self, algo="sssp", weight_mode=WEIGHT_REF128, weight_seed=0, cache_bytes=0,
                  device=0, pr_epsilon=1e-6, pr_max_rounds=100, flags=0, edges_per_lane=0,
                  split_degree=0, max_rounds=0, n_stream=3, segments=1, tile_len=0,
-                 round_blocks_per_cu=0, pr_pull_divisor=0, pull_blocks_per_cu=0, kernel_timing_every=0):
+                 round_blocks_per_cu=0, pr_pull_divisor=0, pull_blocks_per_cu=0, kernel_timing_every=0,
+                 repair_delta=0):
         L = lib()
         cfg = Config()
         _check(L.cig_config_default(C.byref(cfg)))
@@ -449,6 +454,7 @@ class Engine:
         cfg.pr_pull_divisor = pr_pull_divisor
         cfg.pull_blocks_per_cu = pull_blocks_per_cu
         cfg.kernel_timing_every = kernel_timing_every
+        cfg.repair_delta = repair_delta
         self.config = cfg
         self._e = C.c_void_p()
         st = L.cig_create(C.byref(cfg), C.byref(self._e))
@@ -456,6 +462,7 @@ class Engine:
             raise CigError(st, L.cig_last_error(None).decode())
         self.algo = cfg.algo
         self.n = 0
+        self.nsrc = 1
         self._server = None
         _LIVE.add(self)
 
@@ -489,9 +496,12 @@ class Engine:
         self.n = g.n
 
     # InitGraph
-    def init_graph(self, source: int = 0):
-        src = np.array([source], dtype=np.uint32)
-        self._call(lib().cig_init(self._e, _ptr(src), 1))
+    def init_graph(self, source=0):
+        """One source (an int), or a list of 2..64 BFS/SSSP sources kept
+        current together; values()/parents() are then (nsrc, n) arrays."""
+        src = np.atleast_1d(np.asarray(source, dtype=np.uint32))
+        self._call(lib().cig_init(self._e, _ptr(src), len(src)))
+        self.nsrc = len(src) if self.algo in (BFS, SSSP) else 1
 
     # Start / StartPageRank
     def start(self) -> dict:
@@ -594,6 +604,8 @@ class Engine:
             out = np.empty(self.n, dtype=np.uint8)
         elif self.algo == PR and field in (FIELD_VALUE, FIELD_RESIDUAL):
             out = np.empty(self.n, dtype=np.float32)
+        elif self.nsrc > 1 and field in (FIELD_VALUE, FIELD_PARENT):
+            out = np.empty((self.nsrc, self.n), dtype=np.uint32)  # source-major
         else:
             out = np.empty(self.n, dtype=np.uint32)
         self._call(lib().cig_gather(self._e, field, _ptr(out), out.nbytes))
@@ -630,12 +642,17 @@ class Server:
     """The engine's serving loop (cig_serve_*): submit updates and queries
     from any thread; results come back in submission order semantics."""
 
-    def __init__(self, engine: "Engine", max_group=32, max_query_sources=64, hbm_reserve=1 << 30):
+    def __init__(self, engine: "Engine", max_group=32, max_query_sources=64, hbm_reserve=1 << 30,
+                 refresh_every=None, max_query_wait_ms=None):
         cfg = ServeConfig()
         lib().cig_serve_config_default(C.byref(cfg))
         cfg.max_group = max_group
         cfg.max_query_sources = max_query_sources
         cfg.hbm_reserve = hbm_reserve
+        if refresh_every is not None:
+            cfg.refresh_every = refresh_every
+        if max_query_wait_ms is not None:
+            cfg.max_query_wait_ms = max_query_wait_ms
         self._s = C.c_void_p()
         self._eng = engine
         engine._call(lib().cig_serve_start(engine._e, C.byref(cfg), C.byref(self._s)))

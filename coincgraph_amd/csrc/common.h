@@ -75,6 +75,8 @@ struct Ctl {
     uint32_t pull_rounds;   // PR rounds of the current call run in pull form
     uint32_t amend_pull;    // PR batch amendment gathers (1) or scatters (amend_push = 1)
     uint32_t amend_push;
+    uint32_t bmin[3];       // delta-ordered rounds: smallest bucket of round slot k's frontier
+    uint32_t ufsel[2];      // union-find repair: the sampled component left out of hooking, its votes
     uint64_t pool_tail[2]; // bump tails of the HBM (0) and host (1) pools
     uint64_t pool_cap[2];
     uint64_t garbage[2];   // slots abandoned by relocations

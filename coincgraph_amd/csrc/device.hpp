@@ -149,9 +149,13 @@ struct RelaxOp {
     uint32_t *hot;
     uint64_t seed;
     uint32_t tag;
+    // Delta-ordered rounds: only rows whose value / delta is at most `bucket`
+    // expand; the rest were deferred to the next round (delta 0: unordered).
+    uint32_t delta = 0, bucket = kInf;
     __device__ bool source(uint32_t v, uint32_t, Row &row, uint32_t &pay) const {
         const uint32_t val = uint32_t(state[v] >> 32);
         if (val == kInf) return false;
+        if (delta && val / delta > bucket) return false;
         row = rows[v];
         pay = val;
         if (hot) hot[v] += 1;

@@ -34,6 +34,8 @@
 #include "common.h"
 #include "host_par.h"
 #include "kernels.hpp"
+#include "msi.hpp"
+#include "ccuf.hpp"
 
 using namespace cig;
 using namespace cig::dev;
@@ -125,6 +127,74 @@ struct HBuf {  // pinned host memory (hipHostMalloc; never hipHostRegister)
     }
 };
 
+// A stream and two pinned bounce buffers that move device results into
+// caller (pageable) memory: the device copies chunk i into one buffer while
+// host threads move chunk i-1 out of the other. The engine's lane uses the
+// engine stream; the serving loop has a second lane, so that a query's
+// results leave the device while the next update group runs.
+struct CopyLane {
+    static constexpr uint64_t kChunk = uint64_t(256) << 20;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    HBuf<uint8_t> bounce[2];
+    hipEvent_t ev[2] = {};
+    CopyLane() = default;
+    CopyLane(const CopyLane &) = delete;
+    CopyLane &operator=(const CopyLane &) = delete;
+    ~CopyLane() { release(); }
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &e : ev) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+        for (auto &b : bounce) b.release();
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+        own_stream = false;
+    }
+    // Uses `s`, or a new stream of its own when s is null.
+    void init(hipStream_t s) {
+        if (s) {
+            stream = s;
+        } else {
+            HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            own_stream = true;
+        }
+    }
+    // Copies `bytes` from device memory to `out`; returns with the copy complete.
+    void copy(void *out, const void *dev, uint64_t bytes) {
+        if (bytes <= kChunk) {
+            HIPCHK(hipMemcpyAsync(out, dev, bytes, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            return;
+        }
+        if (!bounce[0].p) {
+            for (int b = 0; b < 2; ++b) {
+                bounce[b].alloc(kChunk);
+                HIPCHK(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+            }
+        }
+        const unsigned T = pick_threads(0);
+        auto host_copy = [&](uint64_t chunk) {
+            const uint64_t off = chunk * kChunk, len = std::min(kChunk, bytes - off);
+            HIPCHK(hipEventSynchronize(ev[chunk & 1]));
+            uint8_t *to = static_cast<uint8_t *>(out) + off;
+            const uint8_t *from = bounce[chunk & 1].p;
+            parallel_for(len, T, [&](uint64_t lo, uint64_t hi, unsigned) { std::memcpy(to + lo, from + lo, hi - lo); });
+        };
+        const uint64_t chunks = (bytes + kChunk - 1) / kChunk;
+        for (uint64_t i = 0; i < chunks; ++i) {
+            const uint64_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+            HIPCHK(hipMemcpyAsync(bounce[i & 1].p, static_cast<const uint8_t *>(dev) + off, len,
+                                  hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipEventRecord(ev[i & 1], stream));
+            if (i > 0) host_copy(i - 1);  // frees buffer (i-1)&1 for chunk i+1
+        }
+        host_copy(chunks - 1);
+    }
+};
+
 constexpr uint32_t kTileLenDefault = 512;
 constexpr uint32_t kSplitDefault = 64;
 constexpr uint32_t kQplDefault = 2;  // quads (4 slots) per lane and step
@@ -188,9 +258,14 @@ struct cig_engine {
     // multi-source queries
     DBuf<uint32_t> ms_dist, ms_stamp, ms_out;
     uint8_t *scratch_dev = nullptr;  // this device's copy of the no-allocation scratch array
+    // incremental multi-source state (cig_init with 2..64 BFS/SSSP sources)
+    uint32_t msk = 0;  // sources kept; 0: the single-source state word
+    std::vector<uint32_t> ms_sources;
+    DBuf<unsigned long long> msi_st, msi_act, msi_snap, msi_aff, msi_pinv;
+    // CC by union-find (CIG_CFG_CC_UNION_REPAIR)
+    DBuf<uint32_t> uf, uroot;
     DBuf<unsigned long long> ms_act, ms_snap, ms_seen;
-    HBuf<uint8_t> bounce[2];  // pinned bounce buffers of large device-to-host results
-    hipEvent_t bounce_ev[2] = {};
+    CopyLane lane;  // large device-to-host results on the engine stream
     DBuf<uint2> hcnts;
     DBuf<DelTile> dtiles;
     uint64_t rec_cap = 0;
@@ -241,6 +316,7 @@ struct cig_engine {
         cus = prop.multiProcessorCount;
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         scratch_dev = scratch_address(cfg.device);
+        lane.init(stream);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         if (cfg.flags & CIG_CFG_KERNEL_TIMING) {
             rev_begin.resize(kRoundEvents);
@@ -597,7 +673,13 @@ struct cig_engine {
         a.cfg = ecfg(kStagePropagate);
         a.seed = cfg.weight_seed;
         a.max_rounds = max_rounds;
+        a.delta = ordered() ? (cfg.repair_delta ? cfg.repair_delta : 128u) : 0u;
         return a;
+    }
+
+    // Delta-ordered relaxation rounds: SSSP with chained rounds only.
+    bool ordered() const {
+        return (cfg.flags & CIG_CFG_ORDERED_REPAIR) && cfg.algo == CIG_ALGO_SSSP && !coop;
     }
 
     // Enqueues round kernels back to back in growing chunks (8, 16, ... 128)
@@ -700,6 +782,18 @@ struct cig_engine {
                     launch_coop(kernel, a, grid);
                 }
             }
+        } else if (ordered()) {
+            LoopArgs a = loop_args(1);
+            const uint64_t bound = cfg.max_rounds ? cfg.max_rounds : round_bound();
+            k_ord_reset<<<1, 64, 0, stream>>>(ctl.p);
+            advance_round(run_chain(
+                [&](uint64_t tag, uint32_t seq) {
+                    k_ord_min<<<grid_round, kBlock, 0, stream>>>(a, seq);
+                    timed_round([&] {
+                        k_relax_round<W, S, QPL, kBlock, true><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                    });
+                },
+                bound, "ordered relax fixed point"));
         } else {
             LoopArgs a = loop_args(1);
             const uint64_t bound = cfg.max_rounds ? cfg.max_rounds : round_bound();
@@ -831,14 +925,24 @@ struct cig_engine {
     // ------------------------------------------------------------ init
     void init(const uint32_t *sources, uint32_t nsrc) {
         if (!loaded) throw HipError(CIG_ESTATE, "cig_load_csr must come first");
+        flush_finish();  // the last batch's clear needs its counters, which the reset zeroes
         reset_ctl(false);
         if (is_pr()) {
             k_init_pr<<<grid_small, 256, 0, stream>>>(rank.p, comp.p, res.p, stamp.p, q1.p, n, ctl.p);
         } else if (cfg.algo == CIG_ALGO_CC) {
             k_init_traversal<<<grid_small, 256, 0, stream>>>(state.p, stamp.p, n, 1, 0, q1.p, ctl.p);
         } else {
-            if (!sources || nsrc != 1) throw HipError(CIG_EINVAL, "SSSP/BFS take exactly one source");
-            if (sources[0] >= n) throw HipError(CIG_EINVAL, "source out of range");
+            if (!sources || nsrc < 1 || nsrc > CIG_MAX_SOURCES)
+                throw HipError(CIG_EINVAL, "SSSP/BFS take 1..64 sources");
+            for (uint32_t i = 0; i < nsrc; ++i)
+                if (sources[i] >= n) throw HipError(CIG_EINVAL, "source out of range");
+            if (nsrc > 1) {
+                msi_init(sources, nsrc);
+                sync();
+                initialized = true;
+                return;
+            }
+            msi_release();
             source = sources[0];
             fill32(stamp.p, n, 0);
             k_init_traversal<<<grid_small, 256, 0, stream>>>(state.p, stamp.p, n, 0, source, q1.p, ctl.p);
@@ -924,7 +1028,8 @@ struct cig_engine {
 
     // ------------------------------------------------------------ mutation
     MutArgs mut_args(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na) {
-        MutArgs m{};
+        MutArgs m;
+        std::memset(&m, 0, sizeof(m));  // padding too: graph signatures compare the bytes
         m.ctl = ctl.p;
         m.rows = rows.p;
         m.pools = pools();
@@ -1001,6 +1106,7 @@ struct cig_engine {
     void flush_finish() {
         if (!finish_pending) return;
         finish_pending = false;
+        if (msk) msi_clear();
         mutate_finish(pending_m, pending_aff);
     }
 
@@ -1177,7 +1283,8 @@ struct cig_engine {
 
     // ------------------------------------------------------------ traversal batch
     InvalArgs inval_args(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na) {
-        InvalArgs x{};
+        InvalArgs x;
+        std::memset(&x, 0, sizeof(x));
         x.ctl = ctl.p;
         x.rows = rows.p;
         x.in_rows = in_rows_p();
@@ -1279,9 +1386,135 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
     }
 
+    // ------------------------------------------------ graph-replayed batches
+    // CIG_CFG_GRAPH_BATCHES: the fixed-length parts of a batch run as two
+    // captured HIP graphs, replayed for every batch: (1) the mutation and the
+    // invalidation seeds, (2) the reset of the affected vertices, the pull
+    // repair and the insertion seeds. Their kernels read the batch's records
+    // through a BatchRef that one small kernel fills before each replay; the
+    // round chains between and after them stay host-driven (their length
+    // depends on the data). A graph is captured again when any buffer or
+    // launch shape its arguments name has changed (the signature below).
+    struct Captured {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        std::vector<uint8_t> sig;
+        void reset() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            x = nullptr;
+            g = nullptr;
+            sig.clear();
+        }
+        ~Captured() { reset(); }
+    };
+    Captured graph_mut, graph_repair;
+    DBuf<BatchRef> bref;
+    uint64_t graph_captures = 0, graph_replays = 0;
+
+    bool graph_mode(bool staged) const {
+        return (cfg.flags & CIG_CFG_GRAPH_BATCHES) && !coop && !msk && !uf_mode() && !is_pr() && !staged &&
+               !cur_skip && pull_rows();
+    }
+
+    template <class T>
+    static void sig_add(std::vector<uint8_t> &sig, const T &x) {
+        const uint8_t *b = reinterpret_cast<const uint8_t *>(&x);
+        sig.insert(sig.end(), b, b + sizeof(T));
+    }
+
+    template <class F>
+    void graph_run(Captured &c, const std::vector<uint8_t> &sig, F body) {
+        if (!c.x || c.sig != sig) {
+            c.reset();
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(stream, &c.g));
+            HIPCHK(hipGraphInstantiate(&c.x, c.g, nullptr, nullptr, 0));
+            c.sig = sig;
+            ++graph_captures;
+        }
+        HIPCHK(hipGraphLaunch(c.x, stream));
+        ++graph_replays;
+    }
+
+    template <int W, bool S, int QPL>
+    void graph_repair_body(const InvalArgs &ia) {
+        InvalArgs a = ia;
+        a.cfg = ecfg(kStageRepair);
+        k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
+        k_repair_pull<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
+        k_repair_pull_tiles<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
+        k_insert_seed<W, S><<<grid_small, 256, 0, stream>>>(ia, 1u);  // no vertex is affected without deletes
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W, bool S>
+    void traversal_batch_graph(const cig_edge *d_dels, uint32_t nd, const cig_edge *d_adds, uint32_t na,
+                               cig_batch_stats *st) {
+        begin_call(st);
+        if (!bref.p) bref.alloc(1);
+        k_set_batch<<<1, 64, 0, stream>>>(bref.p, d_dels, nd, d_adds, na);
+        HIPCHK(hipGetLastError());
+        MutArgs m = mut_args(d_dels, nd, d_adds, na);
+        InvalArgs ia = inval_args(d_dels, nd, d_adds, na);
+        // the captured launches: every kernel (nd, na > 0) reading the records through bref
+        MutArgs mc = mut_args(nullptr, 1, nullptr, 1);
+        mc.ref = bref.p;
+        InvalArgs ic = inval_args(nullptr, 1, nullptr, 1);
+        ic.ref = bref.p;
+        std::vector<uint8_t> sig;
+        sig_add(sig, mc);
+        sig_add(sig, ic);
+        sig_add(sig, grid_small);
+        sig_add(sig, qpl);
+        graph_run(graph_mut, sig, [&] {
+            mutate_group(mc);
+            mutate_apply(mc);
+            k_inval_seed<<<grid_small, 256, 0, stream>>>(ic);
+            HIPCHK(hipGetLastError());
+        });
+        mark(1);
+        if (nd) {
+            run_chain([&](uint64_t tag, uint32_t seq) {
+                          k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(ia, prog_dev, tag, seq);
+                      },
+                      round_bound(), "invalidation");
+        }
+        mark(2);
+        graph_run(graph_repair, sig, [&] {
+            switch (qpl) {
+            case 1: graph_repair_body<W, S, 1>(ic); break;
+            case 4: graph_repair_body<W, S, 4>(ic); break;
+            default: graph_repair_body<W, S, 2>(ic); break;
+            }
+        });
+        mark(3);
+        mark(5);
+        relax();
+        mark(4);
+        end_call(st, 4);
+        if (na && hctl.p->overflow_cnt) {
+            recover_overflow(m);
+            ia.ref = bref.p;
+            insert_seed<W, S>(ia, false);
+            relax();
+            end_call(st, 4);
+        }
+        defer_finish(m, nd != 0);
+    }
+
     template <int W, bool S>
     void traversal_batch(const cig_edge *d_dels, uint32_t nd, const cig_edge *d_adds, uint32_t na,
                          bool staged, cig_batch_stats *st) {
+        if (graph_mode(staged)) return traversal_batch_graph<W, S>(d_dels, nd, d_adds, na, st);
         begin_call(st);
         if (staged && nd && na) {
             // stage 1: deletes to their fixed point, then stage 2: adds
@@ -1350,12 +1583,311 @@ struct cig_engine {
 
     void traversal_call(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, bool staged,
                         cig_batch_stats *st) {
+        if (uf_mode()) return uf_batch(d, nd, a, na, st);
+        if (msk) {
+            switch (weight_kind()) {
+            case 0: msi_batch<0>(d, nd, a, na, staged, st); break;
+            case 1: msi_batch<1>(d, nd, a, na, staged, st); break;
+            default: msi_batch<2>(d, nd, a, na, staged, st); break;
+            }
+            return;
+        }
         switch (weight_kind()) {
         case 0: traversal_batch<0, false>(d, nd, a, na, staged, st); break;
         case 1: traversal_batch<1, false>(d, nd, a, na, staged, st); break;
         case 2: traversal_batch<2, false>(d, nd, a, na, staged, st); break;
         default: traversal_batch<3, true>(d, nd, a, na, staged, st); break;
         }
+    }
+
+    // ------------------------------------------------ CC by union-find
+    // (ccuf.hpp): labels re-derived per touched component, inserts united.
+    bool uf_mode() const { return (cfg.flags & CIG_CFG_CC_UNION_REPAIR) && cfg.algo == CIG_ALGO_CC; }
+
+    UfArgs uf_args(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na) {
+        if (uf.count < n) {
+            uf.alloc(n);
+            uroot.alloc(n);
+        }
+        UfArgs x{};
+        x.ctl = ctl.p;
+        x.rows = rows.p;
+        x.state = state.p;
+        x.uf = uf.p;
+        x.roots = uroot.p;
+        x.marks = aff.p;
+        x.alist = alist.p;
+        x.tiles = t0.p;
+        x.cfg = ecfg(kStageRepair);
+        x.dels = d;
+        x.nd = nd;
+        x.adds = a;
+        x.na = na;
+        x.add_skip = cur_skip;
+        x.del_skip = cur_skip ? cur_skip + na : nullptr;
+        x.n = n;
+        return x;
+    }
+
+    // Sample, snapshot, select, hook and flatten the listed vertices.
+    template <int QPL>
+    void uf_derive_qpl(const UfArgs &x) {
+        k_uf_sample<<<grid_small, 256, 0, stream>>>(x);
+        k_uf_snapshot<<<grid_small, 256, 0, stream>>>(x);
+        k_uf_select<<<1, 256, 0, stream>>>(x);
+        mark(8);
+        k_uf_hook<QPL><<<grid_small, kBlock, 0, stream>>>(x);
+        mark(9);
+        scan_timed = stages_now;
+        k_uf_hook_tiles<QPL><<<grid_small, kBlock, 0, stream>>>(x);
+        k_uf_flatten<<<grid_small, 256, 0, stream>>>(x);
+        HIPCHK(hipGetLastError());
+    }
+
+    void uf_derive(const UfArgs &x) {
+        switch (qpl) {
+        case 1: uf_derive_qpl<1>(x); break;
+        case 4: uf_derive_qpl<4>(x); break;
+        default: uf_derive_qpl<2>(x); break;
+        }
+    }
+
+    // The initial labels: every vertex affected.
+    void uf_all() {
+        reset_ctl(false);  // drops the label-propagation frontier cig_init left
+        UfArgs x = uf_args(nullptr, 0, nullptr, 0);
+        k_uf_init<<<grid_small, 256, 0, stream>>>(x);
+        uf_derive(x);
+    }
+
+    void uf_batch(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, cig_batch_stats *st) {
+        begin_call(st);
+        MutArgs m = mut_args(d, nd, a, na);
+        mutate_group(m);
+        mutate_apply(m);
+        mark(1);
+        const UfArgs x = uf_args(d, nd, a, na);
+        if (nd) {
+            k_uf_mark<<<grid_small, 256, 0, stream>>>(x);
+            k_uf_collect<<<grid_small, 256, 0, stream>>>(x);
+            uf_derive(x);
+        }
+        mark(2);
+        mark(3);
+        if (na) {
+            k_uf_union_adds<<<grid_small, 256, 0, stream>>>(x);
+            k_uf_relabel<<<grid_small, 256, 0, stream>>>(x);
+            HIPCHK(hipGetLastError());
+        }
+        mark(5);
+        mark(4);
+        end_call(st, 4);
+        if (na && hctl.p->overflow_cnt) {  // the labels are final; only the rows need room
+            recover_overflow(m);
+            end_call(st, 4);
+        }
+        defer_finish(m, false);
+    }
+
+    // ------------------------------------------------ multi-source maintenance
+    // Up to 64 BFS/SSSP sources kept current through every batch (msi.hpp):
+    // the single-source steps, each run once for all sources.
+    MsiArgs msi_args(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na) {
+        MsiArgs x{};
+        x.ctl = ctl.p;
+        x.rows = rows.p;
+        x.in_rows = in_rows_p();
+        x.st = msi_st.p;
+        x.act = msi_act.p;
+        x.snap = msi_snap.p;
+        x.affm = msi_aff.p;
+        x.pinv = msi_pinv.p;
+        x.stamp = stamp.p;
+        x.q0 = q0.p;
+        x.q1 = q1.p;
+        x.t0 = t0.p;
+        x.t1 = t1.p;
+        x.alist = alist.p;
+        x.cfg = ecfg(kStagePropagate);
+        x.dels = d;
+        x.nd = nd;
+        x.adds = a;
+        x.na = na;
+        x.add_skip = cur_skip;
+        x.del_skip = cur_skip ? cur_skip + na : nullptr;
+        x.n = n;
+        x.k = msk;
+        x.seed = cfg.weight_seed;
+        return x;
+    }
+
+    void msi_release() {
+        msk = 0;
+        ms_sources.clear();
+        msi_st.release();
+        msi_act.release();
+        msi_snap.release();
+        msi_aff.release();
+        msi_pinv.release();
+    }
+
+    void msi_init(const uint32_t *sources, uint32_t k) {
+        if (coop) throw HipError(CIG_EINVAL, "several sources run chained rounds: drop CIG_CFG_COOPERATIVE");
+        if (!pull_rows())
+            throw HipError(CIG_EINVAL, "several sources need the incoming-row index: drop CIG_CFG_SCAN_REPAIR");
+        msi_release();
+        msi_st.alloc(uint64_t(n) * k);
+        msi_act.alloc(n);
+        msi_snap.alloc(n);
+        msi_aff.alloc(n);
+        msi_pinv.alloc(n);
+        fill32(reinterpret_cast<uint32_t *>(msi_st.p), uint64_t(n) * k * 2, kInf);
+        for (auto *b : {&msi_act, &msi_snap, &msi_aff, &msi_pinv})
+            fill32(reinterpret_cast<uint32_t *>(b->p), uint64_t(n) * 2, 0);
+        fill32(stamp.p, n, 0);
+        msk = k;
+        ms_sources.assign(sources, sources + k);
+        SrcList list{};
+        std::memcpy(list.v, sources, size_t(k) * 4);
+        k_msi_init<<<1, 64, 0, stream>>>(msi_args(nullptr, 0, nullptr, 0), list);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W, int QPL>
+    void msi_relax_qpl(uint32_t grid) {
+        MsiArgs a = msi_args(nullptr, 0, nullptr, 0);
+        advance_round(run_chain(
+            [&](uint64_t tag, uint32_t seq) {
+                k_msi_consume<<<grid_small, 256, 0, stream>>>(a, a.act, prog_dev, tag, seq);
+                timed_round([&] { k_msi_expand<W, false, QPL><<<grid, kBlock, 0, stream>>>(a, seq); });
+                k_msi_tiles<W, false, QPL><<<grid, kBlock, 0, stream>>>(a, seq);
+            },
+            round_bound(), "multi-source fixed point"));
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W>
+    void msi_relax_w(uint32_t grid) {
+        switch (qpl) {
+        case 1: msi_relax_qpl<W, 1>(grid); break;
+        case 4: msi_relax_qpl<W, 4>(grid); break;
+        default: msi_relax_qpl<W, 2>(grid); break;
+        }
+    }
+
+    void msi_relax(uint32_t grid) {
+        switch (weight_kind()) {
+        case 0: msi_relax_w<0>(grid); break;
+        case 1: msi_relax_w<1>(grid); break;
+        default: msi_relax_w<2>(grid); break;
+        }
+    }
+
+    // Invalidation waves to their fixed point, then the reset that also lists
+    // every affected vertex.
+    template <int QPL>
+    void msi_invalidate(const MsiArgs &x) {
+        MsiArgs a = x;
+        a.cfg = ecfg(kStageInvalidate);
+        k_msi_inval_seed<<<grid_small, 256, 0, stream>>>(a);
+        HIPCHK(hipGetLastError());
+        advance_round(run_chain(
+            [&](uint64_t tag, uint32_t seq) {
+                k_msi_consume<<<grid_small, 256, 0, stream>>>(a, a.pinv, prog_dev, tag, seq);
+                k_msi_expand<0, true, QPL><<<grid_round, kBlock, 0, stream>>>(a, seq);
+                k_msi_tiles<0, true, QPL><<<grid_round, kBlock, 0, stream>>>(a, seq);
+            },
+            round_bound(), "multi-source invalidation"));
+        k_msi_reset<<<grid_small, 256, 0, stream>>>(a);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W, int QPL>
+    void msi_repair(const MsiArgs &x) {
+        MsiArgs a = x;
+        a.cfg = ecfg(kStageRepair);
+        mark(8);
+        k_msi_repair<W, QPL><<<grid_small, kBlock, 0, stream>>>(a);
+        mark(9);
+        scan_timed = stages_now;
+        k_msi_repair_tiles<W, QPL><<<grid_small, kBlock, 0, stream>>>(a);
+        k_msi_repair_done<<<grid_small, 256, 0, stream>>>(a);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W>
+    void msi_deletes(const MsiArgs &x) {
+        switch (qpl) {
+        case 1: msi_invalidate<1>(x); mark(2); msi_repair<W, 1>(x); break;
+        case 4: msi_invalidate<4>(x); mark(2); msi_repair<W, 4>(x); break;
+        default: msi_invalidate<2>(x); mark(2); msi_repair<W, 2>(x); break;
+        }
+    }
+
+    template <int W>
+    void msi_insert_seed(const MsiArgs &x, bool skip_affected) {
+        k_msi_insert_seed<W><<<grid_small, 256, 0, stream>>>(x, skip_affected ? 1u : 0u);
+        HIPCHK(hipGetLastError());
+    }
+
+    void msi_clear() {
+        k_msi_clear<<<grid_small, 256, 0, stream>>>(msi_args(nullptr, 0, nullptr, 0));
+        HIPCHK(hipGetLastError());
+    }
+
+    template <int W>
+    void msi_batch(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, bool staged,
+                   cig_batch_stats *st) {
+        begin_call(st);
+        if (staged && nd && na) {
+            // stage 1: the deletes to their fixed point
+            MutArgs m = mut_args(d, nd, nullptr, 0);
+            mutate_group(m);
+            mutate_apply(m);
+            const MsiArgs xd = msi_args(d, nd, nullptr, 0);
+            msi_deletes<W>(xd);
+            msi_relax(grid_round);
+            msi_clear();
+            mutate_finish(m, false);
+            // stage 2: the adds
+            k_reset_mut<<<1, 64, 0, stream>>>(ctl.p);
+            MutArgs ma = mut_args(nullptr, 0, a, na);
+            mutate_group(ma);
+            mutate_apply(ma);
+            const MsiArgs xa = msi_args(nullptr, 0, a, na);
+            msi_insert_seed<W>(xa, false);
+            msi_relax(grid_round);
+            mark(4);
+            end_call(st, 0);
+            if (hctl.p->overflow_cnt) {
+                recover_overflow(ma);
+                msi_insert_seed<W>(xa, false);
+                msi_relax(grid_round);
+                end_call(st, 0);
+            }
+            mutate_finish(ma);
+            return;
+        }
+        MutArgs m = mut_args(d, nd, a, na);
+        mutate_group(m);
+        mutate_apply(m);
+        mark(1);
+        const MsiArgs x = msi_args(d, nd, a, na);
+        if (nd) msi_deletes<W>(x);
+        else mark(2);
+        mark(3);
+        if (na) msi_insert_seed<W>(x, nd != 0);
+        mark(5);
+        msi_relax(grid_round);
+        mark(4);
+        end_call(st, 4);
+        if (na && hctl.p->overflow_cnt) {
+            recover_overflow(m);
+            msi_insert_seed<W>(x, false);
+            msi_relax(grid_round);
+            end_call(st, 4);
+        }
+        defer_finish(m, false);
     }
 
     // ------------------------------------------------------------ PR batch
@@ -1444,7 +1976,10 @@ struct cig_engine {
         check_ready();
         begin_call(st);
         for (int k : {1, 2, 3, 5}) mark(k);
-        if (is_pr()) pr_converge(); else relax();
+        if (is_pr()) pr_converge();
+        else if (msk) msi_relax(grid_small);
+        else if (uf_mode()) uf_all();
+        else relax();
         mark(4);
         if (is_pr()) pr_residual_norm();
         end_call(st, 4);
@@ -1844,6 +2379,14 @@ struct cig_engine {
             placed += outs[i].nsrc;
         }
         if (placed != k) throw HipError(CIG_EINVAL, "output buffers must cover every source");
+        query_compute(algo, sources, k, st);
+        query_emit(k, outs, nouts, lane);
+    }
+
+    // The rounds of a query, to its fixed point in ms_dist (vertex-major).
+    // The results stay there until the next query; query_emit moves them out.
+    void query_compute(uint32_t algo, const uint32_t *sources, uint32_t k, cig_batch_stats *st) {
+        query_check(algo, sources, k);
         const uint32_t stride = (k + 3u) & ~3u;
         if (ms_dist.count < uint64_t(n) * stride) ms_dist.alloc(uint64_t(n) * stride);
         if (ms_act.count < n) {
@@ -1861,14 +2404,23 @@ struct cig_engine {
         fill32(reinterpret_cast<uint32_t *>(ms_snap.p), uint64_t(n) * 2, 0);
         if (bfs) fill32(reinterpret_cast<uint32_t *>(ms_seen.p), uint64_t(n) * 2, 0);
         query_run(algo, sources, k, QueryMem{ms_dist.p, ms_act.p, ms_snap.p, ms_seen.p, ms_stamp.p});
-        k_ms_gather<<<grid_small, 256, 0, stream>>>(ms_dist.p, n, stride, k, ms_out.p);
+        end_call(st, 4);
+    }
+
+    // Source-major results of the last query_compute of k sources into the
+    // callers' buffers, on lane L's stream. Touches only ms_dist and ms_out,
+    // so the serving loop runs it on its own lane while the engine stream
+    // serves the next update group.
+    void query_emit(uint32_t k, const QueryOut *outs, size_t nouts, CopyLane &L) {
+        const uint32_t stride = (k + 3u) & ~3u;
+        k_ms_gather<<<grid_small, 256, 0, L.stream>>>(ms_dist.p, n, stride, k, ms_out.p);
         HIPCHK(hipGetLastError());
         uint64_t at = 0;
         for (size_t i = 0; i < nouts; ++i) {
-            copy_out(outs[i].p, ms_out.p + at, uint64_t(n) * outs[i].nsrc * 4);
+            L.copy(outs[i].p, ms_out.p + at, uint64_t(n) * outs[i].nsrc * 4);
             at += uint64_t(n) * outs[i].nsrc;
         }
-        end_call(st, 4);
+        HIPCHK(hipStreamSynchronize(L.stream));
     }
 
     // Byte offsets of a run's working memory inside the no-allocation scratch
@@ -1918,40 +2470,7 @@ struct cig_engine {
         end_call(st, 4);
     }
 
-    // Device-to-host copy into caller (pageable) memory. Large results go
-    // through two pinned bounce buffers: the device copies chunk i while host
-    // threads move chunk i-1 into `out`. Returns with the copy complete.
-    void copy_out(void *out, const void *dev, uint64_t bytes) {
-        constexpr uint64_t kChunk = uint64_t(256) << 20;
-        if (bytes <= kChunk) {
-            HIPCHK(hipMemcpyAsync(out, dev, bytes, hipMemcpyDeviceToHost, stream));
-            sync();
-            return;
-        }
-        if (!bounce[0].p) {
-            for (int b = 0; b < 2; ++b) {
-                bounce[b].alloc(kChunk);
-                HIPCHK(hipEventCreateWithFlags(&bounce_ev[b], hipEventDisableTiming));
-            }
-        }
-        const unsigned T = pick_threads(0);
-        auto host_copy = [&](uint64_t chunk) {
-            const uint64_t off = chunk * kChunk, len = std::min(kChunk, bytes - off);
-            HIPCHK(hipEventSynchronize(bounce_ev[chunk & 1]));
-            uint8_t *to = static_cast<uint8_t *>(out) + off;
-            const uint8_t *from = bounce[chunk & 1].p;
-            parallel_for(len, T, [&](uint64_t lo, uint64_t hi, unsigned) { std::memcpy(to + lo, from + lo, hi - lo); });
-        };
-        const uint64_t chunks = (bytes + kChunk - 1) / kChunk;
-        for (uint64_t i = 0; i < chunks; ++i) {
-            const uint64_t off = i * kChunk, len = std::min(kChunk, bytes - off);
-            HIPCHK(hipMemcpyAsync(bounce[i & 1].p, static_cast<const uint8_t *>(dev) + off, len,
-                                  hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipEventRecord(bounce_ev[i & 1], stream));
-            if (i > 0) host_copy(i - 1);  // frees buffer (i-1)&1 for chunk i+1
-        }
-        host_copy(chunks - 1);
-    }
+    void copy_out(void *out, const void *dev, uint64_t bytes) { lane.copy(out, dev, bytes); }
 
     // ------------------------------------------------------------ cache
     // Sort-based admission (compute_hot_vertices + confirm_candidate_batch,
@@ -2061,6 +2580,16 @@ struct cig_engine {
         }
         if (field != CIG_FIELD_VALUE && field != CIG_FIELD_PARENT)
             throw HipError(CIG_EINVAL, "field not available for traversal");
+        if (msk) {  // source-major: out[s * n + v]
+            const uint64_t words = uint64_t(n) * msk;
+            if (bytes < words * 4) throw HipError(CIG_EINVAL, "output buffer too small (nsrc * n words)");
+            if (ms_out.count < words) ms_out.alloc(words);
+            k_msi_gather<<<grid_small, 256, 0, stream>>>(msi_st.p, n, msk, field == CIG_FIELD_VALUE ? 0u : 1u,
+                                                         ms_out.p);
+            HIPCHK(hipGetLastError());
+            copy_out(out, ms_out.p, words * 4);
+            return;
+        }
         std::vector<uint64_t> hs(n);
         HIPCHK(hipMemcpy(hs.data(), state.p, size_t(n) * 8, hipMemcpyDeviceToHost));
         uint32_t *o = static_cast<uint32_t *>(out);
@@ -2132,8 +2661,7 @@ struct cig_engine {
                 if (e) (void)hipEventDestroy(e);
             for (auto &e : rev_begin) (void)hipEventDestroy(e);
             for (auto &e : rev_end) (void)hipEventDestroy(e);
-            for (auto &e : bounce_ev)
-                if (e) (void)hipEventDestroy(e);
+            lane.release();
             (void)hipStreamDestroy(stream);
         }
     }
@@ -2184,9 +2712,14 @@ cig_status guarded(cig_engine *e, F f) {
 }  // namespace
 
 // ------------------------------------------------------------------ serving
-// One server thread owns the engine (see cig.h). Requests are served in
-// submission order; consecutive update requests form one group, consecutive
-// queries of one algorithm one multi-source run.
+// One server thread owns the engine (see cig.h). Update requests are served
+// in submission order, up to max_group per group; queries are served when no
+// update is waiting (or once the oldest has waited max_query_wait_ms), up to
+// 64 sources of one algorithm merged into one run. A query's rounds run on
+// the engine stream; its results then leave the device on the server's own
+// copy lane (a second stream and a host thread) while the next update group
+// runs. Between groups, when nothing is queued, the cache is refreshed every
+// refresh_every groups.
 struct ServeRequest {
     uint64_t ticket = 0;
     uint32_t kind = 0;  // 0 update, 1 query
@@ -2211,6 +2744,9 @@ struct cig_server {
     std::thread th;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     DBuf<cig_edge> sd, sa;  // device staging of a group's records
+    CopyLane qlane;         // query results leave the device here
+    std::thread emitter;    // moves the last query's results out (reads the engine's query buffers only)
+    uint32_t groups_since_refresh = 0;
 
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2280,26 +2816,79 @@ struct cig_server {
         const uint64_t v0 = version + 1;
         if (st == CIG_OK) version += batch.size();
         finish(batch, st, stats, v0, true);
+        ++groups_since_refresh;
     }
 
+    void join_emitter() {
+        if (emitter.joinable()) emitter.join();
+    }
+
+    // The query's rounds on the engine stream; its results are moved out by
+    // the emitter thread on the copy lane. The next query waits for the
+    // emitter (it reuses the engine's query buffers), an update group does not.
     void serve_queries(std::vector<std::shared_ptr<ServeRequest>> &batch) {
+        join_emitter();
         std::vector<uint32_t> src;
         for (auto &r : batch) src.insert(src.end(), r->sources.begin(), r->sources.end());
         cig_batch_stats stats{};
-        cig_status st = CIG_OK;
         try {
-            std::vector<cig_engine::QueryOut> outs;
-            for (auto &r : batch) outs.push_back({r->out, uint32_t(r->sources.size())});
-            e->query(batch[0]->algo, src.data(), uint32_t(src.size()), outs.data(), outs.size(), &stats);
+            e->query_compute(batch[0]->algo, src.data(), uint32_t(src.size()), &stats);
         } catch (...) {
-            st = current_exception_status(e->err);
+            const cig_status st = current_exception_status(e->err);
+            finish(batch, st, stats, version, false);
+            return;
         }
-        finish(batch, st, stats, version, false);
+        const uint64_t v = version;
+        emitter = std::thread([this, batch, stats, v, k = uint32_t(src.size())]() mutable {
+            cig_status st = CIG_OK;
+            std::string err;
+            try {
+                (void)hipSetDevice(e->cfg.device);
+                std::vector<cig_engine::QueryOut> outs;
+                for (auto &r : batch) outs.push_back({r->out, uint32_t(r->sources.size())});
+                e->query_emit(k, outs.data(), outs.size(), qlane);
+            } catch (...) {
+                st = current_exception_status(err);
+            }
+            finish(batch, st, stats, v, false);
+        });
     }
 
     void fail(std::shared_ptr<ServeRequest> r, cig_status st) {
         std::vector<std::shared_ptr<ServeRequest>> one{r};
         finish(one, st, cig_batch_stats{}, version, false);
+    }
+
+    // Takes the next group from the queue (caller holds mu): the queued
+    // updates in order, or the oldest query with the later ones it can merge.
+    std::vector<std::shared_ptr<ServeRequest>> take_next() {
+        std::vector<std::shared_ptr<ServeRequest>> batch;
+        size_t first_q = queue.size(), first_u = queue.size();
+        for (size_t i = 0; i < queue.size() && (first_q == queue.size() || first_u == queue.size()); ++i) {
+            if (queue[i]->kind == 0 && first_u == queue.size()) first_u = i;
+            if (queue[i]->kind == 1 && first_q == queue.size()) first_q = i;
+        }
+        const bool query_due = first_q < queue.size() &&
+                               (first_u == queue.size() || now_ms() - queue[first_q]->info.submit_ms >=
+                                                               double(cfg.max_query_wait_ms));
+        const uint32_t kind = query_due ? 1u : 0u;
+        uint32_t nsrc = 0, algo = kind ? queue[first_q]->algo : 0u;
+        for (size_t i = kind ? first_q : first_u; i < queue.size();) {
+            const ServeRequest &r = *queue[i];
+            bool take = r.kind == kind;
+            if (take && kind == 0 && batch.size() >= cfg.max_group) break;
+            if (take && kind == 1) {
+                take = r.algo == algo && nsrc + r.sources.size() <= cfg.max_query_sources;
+                if (take) nsrc += uint32_t(r.sources.size());
+            }
+            if (take) {
+                batch.push_back(queue[i]);
+                queue.erase(queue.begin() + int64_t(i));
+            } else {
+                ++i;
+            }
+        }
+        return batch;
     }
 
     void loop() {
@@ -2309,23 +2898,11 @@ struct cig_server {
             {
                 std::unique_lock<std::mutex> lk(mu);
                 work_cv.wait(lk, [&] { return stopping || !queue.empty(); });
-                if (queue.empty()) return;  // stopping and drained
-                const uint32_t kind = queue.front()->kind;
-                uint32_t nsrc = 0;
-                while (!queue.empty() && queue.front()->kind == kind) {
-                    const ServeRequest &r = *queue.front();
-                    if (kind == 0) {
-                        if (batch.size() >= cfg.max_group) break;
-                    } else {
-                        if (r.algo != queue.front()->algo || (!batch.empty() && r.algo != batch[0]->algo)) break;
-                        if (nsrc + r.sources.size() > cfg.max_query_sources) break;
-                        nsrc += uint32_t(r.sources.size());
-                    }
-                    batch.push_back(queue.front());
-                    queue.pop_front();
-                }
+                if (queue.empty()) break;  // stopping and drained
+                batch = take_next();
                 for (auto &r : batch) r->info.start_ms = now_ms();
             }
+            if (batch.empty()) continue;
             // admission by free HBM: shrink the group until its staging fits
             if (batch[0]->kind == 0) {
                 try {
@@ -2349,6 +2926,7 @@ struct cig_server {
                     continue;
                 }
                 serve_updates(batch);
+                maybe_refresh();
             } else {
                 try {
                     const uint32_t k = [&] { uint32_t t = 0; for (auto &r : batch) t += uint32_t(r->sources.size()); return t; }();
@@ -2368,7 +2946,30 @@ struct cig_server {
                 serve_queries(batch);
             }
         }
+        join_emitter();
     }
+
+    // Cache maintenance between update groups (the reference refreshes after
+    // every batch, hybrid_sssp.cu:474-498): every refresh_every groups, when
+    // no request is waiting. Results do not depend on it.
+    void maybe_refresh() {
+        if (!cfg.refresh_every || groups_since_refresh < cfg.refresh_every || !e->budget_slots) return;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!queue.empty()) return;
+        }
+        groups_since_refresh = 0;
+        try {
+            e->cache_refresh(nullptr);
+            ++refreshes;
+        } catch (...) {
+            std::string err;
+            (void)current_exception_status(err);
+            refresh_error = err;
+        }
+    }
+    uint64_t refreshes = 0;
+    std::string refresh_error;
 };
 
 extern "C" {
@@ -2402,6 +3003,11 @@ cig_status cig_create(const cig_config *cfg, cig_engine **out) {
         return CIG_EINVAL;
     }
     if (cfg->segments > 512) { g_create_error = "segments must be <= 512"; return CIG_EINVAL; }
+    if ((cfg->flags & CIG_CFG_CC_UNION_REPAIR) &&
+        (cfg->algo != CIG_ALGO_CC || !(cfg->flags & CIG_CFG_SYMMETRIC) || (cfg->flags & CIG_CFG_COOPERATIVE))) {
+        g_create_error = "CIG_CFG_CC_UNION_REPAIR needs CC with CIG_CFG_SYMMETRIC and chained rounds";
+        return CIG_EINVAL;
+    }
     std::unique_ptr<cig_engine> e(new (std::nothrow) cig_engine());
     if (!e) return CIG_ENOMEM;
     e->cfg = *cfg;
@@ -2591,6 +3197,8 @@ cig_status cig_serve_config_default(cig_serve_config *c) {
     c->max_group = 32;
     c->max_query_sources = 64;
     c->hbm_reserve = uint64_t(1) << 30;
+    c->refresh_every = 8;
+    c->max_query_wait_ms = 250;
     return CIG_OK;
 }
 
@@ -2604,9 +3212,11 @@ cig_status cig_serve_start(cig_engine *e, const cig_serve_config *c, cig_server 
         if (!cfg.max_group || !cfg.max_query_sources || cfg.max_query_sources > 64)
             throw HipError(CIG_EINVAL, "max_group >= 1 and max_query_sources in 1..64");
         if (e->server) throw HipError(CIG_ESTATE, "a server already runs on this engine");
-        auto *s = new cig_server;
-        s->e = e;
-        s->cfg = cfg;
+        std::unique_ptr<cig_server> sp(new cig_server);
+        sp->e = e;
+        sp->cfg = cfg;
+        sp->qlane.init(nullptr);  // a stream of its own
+        cig_server *s = sp.release();
         s->th = std::thread([s] { s->loop(); });
         e->server = s;
         *out = s;

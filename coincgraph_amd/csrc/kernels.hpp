@@ -106,6 +106,19 @@ __global__ void k_init_pr(float *rank, float *comp, float *res, uint32_t *queued
 }
 
 // ------------------------------------------------------------------ mutation
+// A batch's records in device memory. Launches replayed from a captured
+// HIP graph read the records through it (the graph's arguments stay the
+// same for every batch; a one-thread kernel before each replay fills it).
+struct BatchRef {
+    const cig_edge *dels;
+    const cig_edge *adds;
+    uint32_t nd, na;
+};
+
+__global__ void k_set_batch(BatchRef *ref, const cig_edge *dels, uint32_t nd, const cig_edge *adds, uint32_t na) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ref = BatchRef{dels, adds, nd, na};
+}
+
 struct MutArgs {
     Ctl *ctl;
     Row *rows;
@@ -138,7 +151,18 @@ struct MutArgs {
     struct AddPiece *pieces;  // copy pieces of the insert phase (AddPiece below)
     StatShard *stats;
     uint32_t *hot;     // per row activity of the current window (null: not counted)
+    const BatchRef *ref;  // non-null: the records are read from here (graph replays)
 };
+
+// The call's records: from the batch reference when there is one.
+template <class A>
+__device__ __forceinline__ uint32_t rec_nd(const A &a) { return a.ref ? a.ref->nd : a.nd; }
+template <class A>
+__device__ __forceinline__ uint32_t rec_na(const A &a) { return a.ref ? a.ref->na : a.na; }
+template <class A>
+__device__ __forceinline__ const cig_edge *rec_dels(const A &a) { return a.ref ? a.ref->dels : a.dels; }
+template <class A>
+__device__ __forceinline__ const cig_edge *rec_adds(const A &a) { return a.ref ? a.ref->adds : a.adds; }
 
 // Deletes find their slot by row length (BuildDeletionPlan,
 // source_local_chunk_store.h:670-723, restated for the device):
@@ -198,24 +222,26 @@ struct MutRecord {
 };
 
 __device__ __forceinline__ MutRecord mut_record(const MutArgs &a, uint32_t i) {
-    const uint32_t total = a.nd + a.na;
+    const uint32_t nd = rec_nd(a), na = rec_na(a);
+    const uint32_t total = nd + na;
     MutRecord r;
     r.mirror = i >= total;
     const uint32_t j = r.mirror ? i - total : i;
-    r.del = j < a.nd;
-    const cig_edge e = r.del ? a.dels[j] : a.adds[j - a.nd];
+    r.del = j < nd;
+    const cig_edge e = r.del ? rec_dels(a)[j] : rec_adds(a)[j - nd];
     r.bad = !(e.u < a.n && e.v < a.n);
-    r.valid = !r.bad && !(a.skip && a.skip[r.del ? a.na + j : j - a.nd]);
+    r.valid = !r.bad && !(a.skip && a.skip[r.del ? na + j : j - nd]);
     r.row = r.mirror ? a.n + e.v : e.u;
     r.dst = r.mirror ? e.u : e.v;
     return r;
 }
 
-__device__ __forceinline__ uint32_t mut_records(const MutArgs &a) { return (a.nd + a.na) << a.rev; }
+__device__ __forceinline__ uint32_t mut_records(const MutArgs &a) { return (rec_nd(a) + rec_na(a)) << a.rev; }
 
 // Delete record i of [0, nd << rev): the deletes, then their mirrors.
 __device__ __forceinline__ MutRecord del_record(const MutArgs &a, uint32_t i) {
-    return mut_record(a, i < a.nd ? i : i - a.nd + a.nd + a.na);
+    const uint32_t nd = rec_nd(a);
+    return mut_record(a, i < nd ? i : i - nd + nd + rec_na(a));
 }
 
 // GroupedUpdateBatch (effective_update_batch.h:35-87) on the device: count
@@ -357,7 +383,7 @@ __device__ __forceinline__ void del_hole(const MutArgs &a, uint32_t row, uint32_
 
 // Tiny rows: one lane per delete record.
 __global__ void k_del_lane(MutArgs a) {
-    const uint32_t total = a.nd << a.rev;
+    const uint32_t total = rec_nd(a) << a.rev;
     for (uint32_t i = gthread(); i < total; i += gthreads()) {
         const MutRecord r = del_record(a, i);
         if (!r.valid || a.dclass[r.row] != kDelClassTiny) continue;
@@ -988,18 +1014,21 @@ struct InvalArgs {
     uint32_t max_rounds;
     uint64_t seed;
     const uint8_t *add_skip, *del_skip;  // grouped batches: cancelled records (may be null)
+    const BatchRef *ref;                 // non-null: the records are read from here (graph replays)
 };
 
 // reset_del_edges (driver.cuh:371-411): a deleted (u,v) whose v has parent u
 // seeds the affected set. Parallel copies of (u,v) are treated conservatively.
 __global__ void k_inval_seed(InvalArgs a) {
     __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < a.nd; base += gthreads()) {
+    const uint32_t nd = rec_nd(a);
+    const cig_edge *dels = rec_dels(a);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nd; base += gthreads()) {
         const uint32_t i = base + threadIdx.x;
         bool hit = false;
         uint32_t v = 0;
-        if (i < a.nd) {
-            const cig_edge e = a.dels[i];
+        if (i < nd) {
+            const cig_edge e = dels[i];
             if (e.u < a.n && e.v < a.n && !(a.del_skip && a.del_skip[i]) && uint32_t(a.state[e.v]) == e.u) {
                 v = e.v;
                 hit = bit_claim(a.aff, v);
@@ -1097,12 +1126,14 @@ __global__ void k_insert_seed(InvalArgs a, uint32_t skip_affected) {
     __shared__ uint32_t scratch[kMaxWavesPerBlock + 1];
     const uint32_t r = a.ctl->round;
     uint32_t *q = (r & 1) ? a.q1 : a.q0;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < a.na; base += gthreads()) {
+    const uint32_t na = rec_na(a);
+    const cig_edge *adds = rec_adds(a);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < na; base += gthreads()) {
         const uint32_t i = base + threadIdx.x;
         bool push = false;
         uint32_t v = 0;
-        if (i < a.na) {
-            const cig_edge e = a.adds[i];
+        if (i < na) {
+            const cig_edge e = adds[i];
             if (e.u < a.n && e.v < a.n && !(a.add_skip && a.add_skip[i]) &&
                 !(skip_affected && bit_test(a.aff, e.u))) {
                 const uint32_t val = uint32_t(a.state[e.u] >> 32);
@@ -1323,6 +1354,7 @@ struct LoopArgs {
     ExpandCfg cfg;
     uint64_t seed;
     uint32_t max_rounds;
+    uint32_t delta;  // delta-ordered rounds (0: unordered)
 };
 
 // One persistent cooperative launch runs rounds until the frontier and the
@@ -1369,7 +1401,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_loop(LoopArgs a) {
 // a few rounds enqueued ahead and advances `round` after the chain, so no
 // workgroup waits for another and a round has no closing step. A round
 // whose frontier and deferred tiles are empty returns at once.
-template <int W, bool kStrict, int QPL, int B = kBlock>
+template <int W, bool kStrict, int QPL, int B = kBlock, bool ORD = false>
 __global__ __launch_bounds__(B) void k_relax_round(LoopArgs a, uint64_t *prog, uint64_t tag, uint32_t seq) {
     __shared__ WaveLds lds[B / kWave];
     __shared__ RoundStage stage;
@@ -1379,6 +1411,7 @@ __global__ __launch_bounds__(B) void k_relax_round(LoopArgs a, uint64_t *prog, u
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->qcount[(r + 2) % 3] = 0;
         c->tcount[(r + 2) % 3] = 0;
+        if constexpr (ORD) c->bmin[(r + 1) % 3] = kInf;  // for the next round's k_ord_min
         if (nq || nt) c->rounds += 1;
         post_progress(prog, tag, nq != 0 || nt != 0);
     }
@@ -1389,10 +1422,52 @@ __global__ __launch_bounds__(B) void k_relax_round(LoopArgs a, uint64_t *prog, u
     const StagedOut<2048, 128> out{Sink{(r & 1) ? a.q0 : a.q1, &c->qcount[(r + 1) % 3]},
                                    TileSink{(r & 1) ? a.t0 : a.t1, &c->tcount[(r + 1) % 3]}, &stage};
     out_init(out);
+    if constexpr (ORD) {
+        // Items above the round's smallest bucket wait for a later round
+        // (PartitionActive, ordered_gpu_repair.cuh:33-48): re-listed here,
+        // skipped by the expansion below. Tiles always expand: their rows
+        // were cut in the previous round.
+        op.delta = a.delta;
+        op.bucket = c->bmin[r % 3];
+        const uint32_t wpb = blockDim.x / kWave, gw = blockIdx.x * wpb + wave_in_block();
+        for (uint32_t base = gw * kWave; base < nq; base += gridDim.x * wpb * kWave) {
+            const uint32_t i = base + lane_id();
+            bool f[1] = {false};
+            uint32_t v[1] = {0};
+            if (i < nq) {
+                v[0] = qin[i];
+                const uint32_t val = uint32_t(a.state[v[0]] >> 32);
+                f[0] = val != kInf && val / a.delta > op.bucket && claim(a.stamp, v[0], r + 1);
+            }
+            out_push<1>(out, f, v);
+        }
+    }
     expand_range<QPL>(op, a.cfg, qin, nq, out, lds[wave_in_block()], ctr);
     expand_tiles<QPL>(op, a.cfg, (r & 1) ? a.t1 : a.t0, nt, out);
     flush_counters(ctr, a.cfg.stats, a.cfg.stage);
     out_flush(out);
+}
+
+// Delta-ordered rounds (FindActiveBucket, ordered_gpu_repair.cuh:28-32):
+// the smallest value / delta of round r's frontier, one atomic per wave.
+// Values only fall, so a later read in the round can only find the bucket's
+// items at or below it: the round always expands at least one row.
+__global__ void k_ord_min(LoopArgs a, uint32_t seq) {
+    Ctl *c = a.ctl;
+    const uint32_t r = c->round + seq - 1;
+    const uint32_t nq = c->qcount[r % 3];
+    const uint32_t *qin = (r & 1) ? a.q1 : a.q0;
+    uint32_t m = kInf;
+    for (uint32_t i = gthread(); i < nq; i += gthreads()) {
+        const uint32_t val = uint32_t(a.state[qin[i]] >> 32);
+        if (val != kInf) m = min(m, val / a.delta);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) m = min(m, uint32_t(__shfl_xor(int(m), off, kWave)));
+    if (lane_id() == 0 && m != kInf) atomicMin(&c->bmin[r % 3], m);
+}
+
+__global__ void k_ord_reset(Ctl *c) {
+    if (threadIdx.x < 3 && blockIdx.x == 0) c->bmin[threadIdx.x] = kInf;
 }
 
 // Moves the round counter past a finished chain of `by` rounds.

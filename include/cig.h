@@ -81,12 +81,30 @@ enum {
                                         HIP events (cig_batch_stats.round_kernel_ms) */
     CIG_CFG_PR_PUSH_ONLY = 1u << 4, /* PR: keep no incoming-row index; every round
                                       scatters with atomics (the reference's form) */
-    CIG_CFG_SYMMETRIC = 1u << 5    /* the caller asserts an undirected graph: every edge (u,v)
+    CIG_CFG_SYMMETRIC = 1u << 5,   /* the caller asserts an undirected graph: every edge (u,v)
                                       has its reverse (v,u) as often, and every batch keeps it so
                                       (the samples' CC input). The outgoing rows then serve as
                                       the incoming rows: no mirrored updates, half the mutation
                                       work and edge memory. cig_load_csr rejects a graph whose
                                       in- and out-degrees differ. */
+    CIG_CFG_ORDERED_REPAIR = 1u << 6, /* SSSP: delta-ordered rounds (the reference's
+                                      CG_ORDERED_REPAIR executor, ordered_gpu_repair.cuh:117-160):
+                                      each round expands only the frontier rows whose value /
+                                      repair_delta is the round's smallest, the rest wait. Fewer
+                                      redundant relaxations on large-diameter graphs; same results */
+    CIG_CFG_GRAPH_BATCHES = 1u << 8, /* BFS/SSSP/CC single-source batches: the fixed-length parts of a
+                                      batch (the mutation with the invalidation seeds; the reset,
+                                      pull repair and insertion seeds) are replayed from two captured
+                                      HIP graphs, the kernels reading the batch's records through a
+                                      device reference; the round chains stay host-driven. Same
+                                      results; ignored for PR, several sources, staged or grouped
+                                      batches and CIG_CFG_SCAN_REPAIR */
+    CIG_CFG_CC_UNION_REPAIR = 1u << 7 /* CC on an undirected graph (needs CIG_CFG_SYMMETRIC): the
+                                      reference's union repair (cc_union_repair.cuh:9-137) in place
+                                      of label propagation. A delete re-derives the components it
+                                      touches by sampled union-find (the largest sampled component
+                                      skips hooking), an insert unites two components; labels are
+                                      the same, parents are not kept (CIG_INF) */
 };
 
 /* Engine configuration. Flags of the reference samples map as:
@@ -120,7 +138,8 @@ typedef struct cig_config {
                                      fields) and, with CIG_CFG_KERNEL_TIMING, its round kernels
                                      (each event record idles the stream a few us); total_ms is
                                      always timed; 0 or 1 = every call */
-    uint32_t reserved[2];
+    uint32_t repair_delta;    /* bucket width of CIG_CFG_ORDERED_REPAIR; 0 = 128 (the reference's) */
+    uint32_t reserved[1];
 } cig_config;
 
 /* Per-call statistics. Times are device (HIP event) milliseconds of the
@@ -227,9 +246,15 @@ cig_status cig_gather_incoming(cig_engine *e, uint64_t *offsets, uint32_t *src,
                                uint64_t src_capacity);
 
 /* ---- computation -------------------------------------------------------- */
-/* Engine::InitGraph (framework.cuh:991-1041). SSSP/BFS: exactly one source
- * (clamped to [0,n-1] by the samples, hybrid_sssp.cu:352). CC and PR ignore
- * sources. */
+/* Engine::InitGraph (framework.cuh:991-1041). SSSP/BFS: 1..64 sources, each
+ * in [0, n) (the samples clamp theirs, hybrid_sssp.cu:352). With one source
+ * the engine keeps one state word per vertex, as the reference. With 2..64
+ * sources every later call (cig_converge, the batch calls, the batched form,
+ * serving) keeps all of them current together: one mutation per batch, one
+ * invalidation wave, repair and relaxation pass for all sources (the reference
+ * runs one engine per source); cig_gather then returns nsrc*n words. Several
+ * sources need the incoming-row index (not CIG_CFG_SCAN_REPAIR) and chained
+ * rounds (not CIG_CFG_COOPERATIVE). CC and PR ignore sources. */
 cig_status cig_init(cig_engine *e, const uint32_t *sources, uint32_t nsrc);
 /* Engine::Start / StartPageRank (framework.cuh:1151-1182, 1649-1658): runs
  * the fixed point from the initial state. */
@@ -269,22 +294,32 @@ cig_status cig_load_updates(cig_engine *e, const cig_edge *adds, uint64_t n_add,
 cig_status cig_apply_next_batch(cig_engine *e, uint32_t flags, cig_batch_stats *stats);
 /* ---- serving loop (no counterpart in the reference, whose samples run one
  * batch at a time on the caller's thread) ----------------------------------
- * A server thread owns the engine: requests are queued FIFO and served in
- * submission order. Consecutive update requests (one batch each) are
- * applied as one group (cig_apply_next_batches' method) and consecutive
- * queries of one algorithm share one multi-source run of up to 64 sources,
- * so the result of every request equals the result it has when served alone
- * at the same position of the order (same graph version). A group is only
- * admitted when its staging memory fits in free HBM (hipMemGetInfo) minus
- * hbm_reserve; a request that cannot fit even alone fails with CIG_ENOMEM.
- * While a server runs, the engine may only be used through it. */
+ * A server thread owns the engine. Update requests (one batch each) are
+ * applied in submission order, queued ones together as one group
+ * (cig_apply_next_batches' method). Queries do not hold updates back: they are
+ * served when no update waits (or once one has waited max_query_wait_ms),
+ * queued queries of one algorithm merged into one multi-source run of up to
+ * 64 sources. A query's rounds run on the engine stream against the graph of
+ * the version it reports (cig_request_info.version: the update requests
+ * applied before it); its results are then copied out on a second stream
+ * while the next update group runs. Every result equals the result of the
+ * same request served alone at the version it reports. Between groups, when
+ * nothing waits, the cache is refreshed every refresh_every groups. A group
+ * is only admitted when its staging memory fits in free HBM (hipMemGetInfo)
+ * minus hbm_reserve; a request that cannot fit even alone fails with
+ * CIG_ENOMEM. While a server runs, the engine may only be used through it. */
 typedef struct cig_server cig_server;
 
 typedef struct cig_serve_config {
     uint32_t max_group;          /* update requests per group (default 32) */
     uint32_t max_query_sources;  /* sources merged into one query run (1..64, default 64) */
     uint64_t hbm_reserve;        /* bytes of free HBM admission leaves untouched */
-    uint64_t reserved[4];
+    uint32_t refresh_every;      /* cache refresh (cig_cache_refresh) after this many update groups,
+                                    when no request waits; 0 = never; 1 = after every group, as the
+                                    reference after every batch (default 8; engines with a budget) */
+    uint32_t max_query_wait_ms;  /* a query waiting this long is served before further update
+                                    groups (default 250) */
+    uint64_t reserved[3];
 } cig_serve_config;
 
 typedef struct cig_request_info {
@@ -365,7 +400,9 @@ cig_status cig_cache_refresh(cig_engine *e, cig_batch_stats *stats);
 
 /* ---- results ------------------------------------------------------------ */
 /* GatherValue / GatherParent / GatherBuffer (framework.cuh GraphDatum
- * gathers): copies n elements of the field into out (bytes >= n*elem). */
+ * gathers): copies n elements of the field into out (bytes >= n*elem). An
+ * engine initialised with 2..64 sources returns nsrc*n words for VALUE and
+ * PARENT, source-major: out[s*n + v] for sources[s]. */
 cig_status cig_gather(cig_engine *e, uint32_t field, void *out, size_t bytes);
 
 /* ---- host-side utilities (no GPU needed) -------------------------------- */

@@ -6,6 +6,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -90,6 +91,39 @@ static int cmp_u32(const void *a, const void *b) {
     return x < y ? -1 : x > y;
 }
 
+/* Removes from row s the first occurrences of the len requested
+ * destinations list[] (sorted in place; rem is scratch of len words); returns
+ * the number of requests that matched an edge. */
+static uint64_t delete_from_row(og_graph *g, uint32_t s, uint32_t *list, uint64_t len, uint32_t *rem) {
+    qsort(list, len, sizeof(uint32_t), cmp_u32);
+    /* run-length: unique destinations in list[0..nu), multiplicity in rem */
+    uint64_t nu = 0;
+    for (uint64_t i = 0; i < len; ++i) {
+        if (nu && list[nu - 1] == list[i]) { rem[nu - 1]++; continue; }
+        list[nu] = list[i];
+        rem[nu] = 1;
+        ++nu;
+    }
+    uint64_t matched = 0, w = 0;
+    uint32_t *row = g->rows[s];
+    for (uint64_t r = 0; r < g->deg[s]; ++r) {
+        uint32_t d = row[r];
+        uint64_t lo = 0, hi = nu;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi) / 2;
+            if (list[mid] < d) lo = mid + 1; else hi = mid;
+        }
+        if (lo < nu && list[lo] == d && rem[lo] > 0) {
+            rem[lo]--;
+            ++matched;
+            continue;
+        }
+        row[w++] = d;
+    }
+    g->deg[s] = (uint32_t)w;
+    return matched;
+}
+
 /* Deletion phase: BuildDeletionPlan (source_local_chunk_store.h:670-723)
  * collects per source the requested multiplicity of each destination; the
  * compaction (:733-780) then removes, scanning the row in order, the first
@@ -119,35 +153,7 @@ int og_apply_deletes(og_graph *g, const uint32_t *u, const uint32_t *v, uint64_t
     for (uint32_t s = 0; s < n; ++s) {
         uint64_t b = count[s], e = count[s + 1];
         if (b == e) continue;
-        uint32_t *list = req + b;
-        uint64_t len = e - b;
-        qsort(list, len, sizeof(uint32_t), cmp_u32);
-        /* run-length: unique destinations in list[0..nu), multiplicity in remaining */
-        uint64_t nu = 0;
-        for (uint64_t i = 0; i < len; ++i) {
-            if (nu && list[nu - 1] == list[i]) { remaining[b + nu - 1]++; continue; }
-            list[nu] = list[i];
-            remaining[b + nu] = 1;
-            ++nu;
-        }
-        uint64_t matched = 0, w = 0;
-        uint32_t *row = g->rows[s];
-        for (uint64_t r = 0; r < g->deg[s]; ++r) {
-            uint32_t d = row[r];
-            uint64_t lo = 0, hi = nu;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi) / 2;
-                if (list[mid] < d) lo = mid + 1; else hi = mid;
-            }
-            if (lo < nu && list[lo] == d && remaining[b + lo] > 0) {
-                remaining[b + lo]--;
-                ++matched;
-                continue;
-            }
-            row[w++] = d;
-        }
-        g->deg[s] = (uint32_t)w;
-        miss += len - matched;
+        miss += (e - b) - delete_from_row(g, s, req + b, e - b, remaining + b);
     }
     free(count); free(req); free(fill); free(remaining);
     if (missing) *missing += miss;
@@ -168,6 +174,109 @@ int og_apply_adds(og_graph *g, const uint32_t *u, const uint32_t *v, uint64_t k,
     }
     if (invalid) *invalid += inval;
     return 0;
+}
+
+/* The reference's CPU batch maintenance with a worker pool
+ * (ApplyMutationPhase, source_local_chunk_store.h:116-121, 218-391): the
+ * records are grouped by source (stable, as GroupedUpdateBatch,
+ * effective_update_batch.h:35-87), then `threads` workers each take a range
+ * of sources with about equal record counts and apply, per source, its
+ * deletes (delete_from_row) and then its adds in input order. The result is
+ * the same as og_apply_deletes followed by og_apply_adds. */
+typedef struct {
+    og_graph *g;
+    const uint64_t *dcnt, *acnt;
+    uint32_t *dreq, *drem;
+    const uint32_t *areq;
+    uint32_t lo, hi;
+    uint64_t matched;
+    int err;
+} mt_job;
+
+static void *mt_worker(void *arg) {
+    mt_job *j = (mt_job *)arg;
+    for (uint32_t s = j->lo; s < j->hi; ++s) {
+        uint64_t b = j->dcnt[s], e = j->dcnt[s + 1];
+        if (b < e) j->matched += delete_from_row(j->g, s, j->dreq + b, e - b, j->drem + b);
+        b = j->acnt[s];
+        e = j->acnt[s + 1];
+        if (b == e) continue;
+        if (row_reserve(j->g, s, (uint64_t)j->g->deg[s] + (e - b))) { j->err = 1; return NULL; }
+        memcpy(j->g->rows[s] + j->g->deg[s], j->areq + b, (e - b) * sizeof(uint32_t));
+        j->g->deg[s] += (uint32_t)(e - b);
+    }
+    return NULL;
+}
+
+/* per-source counts (n+1 prefix) and the destinations grouped by source */
+static int group_by_source(uint32_t n, const uint32_t *u, const uint32_t *v, uint64_t k, uint64_t **cnt_out,
+                           uint32_t **req_out, uint64_t *inval) {
+    uint64_t *cnt = (uint64_t *)calloc((size_t)n + 1, sizeof(uint64_t));
+    if (!cnt) return -1;
+    for (uint64_t i = 0; i < k; ++i) {
+        if (u[i] >= n || v[i] >= n) { ++*inval; continue; }
+        cnt[u[i] + 1]++;
+    }
+    for (uint32_t s = 0; s < n; ++s) cnt[s + 1] += cnt[s];
+    uint64_t *fill = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+    uint32_t *req = (uint32_t *)malloc((cnt[n] ? cnt[n] : 1) * sizeof(uint32_t));
+    if (!fill || !req) { free(cnt); free(fill); free(req); return -1; }
+    memcpy(fill, cnt, ((size_t)n + 1) * sizeof(uint64_t));
+    for (uint64_t i = 0; i < k; ++i)
+        if (u[i] < n && v[i] < n) req[fill[u[i]]++] = v[i];
+    free(fill);
+    *cnt_out = cnt;
+    *req_out = req;
+    return 0;
+}
+
+int og_apply_batch_mt(og_graph *g, const uint32_t *du, const uint32_t *dv, uint64_t nd, const uint32_t *au,
+                      const uint32_t *av, uint64_t na, uint32_t threads, uint64_t *missing,
+                      uint64_t *invalid) {
+    uint32_t n = g->n;
+    uint64_t inval = 0, *dcnt = NULL, *acnt = NULL;
+    uint32_t *dreq = NULL, *areq = NULL;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if (group_by_source(n, du, dv, nd, &dcnt, &dreq, &inval) ||
+        group_by_source(n, au, av, na, &acnt, &areq, &inval)) {
+        free(dcnt); free(dreq); free(acnt); free(areq);
+        return -1;
+    }
+    uint32_t *drem = (uint32_t *)malloc((dcnt[n] ? dcnt[n] : 1) * sizeof(uint32_t));
+    mt_job *jobs = (mt_job *)calloc(threads, sizeof(mt_job));
+    pthread_t *tid = (pthread_t *)calloc(threads, sizeof(pthread_t));
+    int rc = (!drem || !jobs || !tid) ? -1 : 0;
+    if (rc == 0) {
+        /* source ranges of about equal record counts */
+        uint64_t total = dcnt[n] + acnt[n], s = 0;
+        for (uint32_t t = 0; t < threads; ++t) {
+            uint64_t goal = total * (t + 1) / threads;
+            jobs[t].lo = (uint32_t)s;
+            while (s < n && (t + 1 == threads || dcnt[s + 1] + acnt[s + 1] <= goal)) ++s;
+            jobs[t].hi = (uint32_t)s;
+            jobs[t].g = g;
+            jobs[t].dcnt = dcnt;
+            jobs[t].acnt = acnt;
+            jobs[t].dreq = dreq;
+            jobs[t].drem = drem;
+            jobs[t].areq = areq;
+        }
+        uint32_t started = 0;
+        for (; started < threads; ++started)
+            if (pthread_create(&tid[started], NULL, mt_worker, &jobs[started]) != 0) break;
+        for (uint32_t t = started; t < threads; ++t) mt_worker(&jobs[t]); /* could not start: run here */
+        for (uint32_t t = 0; t < started; ++t) pthread_join(tid[t], NULL);
+        uint64_t matched = 0;
+        for (uint32_t t = 0; t < threads; ++t) {
+            matched += jobs[t].matched;
+            if (jobs[t].err) rc = -1;
+        }
+        if (missing) *missing += dcnt[n] - matched;
+        if (invalid) *invalid += inval;
+    }
+    free(dcnt); free(dreq); free(acnt); free(areq); free(drem); free(jobs); free(tid);
+    return rc;
 }
 
 /* ---- binary min-heap on (dist, vertex) for Dijkstra ---- */

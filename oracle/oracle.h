@@ -67,6 +67,12 @@ int og_apply_deletes(og_graph *g, const uint32_t *u, const uint32_t *v, uint64_t
                      uint64_t *missing, uint64_t *invalid);
 int og_apply_adds(og_graph *g, const uint32_t *u, const uint32_t *v, uint64_t k,
                   uint64_t *invalid);
+/* Deletes then adds with `threads` workers over source ranges (the
+ * reference's CPU mutation pool, source_local_chunk_store.h:116-121, 218-391);
+ * same result as og_apply_deletes followed by og_apply_adds. */
+int og_apply_batch_mt(og_graph *g, const uint32_t *du, const uint32_t *dv, uint64_t nd, const uint32_t *au,
+                      const uint32_t *av, uint64_t na, uint32_t threads, uint64_t *missing,
+                      uint64_t *invalid);
 
 /* Full recomputes. */
 void og_sssp_dijkstra(const og_graph *g, uint32_t src, int wmode, uint64_t seed,

@@ -77,6 +77,9 @@ def lib():
         L.og_pr_jacobi.restype = C.c_int
         L.og_pr_jacobi.argtypes = [C.POINTER(_Graph), P, C.c_double, C.c_int]
         L.og_pr_check.argtypes = [C.POINTER(_Graph), P, P, C.c_double, C.POINTER(PrCheckResult)]
+        L.og_apply_batch_mt.restype = C.c_int
+        L.og_apply_batch_mt.argtypes = [C.POINTER(_Graph), P, P, C.c_uint64, P, P, C.c_uint64, C.c_uint32,
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.og_pr_invariant.restype = C.c_int
         L.og_pr_invariant.argtypes = [C.POINTER(_Graph), P, P, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -162,6 +165,16 @@ class Graph:
         if lib().og_apply_adds(self._g, _p(u), _p(v), u.size, C.byref(inval)):
             raise MemoryError
         return inval.value
+
+    def apply_batch_mt(self, du, dv, au, av, threads: int):
+        """Deletes then adds with `threads` workers (the reference's CPU
+        mutation pool); returns (missing deletes, invalid records)."""
+        arrs = [np.ascontiguousarray(x, dtype=np.uint32) for x in (du, dv, au, av)]
+        miss, inval = C.c_uint64(0), C.c_uint64(0)
+        if lib().og_apply_batch_mt(self._g, _p(arrs[0]), _p(arrs[1]), arrs[0].size, _p(arrs[2]), _p(arrs[3]),
+                                   arrs[2].size, threads, C.byref(miss), C.byref(inval)):
+            raise MemoryError
+        return miss.value, inval.value
 
     # ---- recomputes
     def sssp(self, src: int, wmode: int = W_REF128, seed: int = 0) -> np.ndarray:

@@ -209,3 +209,17 @@ def test_oracle_golden_fixture_known_answers():
     assert og.cc().tolist() == exp["cc"].astype(int).tolist()
     x, _ = og.pagerank()
     np.testing.assert_allclose(x, exp["pr"], rtol=0, atol=1e-12)
+
+
+def test_alternative_executor_configs_are_checked():
+    """CIG_CFG_CC_UNION_REPAIR needs CC on an undirected graph; refused at
+    cig_create, before any device is touched."""
+    cfg = cg.Config()
+    L = cg.lib()
+    e = C.c_void_p()
+    for algo, flags in ((cg.CC, cg.CFG_CC_UNION_REPAIR), (cg.SSSP, cg.CFG_SYMMETRIC | cg.CFG_CC_UNION_REPAIR),
+                        (cg.CC, cg.CFG_SYMMETRIC | cg.CFG_CC_UNION_REPAIR | cg.CFG_COOPERATIVE)):
+        L.cig_config_default(C.byref(cfg))
+        cfg.algo, cfg.flags = algo, flags
+        assert L.cig_create(C.byref(cfg), C.byref(e)) == cg.EINVAL
+        assert b"UNION_REPAIR" in L.cig_last_error(None)

@@ -1,0 +1,69 @@
+"""Graph-replayed batches (CIG_CFG_GRAPH_BATCHES): the fixed-length parts of
+every batch are replayed from two captured HIP graphs; results must equal
+the oracle after every replay, with the program's own eager work (gathers,
+other engines' batches, synchronisations) between replays, and a pool
+compaction (new buffers) must lead to a fresh capture, not a stale replay."""
+import numpy as np
+import pytest
+
+import coincgraph_amd as cg
+from tests.helpers import check_topology, check_traversal, oracle_apply, oracle_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if cg.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+@pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_HASH64), ("bfs", 0), ("cc", 0)])
+@pytest.mark.parametrize("resident", [True, False])
+def test_graph_batches_match_oracle(algo, wmode, resident):
+    print(f"[gpu-case] graph batches {algo} resident={resident}", flush=True)
+    sym = algo == "cc"
+    g = cg.rmat_graph(12, 16, seed=111, symmetric=sym)
+    s = cg.rmat_stream(g, 12, 8, 400, 400, seed=112, symmetric=sym)
+    og = oracle_graph(g)
+    flags = cg.CFG_GRAPH_BATCHES | (cg.CFG_SYMMETRIC if sym else 0)
+    with cg.Engine(algo, weight_mode=wmode, weight_seed=31, flags=flags) as e, \
+            cg.Engine(algo, weight_mode=wmode, weight_seed=31, flags=flags & ~cg.CFG_GRAPH_BATCHES) as eager:
+        for x in (e, eager):
+            x.load_graph(g)
+            x.init_graph(0)
+            x.start()
+        if resident:
+            e.load_updates(s)
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            st = e.apply_next_batch() if resident else e.apply_batch(dels, adds)
+            eager.apply_batch(dels, adds)  # eager work on the same device between replays
+            miss, inval = oracle_apply(og, dels, adds)
+            assert st["missing_deletes"] == miss and st["invalid_updates"] == inval
+            check_traversal(e, og, algo, 0, wmode, 31, where=f"graph batch {b}")
+            assert np.array_equal(e.values(), eager.values()), b
+        check_topology(e, og, "graph final")
+
+
+def test_graph_batches_recapture_after_compaction():
+    """A pool too small for the inserts forces compactions (new pool buffers):
+    the next batches must capture new graphs and stay exact."""
+    print("[gpu-case] graph recapture", flush=True)
+    g = cg.rmat_graph(11, 16, seed=113)
+    s = cg.rmat_stream(g, 11, 6, 100, 3000, seed=114)  # many more inserts than deletes
+    og = oracle_graph(g)
+    caps = sum(max(4, 1 << int(np.ceil(np.log2(max(d, 1))))) if d else 0 for d in np.diff(g.offsets))
+    with cg.Engine("sssp", flags=cg.CFG_GRAPH_BATCHES, cache_bytes=int(caps * 4 * 1.15)) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        comp = 0
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            st = e.apply_batch(dels, adds)
+            comp = max(comp, st["pool_compactions"])
+            oracle_apply(og, dels, adds)
+            check_traversal(e, og, "sssp", 0, where=f"recapture batch {b}")
+        assert comp > 0, "the stream never overflowed the pool"
+        check_topology(e, og, "recapture final")

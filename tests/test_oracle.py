@@ -220,6 +220,11 @@ def test_mutation_semantics_vs_bruteforce():
         inval += g.apply_adds(au, av)
         assert g.rows() == exp_rows, trial  # order-exact, not only multiset
         assert (miss, inval) == (exp_miss, exp_inval)
+        # the worker-pool form (any thread count) gives the same rows, in order
+        for threads in (1, 3, 8):
+            gm = O.Graph.from_edges(n, src, dst)
+            assert gm.apply_batch_mt(du, dv, au, av, threads) == (exp_miss, exp_inval), (trial, threads)
+            assert gm.rows() == exp_rows, (trial, threads)
 
 
 def test_bellman_and_parent_checks_catch_planted_mistakes():
