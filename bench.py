@@ -234,6 +234,12 @@ def main(argv=None):
     ap.add_argument("--autotune", action="store_true", help="pick the launch shape first (cig_autotune)")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner (config 3 runs it by default)")
     ap.add_argument("--no-multi-source", action="store_true", help="skip the 64-source engine (configs 1, 2)")
+    ap.add_argument("--ordered", action="store_true",
+                    help="SSSP: delta-ordered rounds (CIG_CFG_ORDERED_REPAIR, the reference's ordered executor)")
+    ap.add_argument("--cc-union", action="store_true",
+                    help="CC: union-find repair (CIG_CFG_CC_UNION_REPAIR, the reference's union repair)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each batch's fixed-length parts from captured HIP graphs (CIG_CFG_GRAPH_BATCHES)")
     ap.add_argument("--pr-push", action="store_true",
                     help="PageRank: scatter every round with atomics (keeps no incoming rows)")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving: arrival window (s)")
@@ -283,6 +289,12 @@ def main(argv=None):
               kernel_timing_every=TIMING_EVERY)
     if c["sym"]:
         kw["flags"] |= cg.CFG_SYMMETRIC  # undirected graph: outgoing rows double as incoming rows
+    if args.ordered:
+        kw["flags"] |= cg.CFG_ORDERED_REPAIR
+    if args.cc_union:
+        kw["flags"] |= cg.CFG_CC_UNION_REPAIR
+    if args.graph:
+        kw["flags"] |= cg.CFG_GRAPH_BATCHES
     if pr:
         kw.update(pr_epsilon=1e-7, pr_max_rounds=1000)
         if args.pr_push:
@@ -417,6 +429,8 @@ def main(argv=None):
             f"{algo} R-MAT scale {scale} edge factor {ef}, batches of {batch} deletes + {batch} inserts",
             "config": args.config, "vertices": g.n, "edges": g.m, "batch_deletes": batch,
             "batch_inserts": batch, "symmetric_pairs": c["sym"], "mode": "staged" if args.staged else "fused",
+            "executor": ("delta-ordered" if args.ordered else "union-find" if args.cc_union else "default")
+            + (", graph-replayed batches" if args.graph else ""),
         },
         "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4),
                        "mean": round(float(np.mean(lat)), 4), "max": round(max(lat), 4), "samples": len(lat)},
@@ -450,6 +464,24 @@ def main(argv=None):
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def slowest_groups(res, k=6):
+    """The k longest services (one entry per group): when the group started
+    (s after the first submission), its service and queueing time, its size."""
+    t0 = min(r["submit_ms"] for _, _, r in res)
+    seen, out = set(), []
+    for kind, _, r in res:
+        key = (r["start_ms"], r["done_ms"])
+        if key in seen:
+            continue
+        seen.add(key)
+        out.append({"kind": kind, "start_s": round((r["start_ms"] - t0) / 1e3, 3),
+                    "service_ms": round(r["done_ms"] - r["start_ms"], 1),
+                    "waited_ms": round(r["start_ms"] - r["submit_ms"], 1), "group_size": r["group_size"],
+                    "device_ms": round(r["stats"]["total_ms"], 1)})
+    out.sort(key=lambda x: -x["service_ms"])
+    return out[:k]
 
 
 def serve_bench(args, c):
@@ -537,6 +569,9 @@ def serve_bench(args, c):
         if time.perf_counter() - last > 10:
             last = time.perf_counter()
             log(f"completed {len(done)}/{len(tickets)}")
+            if time.perf_counter() - start > args.serve_seconds + 300:
+                log("requests still pending 300 s after the last arrival: giving up")
+                return 1
         time.sleep(0.01)
     srv.stop()
     # cache admission on the activity the serving run left (age, histogram,
@@ -595,6 +630,7 @@ def serve_bench(args, c):
                     "service_ms": {"p50": round(percentile([r["done_ms"] - r["start_ms"] for r in upd], 50), 3),
                                    "p99": round(percentile([r["done_ms"] - r["start_ms"] for r in upd], 99), 3)}},
         "queries": queries,
+        "slowest_groups": slowest_groups(res),
         "cache_refresh": refresh,
         "host_link": {"cold_edge_fraction": round(cold, 4),
                       "probe_gbs": round(probe_gbs, 1),
@@ -605,6 +641,9 @@ def serve_bench(args, c):
         "host": host_info(),
         "engine": {k: info[k] for k in ("hbm_pool_slots", "host_pool_slots", "cached_rows", "cached_edges",
                                         "device_bytes", "host_pinned_bytes")},
+        "compactions": {"total": end_info["compactions"], "in_place": end_info["inplace_compactions"],
+                        "host_tier": end_info["host_compactions"], "ms_hbm": end_info["compaction_ms_hbm"],
+                        "ms_host": end_info["compaction_ms_host"]},
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -44,6 +44,7 @@ CFG_SYMMETRIC = 1 << 5
 CFG_ORDERED_REPAIR = 1 << 6
 CFG_CC_UNION_REPAIR = 1 << 7
 CFG_GRAPH_BATCHES = 1 << 8
+CFG_PR_KEEP_IDS = 1 << 9
 BATCH_STAGED = 1
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
@@ -74,7 +75,7 @@ class Config(C.Structure):
         ("pr_max_rounds", C.c_uint32), ("pr_epsilon", C.c_float),
         ("tile_len", C.c_uint32), ("round_blocks_per_cu", C.c_uint32),
         ("pr_pull_divisor", C.c_uint32), ("pull_blocks_per_cu", C.c_uint32),
-        ("kernel_timing_every", C.c_uint32), ("repair_delta", C.c_uint32), ("reserved", C.c_uint32 * 1),
+        ("kernel_timing_every", C.c_uint32), ("repair_delta", C.c_uint32), ("pool_slack", C.c_uint32),
     ]
 
 
@@ -140,7 +141,9 @@ class GraphInfo(C.Structure):
         ("device_bytes", C.c_uint64), ("host_pinned_bytes", C.c_uint64),
         ("grid_blocks", C.c_uint32), ("block_size", C.c_uint32),
         ("edges_per_lane", C.c_uint32), ("split_degree", C.c_uint32),
-        ("cu_count", C.c_uint32), ("reserved", C.c_uint32 * 7),
+        ("cu_count", C.c_uint32), ("compactions", C.c_uint32), ("inplace_compactions", C.c_uint32),
+        ("host_compactions", C.c_uint32), ("compaction_ms_hbm", C.c_uint32),
+        ("compaction_ms_host", C.c_uint32), ("reserved", C.c_uint32 * 2),
     ]
 
     def as_dict(self):
@@ -226,6 +229,9 @@ def lib():
                                        C.POINTER(C.c_uint32)]),
         "cig_write_csr_bin": (C.c_int, [C.c_char_p, C.c_uint32, P, P]),
         "cig_read_csr_bin": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(P), C.POINTER(P)]),
+        "cig_rmat_write": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                     C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                     C.POINTER(C.c_uint64)]),
         "cig_write_updates_bin": (C.c_int, [C.c_char_p, P, C.c_uint64, P, C.c_uint64, P, C.c_uint32]),
         "cig_read_updates_bin": (C.c_int, [C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64), C.POINTER(P),
                                            C.POINTER(C.c_uint64), C.POINTER(P), C.POINTER(C.c_uint32)]),
@@ -378,6 +384,16 @@ def write_csr_bin(g: HostGraph, path: str) -> None:
     _check(lib().cig_write_csr_bin(path.encode(), g.n, _ptr(g.offsets), _ptr(g.dst) if g.m else None))
 
 
+def rmat_write(path: str, scale: int, edge_factor: int = 16, seed: int = 1, symmetric: bool = False,
+               segments: int = 0, mem_bytes: int = 1 << 30, abc=RMAT_ABC, threads: int = 0) -> int:
+    """Out-of-core R-MAT (cig_rmat_write): the graph rmat_graph builds with the
+    same arguments, written to a CIGG file by source segments; returns m."""
+    m = C.c_uint64()
+    _check(lib().cig_rmat_write(path.encode(), scale, edge_factor, abc[0], abc[1], abc[2], seed,
+                                1 if symmetric else 0, segments, mem_bytes, threads, C.byref(m)))
+    return m.value
+
+
 def read_csr_bin(path: str) -> HostGraph:
     n = C.c_uint32()
     off = C.c_void_p()
@@ -432,7 +448,7 @@ class Engine:
                  device=0, pr_epsilon=1e-6, pr_max_rounds=100, flags=0, edges_per_lane=0,
                  split_degree=0, max_rounds=0, n_stream=3, segments=1, tile_len=0,
                  round_blocks_per_cu=0, pr_pull_divisor=0, pull_blocks_per_cu=0, kernel_timing_every=0,
-                 repair_delta=0):
+                 repair_delta=0, pool_slack=0):
         L = lib()
         cfg = Config()
         _check(L.cig_config_default(C.byref(cfg)))
@@ -455,6 +471,7 @@ class Engine:
         cfg.pull_blocks_per_cu = pull_blocks_per_cu
         cfg.kernel_timing_every = kernel_timing_every
         cfg.repair_delta = repair_delta
+        cfg.pool_slack = pool_slack
         self.config = cfg
         self._e = C.c_void_p()
         st = L.cig_create(C.byref(cfg), C.byref(self._e))

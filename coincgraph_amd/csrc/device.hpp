@@ -529,6 +529,21 @@ __device__ __forceinline__ uint32_t load_quad(const uint32_t *row, uint32_t qi, 
     return live;
 }
 
+// load_quad for rows streamed once per pass (PageRank's incoming-row
+// sweeps): a nontemporal load, so the stream does not push the gathered
+// contributions out of L2.
+typedef uint32_t u32x4_nt __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint32_t load_quad_stream(const uint32_t *row, uint32_t qi, uint32_t deg,
+                                                     uint32_t (&d)[4]) {
+    const u32x4_nt x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_nt *>(row + 4ull * qi));
+    const uint32_t live = min(deg - 4u * qi, 4u);
+    d[0] = x.x;
+    d[1] = live > 1 ? x.y : x.x;
+    d[2] = live > 2 ? x.z : x.x;
+    d[3] = live > 3 ? x.w : x.x;
+    return live;
+}
+
 // Expands the 64 items [base, base+64) of `items` (or the identity range when
 // items == nullptr) that are below `count`. Each lane takes QPL quads of the
 // wave's concatenated rows per step, so a step covers 256*QPL slots.

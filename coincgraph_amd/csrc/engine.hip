@@ -229,6 +229,11 @@ struct cig_engine {
     bool sym = false;    // symmetric graph: the outgoing rows are the incoming rows
     bool loaded = false, initialized = false;
     uint32_t source = 0;
+    // PR vertex renumbering (see degree_order): perm[v] = internal id of the
+    // caller's vertex v, inv its inverse; empty when ids are kept
+    bool renum = false;
+    std::vector<uint32_t> perm_h, inv_h;
+    DBuf<uint32_t> perm_d;
 
     DBuf<Row> rows;
     DBuf<uint64_t> state;
@@ -475,6 +480,80 @@ struct cig_engine {
         if (bad) throw HipError(CIG_EINVAL, "CIG_CFG_SYMMETRIC: a vertex's in-degree differs from its out-degree");
     }
 
+    // PageRank's dense rounds gather one contribution per incoming edge, from
+    // the edge's source. Internal ids by decreasing out-degree (ties by id;
+    // degrees of 4095 and more form the first class) put the contributions of
+    // the hubs, which most gathers read, in a few megabytes at the front of
+    // the per-vertex arrays (R-MAT scale 25: the first 2^20 internal ids are
+    // the sources of 80% of the edges), where they share cache lines and stay
+    // in L2. A stable counting sort over contiguous thread chunks.
+    static void degree_order(uint32_t nv, const uint64_t *off, unsigned T, std::vector<uint32_t> &perm,
+                             std::vector<uint32_t> &inv) {
+        constexpr uint32_t K = 4096;
+        perm.assign(nv, 0u);
+        inv.assign(nv, 0u);
+        auto key = [&](uint64_t v) { return uint32_t(K - 1 - std::min<uint64_t>(off[v + 1] - off[v], K - 1)); };
+        std::vector<uint64_t> cnt(size_t(T) * K, 0);
+        parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned t) {
+            uint64_t *c = cnt.data() + size_t(t) * K;
+            for (uint64_t v = lo; v < hi; ++v) c[key(v)]++;
+        });
+        uint64_t run = 0;
+        for (uint32_t k = 0; k < K; ++k)
+            for (unsigned t = 0; t < T; ++t) {
+                const uint64_t x = cnt[size_t(t) * K + k];
+                cnt[size_t(t) * K + k] = run;
+                run += x;
+            }
+        parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned t) {
+            uint64_t *c = cnt.data() + size_t(t) * K;
+            for (uint64_t v = lo; v < hi; ++v) {
+                const uint32_t at = uint32_t(c[key(v)]++);
+                inv[at] = uint32_t(v);
+                perm[v] = at;
+            }
+        });
+    }
+
+    // The graph in internal ids: row i is the caller's row inv[i], its
+    // destinations mapped through perm.
+    static void permute_csr(uint32_t nv, const uint64_t *off, const uint32_t *dst, unsigned T,
+                            const std::vector<uint32_t> &perm, const std::vector<uint32_t> &inv,
+                            std::vector<uint64_t> &poff, std::vector<uint32_t> &pdst) {
+        poff.assign(size_t(nv) + 1, 0);
+        std::vector<uint64_t> part(T + 1, 0);
+        parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned t) {
+            uint64_t s = 0;
+            for (uint64_t i = lo; i < hi; ++i) s += off[inv[i] + 1] - off[inv[i]];
+            part[t + 1] = s;
+        });
+        for (unsigned t = 0; t < T; ++t) part[t + 1] += part[t];
+        parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned t) {
+            uint64_t o = part[t];
+            for (uint64_t i = lo; i < hi; ++i) {
+                poff[i] = o;
+                o += off[inv[i] + 1] - off[inv[i]];
+            }
+        });
+        poff[nv] = off[nv];
+        pdst.resize(off[nv]);
+        parallel_for(nv, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+            for (uint64_t i = lo; i < hi; ++i) {
+                const uint32_t *s = dst + off[inv[i]];
+                uint32_t *d = pdst.data() + poff[i];
+                for (uint64_t k = 0, deg = poff[i + 1] - poff[i]; k < deg; ++k) d[k] = perm[s[k]];
+            }
+        });
+    }
+
+    // Caller ids of update records to internal ids, in place (ids >= n stay,
+    // so invalid records are rejected as before).
+    void records_to_internal(cig_edge *p, uint64_t cnt) {
+        if (!renum || cnt == 0) return;
+        k_map_records<<<grid_small, 256, 0, stream>>>(p, cnt, perm_d.p, n);
+        HIPCHK(hipGetLastError());
+    }
+
     // ------------------------------------------------------------ load
     void load(uint32_t nv, const uint64_t *off, const uint32_t *dst) {
         if (nv == 0) throw HipError(CIG_EINVAL, "graph must have at least one vertex");
@@ -492,6 +571,18 @@ struct cig_engine {
                 if (dst[e] >= nv) bad = 1;
         });
         if (bad) throw HipError(CIG_EINVAL, "destination id out of range");
+        renum = is_pr() && !(cfg.flags & CIG_CFG_PR_KEEP_IDS);
+        std::vector<uint64_t> poff;
+        std::vector<uint32_t> pdst;
+        if (renum) {
+            degree_order(nv, off, T, perm_h, inv_h);
+            permute_csr(nv, off, dst, T, perm_h, inv_h, poff, pdst);
+            off = poff.data();
+            dst = pdst.data();
+        } else {
+            perm_h.clear();
+            inv_h.clear();
+        }
         n = nv;
         budget_slots = cfg.cache_bytes / sizeof(uint32_t);
         // incoming rows (the device reverse index) for deletion repair
@@ -538,8 +629,10 @@ struct cig_engine {
             hbm_need = used;
             host_need = total_cap - used;
         }
-        hbm_slots = budget_slots ? budget_slots : hbm_need + hbm_need / 4 + (uint64_t(1) << 22);
-        host_slots = host_need ? host_need + host_need / 4 + (uint64_t(1) << 20) : (uint64_t(1) << 16);
+        const uint64_t slack = cfg.pool_slack ? cfg.pool_slack : (uint64_t(1) << 22);
+        hbm_slots = budget_slots ? budget_slots : hbm_need + hbm_need / 4 + slack;
+        host_slots = host_need ? host_need + host_need / 4 + std::min<uint64_t>(slack, uint64_t(1) << 20)
+                               : (uint64_t(1) << 16);
         hbm_pool.alloc(hbm_slots);
         host_pool.alloc(host_slots);
         host_pool_dev = host_pool.device_ptr();
@@ -618,6 +711,10 @@ struct cig_engine {
         hctl.p->pool_cap[0] = hbm_slots;
         hctl.p->pool_cap[1] = host_slots;
         HIPCHK(hipMemcpyAsync(ctl.p, hctl.p, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+        if (renum) {
+            perm_d.alloc(n);
+            HIPCHK(hipMemcpyAsync(perm_d.p, perm_h.data(), size_t(n) * 4, hipMemcpyHostToDevice, stream));
+        }
         sync();
         loaded = true;
         initialized = false;
@@ -1151,8 +1248,24 @@ struct cig_engine {
     // Compacts one tier's pool (dropping relocation garbage) into a new
     // allocation with room for `extra` more slots. HBM is bounded by the
     // cache budget: rows that do not fit are demoted to the host tier.
+    uint64_t host_compactions = 0;
+    double compaction_ms[2] = {0.0, 0.0};
+
     void compact_pool(int tier, uint64_t extra) {
+        const auto t0 = std::chrono::steady_clock::now();
+        struct Clock {  // wall time of this compaction, whichever way it ends
+            std::chrono::steady_clock::time_point t0;
+            double *acc;
+            ~Clock() {
+                *acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            }
+        } clock{t0, &compaction_ms[tier]};
+        if (tier == 1) ++host_compactions;
         reduce_stats();  // garbage counted so far lands in the control block first
+        if (compact_in_place(tier, extra)) {
+            ++compactions;
+            return;
+        }
         sync();
         std::vector<Row> hrows(nrows);
         HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
@@ -1191,11 +1304,13 @@ struct cig_engine {
         } else {
             HBuf<uint32_t> np;
             np.alloc(cap);
-            for (uint32_t v = 0; v < nrows; ++v) {
-                if (!in_tier(hrows[v]) || !hrows[v].deg) continue;
-                std::memcpy(np.p + (moved[v].off & ~kHostTier), host_pool.p + (hrows[v].off & ~kHostTier),
-                            size_t(hrows[v].deg) * sizeof(uint32_t));
-            }
+            parallel_for(nrows, pick_threads(0), [&](uint64_t lo, uint64_t hi, unsigned) {
+                for (uint64_t v = lo; v < hi; ++v) {
+                    if (!in_tier(hrows[v]) || !hrows[v].deg) continue;
+                    std::memcpy(np.p + (moved[v].off & ~kHostTier), host_pool.p + (hrows[v].off & ~kHostTier),
+                                size_t(hrows[v].deg) * sizeof(uint32_t));
+                }
+            });
             HIPCHK(hipMemcpy(rows.p, moved.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice));
             std::swap(host_pool.p, np.p);
             std::swap(host_pool.count, np.count);
@@ -1209,6 +1324,164 @@ struct cig_engine {
         HIPCHK(hipMemcpy(&ctl.p->garbage[tier], &zero, sizeof(uint64_t), hipMemcpyHostToDevice));
         ensure_tiles();
         ++compactions;
+    }
+
+    // In-place compaction of one tier on the device (kernels k_cmp_*): rows are
+    // bucketed by 2^k-slot windows of their offset, each window packed through
+    // a staging buffer. The host reads only the per-window totals (at most
+    // kCmpMaxWin of them). Returns false, changing nothing, when the live rows
+    // plus `extra` do not fit the tier's pool (or the HBM budget): the caller
+    // then grows the pool or demotes rows.
+    DBuf<unsigned long long> cmp_win;  // wcnt | wcap | lfill | cursor, kCmpMaxWin each
+    DBuf<uint32_t> cmp_list, cmp_stage;
+    DBuf<CmpJob> cmp_jobs;
+    uint64_t inplace_compactions = 0;
+
+    bool compact_in_place(int tier, uint64_t extra) {
+        if (tier == 1) return compact_host_in_place(extra);
+        read_ctl();
+        const uint64_t tail = hctl.p->pool_tail[tier], cap = hctl.p->pool_cap[tier];
+        if (tail == 0) return false;
+        uint64_t wslots = uint64_t(1) << 22;
+        while ((tail + wslots - 1) / wslots > kCmpMaxWin) wslots <<= 1;
+        const uint32_t nwin = uint32_t((tail + wslots - 1) / wslots);
+        if (!cmp_win.p) cmp_win.alloc(4 * kCmpMaxWin);
+        fill32(reinterpret_cast<uint32_t *>(cmp_win.p), 8ull * kCmpMaxWin, 0u);
+        CmpArgs a{};
+        a.rows = rows.p;
+        a.nrows = nrows;
+        a.tier = uint32_t(tier);
+        a.nwin = nwin;
+        a.wslots = wslots;
+        a.wcnt = cmp_win.p;
+        a.wcap = cmp_win.p + kCmpMaxWin;
+        a.lfill = cmp_win.p + 2 * kCmpMaxWin;
+        a.pools = pools();
+        const uint32_t chunks = (nrows + kCmpChunk - 1) / kCmpChunk;
+        k_cmp_hist<<<chunks, 256, 0, stream>>>(a);
+        HIPCHK(hipGetLastError());
+        std::vector<unsigned long long> h(2 * kCmpMaxWin);
+        HIPCHK(hipMemcpyAsync(h.data(), cmp_win.p, h.size() * 8, hipMemcpyDeviceToHost, stream));
+        sync();
+        uint64_t live = 0, rowsn = 0, maxcnt = 0, maxcap = 0;
+        std::vector<unsigned long long> lbase(kCmpMaxWin, 0), nbase(nwin, 0);
+        for (uint32_t w = 0; w < nwin; ++w) {
+            lbase[w] = rowsn;
+            nbase[w] = live;
+            rowsn += h[w];
+            live += h[kCmpMaxWin + w];
+            maxcnt = std::max<uint64_t>(maxcnt, h[w]);
+            maxcap = std::max<uint64_t>(maxcap, h[kCmpMaxWin + w]);
+        }
+        const uint64_t limit = tier == 0 && budget_slots ? std::min(cap, budget_slots) : cap;
+        if (live + extra > limit || maxcnt > 0xFFFFFFFFull) return false;
+        if (cmp_list.count < rowsn) cmp_list.alloc(rowsn);
+        if (cmp_jobs.count < maxcnt) cmp_jobs.alloc(maxcnt);
+        if (cmp_stage.count < maxcap) cmp_stage.alloc(maxcap);
+        a.list = cmp_list.p;
+        HIPCHK(hipMemcpyAsync(a.lfill, lbase.data(), kCmpMaxWin * 8, hipMemcpyHostToDevice, stream));
+        k_cmp_scatter<<<chunks, 256, 0, stream>>>(a);
+        HIPCHK(hipGetLastError());
+        unsigned long long *cursor = cmp_win.p + 3 * kCmpMaxWin;
+        for (uint32_t w = 0; w < nwin; ++w) {
+            const uint32_t cnt = uint32_t(h[w]);
+            if (!cnt) continue;
+            k_cmp_plan<<<grid_small, 256, 0, stream>>>(a, cmp_list.p + lbase[w], cnt, cursor + w, cmp_jobs.p);
+            k_cmp_in<<<grid_small, 256, 0, stream>>>(a, cmp_jobs.p, cnt, cmp_stage.p);
+            k_cmp_out<<<grid_small, 256, 0, stream>>>(a, cmp_jobs.p, cnt, cmp_stage.p, nbase[w]);
+            HIPCHK(hipGetLastError());
+        }
+        const uint64_t zero = 0;
+        HIPCHK(hipMemcpyAsync(&ctl.p->pool_tail[tier], &live, sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(&ctl.p->garbage[tier], &zero, sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+        sync();
+        ++inplace_compactions;
+        return true;
+    }
+
+    // The host tier packed in place by host threads on the pinned pool, with
+    // the same windows as compact_in_place. Copying host-tier rows through the
+    // device would cross the host link twice per row, one small row per wave
+    // step (latency-bound: minutes for the 10^8 short rows of a scale-27
+    // graph); host threads move them at memory bandwidth instead. The row
+    // descriptors make one round trip; nothing else is allocated but a
+    // staging buffer of one window.
+    bool compact_host_in_place(uint64_t extra) {
+        reduce_stats();
+        read_ctl();
+        const uint64_t tail = hctl.p->pool_tail[1], cap = hctl.p->pool_cap[1];
+        if (tail == 0) return false;
+        uint64_t wslots = uint64_t(1) << 22;
+        while ((tail + wslots - 1) / wslots > kCmpMaxWin) wslots <<= 1;
+        const uint32_t nwin = uint32_t((tail + wslots - 1) / wslots);
+        std::vector<Row> hrows(nrows);
+        HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
+        const unsigned T = pick_threads(0);
+        auto in_host = [](const Row &r) { return r.cap != 0 && (r.off & kHostTier); };
+        // per-thread window histograms, then row ids grouped by window
+        std::vector<uint64_t> tcnt(size_t(T) * nwin, 0), tcap(size_t(T) * nwin, 0);
+        parallel_for(nrows, T, [&](uint64_t lo, uint64_t hi, unsigned t) {
+            for (uint64_t v = lo; v < hi; ++v)
+                if (in_host(hrows[v])) {
+                    const uint64_t w = (hrows[v].off & ~kHostTier) / wslots;
+                    tcnt[size_t(t) * nwin + w]++;
+                    tcap[size_t(t) * nwin + w] += hrows[v].cap;
+                }
+        });
+        std::vector<uint64_t> wbase(nwin + 1, 0), nbase(nwin + 1, 0), wcap(nwin, 0);
+        uint64_t rowsn = 0, live = 0, maxcap = 0;
+        for (uint32_t w = 0; w < nwin; ++w) {
+            wbase[w] = rowsn;
+            nbase[w] = live;
+            for (unsigned t = 0; t < T; ++t) {
+                const uint64_t c = tcnt[size_t(t) * nwin + w];
+                tcnt[size_t(t) * nwin + w] = rowsn;  // this thread's first slot in the window's list
+                rowsn += c;
+                wcap[w] += tcap[size_t(t) * nwin + w];
+            }
+            live += wcap[w];
+            maxcap = std::max(maxcap, wcap[w]);
+        }
+        wbase[nwin] = rowsn;
+        if (live + extra > cap) return false;
+        std::vector<uint32_t> list(rowsn);
+        parallel_for(nrows, T, [&](uint64_t lo, uint64_t hi, unsigned t) {
+            for (uint64_t v = lo; v < hi; ++v)
+                if (in_host(hrows[v])) list[tcnt[size_t(t) * nwin + (hrows[v].off & ~kHostTier) / wslots]++] = uint32_t(v);
+        });
+        std::vector<uint32_t> stage(maxcap);
+        std::vector<uint64_t> pos;
+        uint32_t *pool = host_pool.p;
+        for (uint32_t w = 0; w < nwin; ++w) {
+            const uint64_t b = wbase[w], e = wbase[w + 1];
+            if (b == e) continue;
+            pos.resize(e - b);
+            uint64_t run = 0;
+            for (uint64_t i = b; i < e; ++i) {
+                pos[i - b] = run;
+                run += hrows[list[i]].cap;
+            }
+            // all of the window's rows are read before any is written
+            parallel_for(e - b, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+                for (uint64_t i = lo; i < hi; ++i) {
+                    const Row &r = hrows[list[b + i]];
+                    std::memcpy(stage.data() + pos[i], pool + (r.off & ~kHostTier), size_t(r.deg) * 4);
+                }
+            });
+            parallel_for(e - b, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+                for (uint64_t i = lo; i < hi; ++i) {
+                    Row &r = hrows[list[b + i]];
+                    r.off = (nbase[w] + pos[i]) | kHostTier;
+                    std::memcpy(pool + (r.off & ~kHostTier), stage.data() + pos[i], size_t(r.deg) * 4);
+                }
+            });
+        }
+        HIPCHK(hipMemcpy(rows.p, hrows.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice));
+        const uint64_t zero = 0;
+        HIPCHK(hipMemcpy(&ctl.p->pool_tail[1], &live, sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&ctl.p->garbage[1], &zero, sizeof(uint64_t), hipMemcpyHostToDevice));
+        ++inplace_compactions;
+        return true;
     }
 
     // Moves the coldest HBM rows (lowest activity score, then shortest) to the
@@ -1993,6 +2266,8 @@ struct cig_engine {
         ensure_records(nd, na);
         if (nd) HIPCHK(hipMemcpyAsync(rec_dels.p, dels, nd * sizeof(cig_edge), hipMemcpyHostToDevice, stream));
         if (na) HIPCHK(hipMemcpyAsync(rec_adds.p, adds, na * sizeof(cig_edge), hipMemcpyHostToDevice, stream));
+        records_to_internal(rec_dels.p, nd);
+        records_to_internal(rec_adds.p, na);
         apply_device(rec_dels.p, uint32_t(nd), rec_adds.p, uint32_t(na), flags, st);
     }
 
@@ -2044,6 +2319,9 @@ struct cig_engine {
         s_dels.alloc(td);
         if (ta) HIPCHK(hipMemcpy(s_adds.p, adds, ta * sizeof(cig_edge), hipMemcpyHostToDevice));
         if (td) HIPCHK(hipMemcpy(s_dels.p, dels, td * sizeof(cig_edge), hipMemcpyHostToDevice));
+        records_to_internal(s_adds.p, ta);
+        records_to_internal(s_dels.p, td);
+        sync();
         batch_sizes.assign(sizes, sizes + 2 * nb);
         next_batch = 0;
         deletes_done = false;
@@ -2329,6 +2607,7 @@ struct cig_engine {
     void query_check(uint32_t algo, const uint32_t *sources, uint32_t k) const {
         if (!loaded) throw HipError(CIG_ESTATE, "no graph loaded");
         if (algo != CIG_ALGO_BFS && algo != CIG_ALGO_SSSP) throw HipError(CIG_EINVAL, "queries are BFS or SSSP");
+        if (renum) throw HipError(CIG_EINVAL, "queries on a PR engine need CIG_CFG_PR_KEEP_IDS");
         if (k == 0 || k > 64 || !sources) throw HipError(CIG_EINVAL, "1..64 sources");
         for (uint32_t s = 0; s < k; ++s)
             if (sources[s] >= n) throw HipError(CIG_EINVAL, "source out of range");
@@ -2558,23 +2837,38 @@ struct cig_engine {
         sync();
         const size_t need = field == CIG_FIELD_TIER ? n : size_t(n) * 4;
         if (!out || bytes < need) throw HipError(CIG_EINVAL, "output buffer too small");
+        // row / word of the caller's vertex v
+        auto at = [&](uint32_t v) { return renum ? perm_h[v] : v; };
         if (field == CIG_FIELD_DEGREE || field == CIG_FIELD_TIER) {
             std::vector<Row> hrows(n);
             HIPCHK(hipMemcpy(hrows.data(), rows.p, n * sizeof(Row), hipMemcpyDeviceToHost));
             for (uint32_t v = 0; v < n; ++v) {
-                if (field == CIG_FIELD_DEGREE) static_cast<uint32_t *>(out)[v] = hrows[v].deg;
-                else static_cast<uint8_t *>(out)[v] = (hrows[v].off & kHostTier) ? 0 : 1;
+                const Row &r = hrows[at(v)];
+                if (field == CIG_FIELD_DEGREE) static_cast<uint32_t *>(out)[v] = r.deg;
+                else static_cast<uint8_t *>(out)[v] = (r.off & kHostTier) ? 0 : 1;
             }
             return;
         }
+        auto words = [&](const uint32_t *src) {
+            if (!renum) {
+                HIPCHK(hipMemcpy(out, src, size_t(n) * 4, hipMemcpyDeviceToHost));
+                return;
+            }
+            std::vector<uint32_t> tmp(n);
+            HIPCHK(hipMemcpy(tmp.data(), src, size_t(n) * 4, hipMemcpyDeviceToHost));
+            uint32_t *o = static_cast<uint32_t *>(out);
+            parallel_for(n, pick_threads(0), [&](uint64_t lo, uint64_t hi, unsigned) {
+                for (uint64_t v = lo; v < hi; ++v) o[v] = tmp[perm_h[v]];
+            });
+        };
         if (field == CIG_FIELD_HOTNESS) {
-            HIPCHK(hipMemcpy(out, score.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+            words(score.p);
             return;
         }
         if (!initialized) throw HipError(CIG_ESTATE, "cig_init must come first");
         if (is_pr()) {
-            if (field == CIG_FIELD_VALUE) HIPCHK(hipMemcpy(out, rank.p, size_t(n) * 4, hipMemcpyDeviceToHost));
-            else if (field == CIG_FIELD_RESIDUAL) HIPCHK(hipMemcpy(out, res.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+            if (field == CIG_FIELD_VALUE) words(reinterpret_cast<const uint32_t *>(rank.p));
+            else if (field == CIG_FIELD_RESIDUAL) words(reinterpret_cast<const uint32_t *>(res.p));
             else throw HipError(CIG_EINVAL, "field not available for PR");
             return;
         }
@@ -2611,10 +2905,13 @@ struct cig_engine {
         uint64_t o = 0;
         for (uint32_t v = 0; v < n; ++v) {
             offs[v] = o;
-            const Row &r = hrows[v];
+            const Row &r = hrows[renum ? perm_h[v] : v];
             if (o + r.deg > capacity) throw HipError(CIG_EINVAL, "dst capacity too small");
             const uint32_t *src = (r.off & kHostTier) ? host_pool.p + (r.off & ~kHostTier) : hp.data() + r.off;
-            if (r.deg) std::memcpy(dst + o, src, size_t(r.deg) * 4);
+            if (renum)
+                for (uint32_t k = 0; k < r.deg; ++k) dst[o + k] = inv_h[src[k]];
+            else if (r.deg)
+                std::memcpy(dst + o, src, size_t(r.deg) * 4);
             o += r.deg;
         }
         offs[n] = o;
@@ -2628,6 +2925,11 @@ struct cig_engine {
         gi->edges_per_lane = 4 * qpl;
         gi->split_degree = split;
         gi->cu_count = uint32_t(cus);
+        gi->compactions = uint32_t(compactions);
+        gi->inplace_compactions = uint32_t(inplace_compactions);
+        gi->host_compactions = uint32_t(host_compactions);
+        gi->compaction_ms_hbm = uint32_t(compaction_ms[0]);
+        gi->compaction_ms_host = uint32_t(compaction_ms[1]);
         gi->grid_blocks = grid_coop;
         if (!loaded) return;
         sync();
@@ -2809,6 +3111,8 @@ struct cig_server {
                 oa += r->adds.size();
             }
             e->check_ready();
+            e->records_to_internal(sd.p, nd);
+            e->records_to_internal(sa.p, na);
             e->apply_group(sd.p, uint32_t(nd), sa.p, uint32_t(na), dcut, acut, 0, &stats);
         } catch (...) {
             st = current_exception_status(e->err);

@@ -566,6 +566,98 @@ extern "C" cig_status cig_write_csr_bin(const char *path, uint32_t n, const uint
     return ok ? CIG_OK : CIG_EIO;
 }
 
+// Out-of-core R-MAT: the samples of cig_rmat_csr (same counter-based
+// sampler and seed, so the same graph), written to a CIGG file one range of
+// source segments at a time. Each pass regenerates every sample and keeps the
+// edges whose source lies in its range (and, undirected, those whose
+// destination does, reversed); a pass covers consecutive segments whose
+// expected edge count fits `mem_bytes`. All edges of a source fall in one
+// segment, so dedup and the per-row sort are local to a pass; nothing of size
+// m is held in memory.
+extern "C" cig_status cig_rmat_write(const char *path, uint32_t scale, uint32_t edge_factor, double a, double b,
+                                     double c, uint64_t seed, uint32_t symmetric, uint32_t segments,
+                                     uint64_t mem_bytes, uint32_t threads, uint64_t *m_out) {
+    if (!path || scale == 0 || scale > 31 || edge_factor == 0 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+        return CIG_EINVAL;
+    const uint32_t n = 1u << scale;
+    if (segments == 0) segments = std::min<uint32_t>(512, n);
+    if (segments > n || (segments & (segments - 1))) return CIG_EINVAL;
+    unsigned sbits = 0;
+    while ((1u << sbits) < segments) ++sbits;
+    const unsigned T = pick_threads(threads);
+    const uint64_t samples = uint64_t(edge_factor) << scale;
+    const RmatSampler smp{scale, a + b, a, a + b + c, seed};
+    const uint32_t seg_len = n >> sbits;
+    // expected edges of segment p: its source prefix (a+b for a 0 bit, c+d for
+    // a 1 bit, top bit first), plus for an undirected graph its destination
+    // prefix (a+c, b+d)
+    std::vector<double> expect(segments);
+    for (uint32_t p = 0; p < segments; ++p) {
+        double ps = 1, pd = 1;
+        for (unsigned k = 0; k < sbits; ++k) {
+            const bool one = (p >> (sbits - 1 - k)) & 1;
+            ps *= one ? 1.0 - (a + b) : a + b;
+            pd *= one ? 1.0 - (a + c) : a + c;
+        }
+        expect[p] = double(samples) * (ps + (symmetric ? pd : 0.0));
+    }
+    const double per_pass = std::max(1.0, double(mem_bytes) / 24.0);  // keys, sort buffer, thread parts
+    File out(path, "wb");
+    if (!out.f) return CIG_EIO;
+    const uint64_t dst_base = sizeof(BinHeader) + (uint64_t(n) + 1) * 8;
+    uint64_t m = 0;
+    try {
+        for (uint32_t p0 = 0; p0 < segments;) {
+            uint32_t p1 = p0 + 1;
+            double e = expect[p0];
+            while (p1 < segments && e + expect[p1] <= per_pass) e += expect[p1++];
+            const uint32_t lo = p0 * seg_len, hi = p1 == segments ? n : p1 * seg_len;
+            std::vector<std::vector<uint64_t>> part(T);
+            parallel_for(samples, T, [&](uint64_t i0, uint64_t i1, unsigned t) {
+                std::vector<uint64_t> &v = part[t];
+                for (uint64_t i = i0; i < i1; ++i) {
+                    uint32_t su, dv;
+                    smp.sample(i, su, dv);
+                    if (su == dv) continue;
+                    if (su >= lo && su < hi) v.push_back(uint64_t(su) << scale | dv);
+                    if (symmetric && dv >= lo && dv < hi) v.push_back(uint64_t(dv) << scale | su);
+                }
+            });
+            uint64_t total = 0;
+            for (auto &v : part) total += v.size();
+            std::vector<uint64_t> keys;
+            keys.reserve(total);
+            for (auto &v : part) {
+                keys.insert(keys.end(), v.begin(), v.end());
+                std::vector<uint64_t>().swap(v);
+            }
+            radix_sort_u64(keys, 2 * scale, T);
+            keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+            std::vector<uint64_t> off(size_t(hi - lo) + 1, 0);
+            for (uint64_t k : keys) off[(k >> scale) - lo + 1]++;
+            off[0] = m;
+            for (uint32_t i = 0; i < hi - lo; ++i) off[i + 1] += off[i];
+            std::vector<uint32_t> dst(keys.size());
+            for (size_t i = 0; i < keys.size(); ++i) dst[i] = uint32_t(keys[i] & (n - 1));
+            if (std::fseek(out.f, long(sizeof(BinHeader) + uint64_t(lo) * 8), SEEK_SET) != 0 ||
+                !out.write(off.data(), (uint64_t(hi) - lo) * 8))
+                return CIG_EIO;
+            if (std::fseek(out.f, long(dst_base + m * 4), SEEK_SET) != 0 || !out.write(dst.data(), dst.size() * 4))
+                return CIG_EIO;
+            m += keys.size();
+            p0 = p1;
+        }
+    } catch (const std::bad_alloc &) {
+        return CIG_ENOMEM;
+    }
+    const BinHeader h{kGraphMagic, kBinVersion, n, 0, m, 0};
+    if (std::fseek(out.f, long(sizeof(BinHeader) + uint64_t(n) * 8), SEEK_SET) != 0 || !out.write(&m, 8) ||
+        std::fseek(out.f, 0, SEEK_SET) != 0 || !out.write(&h, sizeof(h)))
+        return CIG_EIO;
+    if (m_out) *m_out = m;
+    return CIG_OK;
+}
+
 extern "C" cig_status cig_read_csr_bin(const char *path, uint32_t *n_out, uint64_t **offsets_out,
                                        uint32_t **dst_out) {
     if (!path || !n_out || !offsets_out || !dst_out) return CIG_EINVAL;

@@ -119,6 +119,17 @@ __global__ void k_set_batch(BatchRef *ref, const cig_edge *dels, uint32_t nd, co
     if (threadIdx.x == 0 && blockIdx.x == 0) *ref = BatchRef{dels, adds, nd, na};
 }
 
+// Update records from the caller's vertex ids to a renumbered engine's;
+// ids >= n stay as they are, so invalid records are still rejected.
+__global__ void k_map_records(cig_edge *r, uint64_t cnt, const uint32_t *perm, uint32_t n) {
+    for (uint64_t i = gthread(); i < cnt; i += gthreads()) {
+        cig_edge e = r[i];
+        if (e.u < n) e.u = perm[e.u];
+        if (e.v < n) e.v = perm[e.v];
+        r[i] = e;
+    }
+}
+
 struct MutArgs {
     Ctl *ctl;
     Row *rows;
@@ -1851,7 +1862,7 @@ __device__ void pull_chunk(const PrArgs &a, const unsigned long long *mask, uint
                 const uint32_t kk = k < total ? k : total - 1;
                 own[q] = owner_of(L.pref, kk);
                 const uint64_t o = (uint64_t(L.offhi[own[q]]) << 32) | L.offlo[own[q]];
-                live[q] = load_quad(a.cfg.pools.row(o), kk - L.pref[own[q]], L.deg[own[q]], d[q]);
+                live[q] = load_quad_stream(a.cfg.pools.row(o), kk - L.pref[own[q]], L.deg[own[q]], d[q]);
                 if (k >= total) live[q] = 0;
             }
             float s[QPL];
@@ -1884,7 +1895,7 @@ __device__ void pull_tile(const PrArgs &a, const unsigned long long *mask, uint3
 #pragma unroll
         for (int q = 0; q < QPL; ++q) {
             const uint32_t k = k0 + uint32_t(q) * kWave + lane;
-            live[q] = load_quad(p, k < nq ? k : nq - 1, t.len, d[q]);
+            live[q] = load_quad_stream(p, k < nq ? k : nq - 1, t.len, d[q]);
             if (k >= nq) live[q] = 0;
         }
         float s[QPL];
@@ -2324,6 +2335,120 @@ __global__ void k_relocate_rows(const Row *old_rows, Row *new_rows, uint32_t n, 
         const uint32_t *s = src_pool + (o.off & ~kHostTier);
         uint32_t *d = dst_pool + (nr.off & ~kHostTier);
         for (uint32_t q = lane; q < o.deg; q += kWave) d[q] = s[q];
+    }
+}
+
+// ------------------------------------------------ in-place pool compaction
+// Replaces the full-budget compaction copy (driver.cuh:236-247): the rows of
+// one tier are bucketed by windows of `wslots` slots of their current offset.
+// Window k's rows are packed into a staging buffer and then written to their
+// final places, starting at the total capacity of the earlier windows. Rows
+// do not overlap, so everything window k writes lies below its last row's
+// end, i.e. below every row of window k+1 (not yet moved), and nothing is
+// overwritten before it is read. No second pool and no host copy of the rows.
+constexpr uint32_t kCmpMaxWin = 1024;
+constexpr uint32_t kCmpChunk = 1u << 16;  // rows per workgroup of the bucketing passes
+
+struct CmpJob {
+    uint64_t from;   // current offset (tier bit kept)
+    uint64_t spos;   // position in the staging buffer = offset within the window's packed range
+    uint32_t len;    // live slots
+    uint32_t row;
+};
+
+struct CmpArgs {
+    Row *rows;
+    uint32_t nrows, tier, nwin;
+    uint64_t wslots;
+    unsigned long long *wcnt, *wcap;  // per window: rows, capacity
+    unsigned long long *lfill;        // per window: next free list position
+    uint32_t *list;                   // row ids grouped by window
+    Pools pools;
+};
+
+__device__ __forceinline__ bool cmp_in_tier(const Row &r, uint32_t tier) {
+    return r.cap != 0 && ((r.off & kHostTier) != 0) == (tier == 1);
+}
+
+// Per window: rows and capacity (LDS counts per workgroup chunk).
+__global__ void k_cmp_hist(CmpArgs a) {
+    __shared__ unsigned long long cnt[kCmpMaxWin], cap[kCmpMaxWin];
+    for (uint32_t w = threadIdx.x; w < a.nwin; w += blockDim.x) cnt[w] = cap[w] = 0;
+    __syncthreads();
+    const uint64_t lo = uint64_t(blockIdx.x) * kCmpChunk, hi = min<uint64_t>(a.nrows, lo + kCmpChunk);
+    for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
+        const Row r = a.rows[v];
+        if (!cmp_in_tier(r, a.tier)) continue;
+        const uint32_t w = uint32_t((r.off & ~kHostTier) / a.wslots);
+        atomicAdd(&cnt[w], 1ull);
+        atomicAdd(&cap[w], (unsigned long long)r.cap);
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < a.nwin; w += blockDim.x) {
+        if (cnt[w]) atomicAdd(&a.wcnt[w], cnt[w]);
+        if (cap[w]) atomicAdd(&a.wcap[w], cap[w]);
+    }
+}
+
+// Row ids grouped by window (lfill starts at each window's list base).
+__global__ void k_cmp_scatter(CmpArgs a) {
+    __shared__ unsigned long long cnt[kCmpMaxWin], base[kCmpMaxWin];
+    for (uint32_t w = threadIdx.x; w < a.nwin; w += blockDim.x) cnt[w] = 0;
+    __syncthreads();
+    const uint64_t lo = uint64_t(blockIdx.x) * kCmpChunk, hi = min<uint64_t>(a.nrows, lo + kCmpChunk);
+    for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
+        const Row r = a.rows[v];
+        if (cmp_in_tier(r, a.tier)) atomicAdd(&cnt[uint32_t((r.off & ~kHostTier) / a.wslots)], 1ull);
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < a.nwin; w += blockDim.x) {
+        base[w] = cnt[w] ? atomicAdd(&a.lfill[w], cnt[w]) : 0ull;
+        cnt[w] = 0;
+    }
+    __syncthreads();
+    for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
+        const Row r = a.rows[v];
+        if (!cmp_in_tier(r, a.tier)) continue;
+        const uint32_t w = uint32_t((r.off & ~kHostTier) / a.wslots);
+        a.list[base[w] + atomicAdd(&cnt[w], 1ull)] = uint32_t(v);
+    }
+}
+
+// One window: each row's place in the packed range (workgroup-aggregated
+// reservations on the window's cursor) and its copy job.
+__global__ void k_cmp_plan(CmpArgs a, const uint32_t *list, uint32_t cnt, unsigned long long *cursor, CmpJob *jobs) {
+    __shared__ unsigned long long scratch64[kMaxWavesPerBlock + 1];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += gthreads()) {
+        const uint32_t i = base + threadIdx.x;
+        Row r{};
+        uint32_t v = 0;
+        if (i < cnt) {
+            v = list[i];
+            r = a.rows[v];
+        }
+        const uint64_t pos = block_reserve<unsigned long long>(i < cnt ? r.cap : 0ull, cursor, scratch64);
+        if (i < cnt) jobs[i] = CmpJob{r.off, pos, r.deg, v};
+    }
+}
+
+// Live slots into the staging buffer (a wave per row).
+__global__ void k_cmp_in(CmpArgs a, const CmpJob *jobs, uint32_t cnt, uint32_t *stage) {
+    for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
+        const CmpJob jb = jobs[j];
+        const uint32_t *s = a.pools.row(jb.from);
+        for (uint32_t q = lane_id(); q < jb.len; q += kWave) stage[jb.spos + q] = s[q];
+    }
+}
+
+// From the staging buffer to the final place, which the row now names.
+__global__ void k_cmp_out(CmpArgs a, const CmpJob *jobs, uint32_t cnt, const uint32_t *stage, uint64_t newbase) {
+    const uint64_t tbit = a.tier ? kHostTier : 0ull;
+    for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
+        const CmpJob jb = jobs[j];
+        const uint64_t off = (newbase + jb.spos) | tbit;
+        uint32_t *d = a.pools.row(off);
+        for (uint32_t q = lane_id(); q < jb.len; q += kWave) d[q] = stage[jb.spos + q];
+        if (lane_id() == 0) a.rows[jb.row].off = off;
     }
 }
 

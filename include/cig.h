@@ -99,12 +99,18 @@ enum {
                                       device reference; the round chains stay host-driven. Same
                                       results; ignored for PR, several sources, staged or grouped
                                       batches and CIG_CFG_SCAN_REPAIR */
-    CIG_CFG_CC_UNION_REPAIR = 1u << 7 /* CC on an undirected graph (needs CIG_CFG_SYMMETRIC): the
+    CIG_CFG_CC_UNION_REPAIR = 1u << 7, /* CC on an undirected graph (needs CIG_CFG_SYMMETRIC): the
                                       reference's union repair (cc_union_repair.cuh:9-137) in place
                                       of label propagation. A delete re-derives the components it
                                       touches by sampled union-find (the largest sampled component
                                       skips hooking), an insert unites two components; labels are
                                       the same, parents are not kept (CIG_INF) */
+    CIG_CFG_PR_KEEP_IDS = 1u << 9     /* PR: keep the caller's vertex ids inside the engine. By
+                                      default a PR engine numbers vertices internally by
+                                      decreasing out-degree, so the contributions most gathers
+                                      read share cache lines; every id crossing the ABI (records,
+                                      gathers, topology) stays the caller's. A renumbered engine
+                                      runs no cig_query (CIG_EINVAL). */
 };
 
 /* Engine configuration. Flags of the reference samples map as:
@@ -139,7 +145,9 @@ typedef struct cig_config {
                                      (each event record idles the stream a few us); total_ms is
                                      always timed; 0 or 1 = every call */
     uint32_t repair_delta;    /* bucket width of CIG_CFG_ORDERED_REPAIR; 0 = 128 (the reference's) */
-    uint32_t reserved[1];
+    uint32_t pool_slack;      /* edge slots each pool gets beyond 1.25x the loaded rows' capacity
+                                 (room for row growth before a compaction); 0 = 4 Mi (HBM),
+                                 1 Mi (host tier) */
 } cig_config;
 
 /* Per-call statistics. Times are device (HIP event) milliseconds of the
@@ -199,7 +207,13 @@ typedef struct cig_graph_info {
     uint32_t edges_per_lane;
     uint32_t split_degree;
     uint32_t cu_count;
-    uint32_t reserved[7];
+    uint32_t compactions;         /* pool compactions so far */
+    uint32_t inplace_compactions; /* of which packed in place (device kernels for HBM, host
+                                     threads for the host tier) */
+    uint32_t host_compactions;    /* of which of the host tier */
+    uint32_t compaction_ms_hbm;   /* wall time spent compacting each tier, milliseconds */
+    uint32_t compaction_ms_host;
+    uint32_t reserved[2];
 } cig_graph_info;
 
 /* Result fields for cig_gather. */
@@ -444,6 +458,17 @@ cig_status cig_read_updates(const char *update_path, const char *batch_path,
  * the file length (CIG_EIO otherwise); their arrays are malloc'd (free with
  * cig_host_free). */
 cig_status cig_write_csr_bin(const char *path, uint32_t n, const uint64_t *offsets, const uint32_t *dst);
+/* Out-of-core R-MAT generator (no counterpart in the reference, whose
+ * generator (generator.h:80-113) and loader (csr_graph.cuh:403-479) hold the
+ * whole edge list): writes the graph cig_rmat_csr builds from the same
+ * arguments to a CIGG file, one range of source segments at a time.
+ * `segments` (a power of two <= 2^scale; 0 = 512) are equal vertex ranges;
+ * a pass covers consecutive segments whose expected edges fit mem_bytes, so
+ * peak memory is about mem_bytes plus one pass's rows, never O(m). *m_out
+ * (may be null) receives the edge count. */
+cig_status cig_rmat_write(const char *path, uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                          uint64_t seed, uint32_t symmetric, uint32_t segments, uint64_t mem_bytes,
+                          uint32_t threads, uint64_t *m_out);
 cig_status cig_read_csr_bin(const char *path, uint32_t *n_out, uint64_t **offsets_out, uint32_t **dst_out);
 cig_status cig_write_updates_bin(const char *path, const cig_edge *adds, uint64_t n_add,
                                  const cig_edge *dels, uint64_t n_del, const uint32_t *batch_sizes,

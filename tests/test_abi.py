@@ -223,3 +223,20 @@ def test_alternative_executor_configs_are_checked():
         cfg.algo, cfg.flags = algo, flags
         assert L.cig_create(C.byref(cfg), C.byref(e)) == cg.EINVAL
         assert b"UNION_REPAIR" in L.cig_last_error(None)
+
+
+def test_out_of_core_generator_matches_in_memory(tmp_path):
+    """cig_rmat_write (source segments, bounded passes) writes exactly the graph
+    cig_rmat_csr builds from the same arguments, directed and undirected, for
+    one pass and for many."""
+    for scale, ef, sym, segments, mem in ((11, 8, False, 16, 1 << 12), (11, 8, True, 8, 1 << 14),
+                                          (10, 4, False, 1, 1 << 30), (12, 16, False, 512, 1 << 16)):
+        ref = cg.rmat_graph(scale, ef, seed=scale * 7, symmetric=sym)
+        path = str(tmp_path / f"g{scale}_{ef}_{sym}_{segments}.cigg")
+        m = cg.rmat_write(path, scale, ef, seed=scale * 7, symmetric=sym, segments=segments, mem_bytes=mem)
+        got = cg.read_csr_bin(path)
+        assert m == ref.m and got.n == ref.n
+        assert np.array_equal(got.offsets, ref.offsets), (scale, sym, segments)
+        assert np.array_equal(got.dst, ref.dst), (scale, sym, segments)
+    with pytest.raises(cg.CigError):
+        cg.rmat_write(str(tmp_path / "bad.cigg"), 10, 4, segments=3)  # not a power of two

@@ -67,3 +67,29 @@ def test_graph_batches_recapture_after_compaction():
             check_traversal(e, og, "sssp", 0, where=f"recapture batch {b}")
         assert comp > 0, "the stream never overflowed the pool"
         check_topology(e, og, "recapture final")
+
+
+@pytest.mark.parametrize("tier", ["hbm", "host"])
+def test_in_place_compaction_keeps_results(tier):
+    """A pool with almost no slack overflows from relocation garbage alone
+    (as many deletes as inserts): the tier is packed in place on the device
+    (no second pool, no host copy of the rows) and every result stays exact."""
+    print(f"[gpu-case] in-place compaction {tier}", flush=True)
+    g = cg.rmat_graph(11, 16, seed=115)
+    s = cg.rmat_stream(g, 11, 12, 1500, 1500, seed=116)
+    og = oracle_graph(g)
+    kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=3, pool_slack=64)
+    if tier == "host":
+        kw["cache_bytes"] = g.m * 4 // 5  # most rows of both directions in the host tier
+    with cg.Engine("sssp", **kw) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            e.apply_batch(dels, adds)
+            oracle_apply(og, dels, adds)
+            check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 3, where=f"{tier} batch {b}")
+        info = e.info()
+        assert info["inplace_compactions"] > 0, info
+        check_topology(e, og, f"{tier} final")

@@ -106,8 +106,9 @@ def test_sample_rmat_stream(app, tmp_path):
     args = [f"--graphfile={gp}", f"--updatefile={up}", f"--update_size={bp}", "--check=true",
             f"--output={out}", f"--{app}_max_batches=10", "--source_node=2"] + extra
     if app == "pr":
-        # the sample keeps the reference's 1..100 round limit (hybrid_pr.cu:129-132);
-        # 1e-6 converges within it on this graph
+        # the sample keeps the reference's 1..100 round limit (hybrid_pr.cu:129-132):
+        # from the initial residual of 0.15, epsilon 1e-7 needs more rounds than
+        # that on the hubs; 1e-6 converges within it on this graph
         args += ["--error=0.000001", "--pr_max_rounds=100"]
     r = _run(app, *args)
     print(r.stdout[-3000:], r.stderr[-2000:])
@@ -118,8 +119,12 @@ def test_sample_rmat_stream(app, tmp_path):
     got = np.loadtxt(out, ndmin=2)
     if app == "pr":
         assert "Overall: Test passed" in r.stdout
-        l1n, _ = pr_errors(got[:, 1].astype(np.float32), og.pagerank()[0])
-        assert l1n < 1e-4
+        ref = og.pagerank()[0]
+        l1n, _ = pr_errors(got[:, 1].astype(np.float32), ref)
+        # with every |residual| <= eps the error is (I - 0.85 P^T)^-1 r, whose L1
+        # norm is at most n * eps / 0.15: normalised by the FP64 solve's mass
+        bound = g.n * 1e-6 / 0.15 / float(np.abs(ref).sum())
+        assert l1n <= bound, (l1n, bound)
         return
     assert "Overall: Test passed" in r.stdout
     exp = {"bfs": lambda: og.bfs(2), "sssp": lambda: og.sssp(2), "cc": og.cc}[app]()
