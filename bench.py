@@ -285,8 +285,9 @@ def main(argv=None):
     # round kernels are timed (an event pair around each launch) in one batch
     # of every TIMING_EVERY: the pairs idle the stream ~10 us each, so timing
     # every batch would inflate the latency it reports
+    # (PageRank's round kernels run for milliseconds: every batch is timed)
     kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=args.weight_seed, flags=cg.CFG_KERNEL_TIMING,
-              kernel_timing_every=TIMING_EVERY)
+              kernel_timing_every=1 if pr else TIMING_EVERY)
     if c["sym"]:
         kw["flags"] |= cg.CFG_SYMMETRIC  # undirected graph: outgoing rows double as incoming rows
     if args.ordered:
@@ -367,7 +368,8 @@ def main(argv=None):
     # bytes are its pass's (stage_bytes) spread over all its launches (no-op
     # lookahead rounds, and for k_pr_pull the sparse rounds it leaves to the
     # scatter, included, as rocprofv3 counts them).
-    # Only the batches whose round kernels were timed enter (one in TIMING_EVERY).
+    # Only the batches whose round kernels were timed enter (one in TIMING_EVERY;
+    # every batch for PageRank).
     timed = [s for s in stats if s["round_launches"]]
     launches = sum(s["round_launches"] for s in timed)
     kern_ms = sum(s["round_kernel_ms"] for s in timed)

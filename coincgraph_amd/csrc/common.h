@@ -44,7 +44,7 @@ struct CopyJob {
     uint64_t from;  // pool offset with tier bit
     uint64_t to;
     uint32_t len;
-    uint32_t pad;
+    uint32_t pfirst;  // first of the job's copy pieces (k_move_rows)
 };
 
 // Device control block: counters of the current batch and of the fixed-point

@@ -246,6 +246,7 @@ struct cig_engine {
     DBuf<PullRow> prow;
     DBuf<float> tpart;
     DBuf<unsigned long long> amask;  // active-source mask of the pull passes
+    DBuf<uint32_t> odeg;             // out-degrees for the dense consume of one solve
     DBuf<unsigned long long> adm_hist, adm_acc;  // cache admission: capacity per score, sums
     uint64_t ptile_cap = 0;
     DBuf<uint64_t> cnt64, gofs;
@@ -953,6 +954,7 @@ struct cig_engine {
             a.tpart = tpart.p;
             a.amask = amask.p;
             a.mask_max = std::max<uint32_t>(1u, n / kPullMaskDivisor);
+            a.odeg = odeg.p;
         }
         return a;
     }
@@ -963,7 +965,9 @@ struct cig_engine {
         const uint64_t slots = hbm_slots + host_slots;
         const uint64_t rows_long = std::min<uint64_t>(n, slots / kPullSplit);
         const uint64_t need = slots / kPullTile + rows_long + 1;
-        if (!amask.p) amask.alloc((uint64_t(n) + 63) / 64);
+        // sized by n: a later cig_load_csr may bring a larger graph
+        if (amask.count < (uint64_t(n) + 63) / 64) amask.alloc((uint64_t(n) + 63) / 64);
+        if (odeg.count < n) odeg.alloc(n);
         if (need <= ptile_cap) return;
         ptiles.alloc(need);
         tpart.alloc(need);
@@ -996,6 +1000,7 @@ struct cig_engine {
             // k_pr_pull_long), a sparse one scatters; each kernel checks
             // the round's frontier size and returns when it is not its form
             if (!plan_ready) pull_plan(a, nullptr);
+            k_pr_degrees<<<grid_small, 256, 0, stream>>>(rows.p, n, odeg.p);
             advance_round(run_chain(
                 [&](uint64_t tag, uint32_t seq) {
                     k_pr_consume<<<grid_pull, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
@@ -1334,6 +1339,7 @@ struct cig_engine {
     // then grows the pool or demotes rows.
     DBuf<unsigned long long> cmp_win;  // wcnt | wcap | lfill | cursor, kCmpMaxWin each
     DBuf<uint32_t> cmp_list, cmp_stage;
+    DBuf<uint32_t> cmp_pc, cmp_pjob;  // per window: copy pieces; piece -> job
     DBuf<CmpJob> cmp_jobs;
     uint64_t inplace_compactions = 0;
 
@@ -1346,7 +1352,9 @@ struct cig_engine {
         while ((tail + wslots - 1) / wslots > kCmpMaxWin) wslots <<= 1;
         const uint32_t nwin = uint32_t((tail + wslots - 1) / wslots);
         if (!cmp_win.p) cmp_win.alloc(4 * kCmpMaxWin);
+        if (!cmp_pc.p) cmp_pc.alloc(kCmpMaxWin);
         fill32(reinterpret_cast<uint32_t *>(cmp_win.p), 8ull * kCmpMaxWin, 0u);
+        fill32(cmp_pc.p, kCmpMaxWin, 0u);
         CmpArgs a{};
         a.rows = rows.p;
         a.nrows = nrows;
@@ -1363,7 +1371,7 @@ struct cig_engine {
         std::vector<unsigned long long> h(2 * kCmpMaxWin);
         HIPCHK(hipMemcpyAsync(h.data(), cmp_win.p, h.size() * 8, hipMemcpyDeviceToHost, stream));
         sync();
-        uint64_t live = 0, rowsn = 0, maxcnt = 0, maxcap = 0;
+        uint64_t live = 0, rowsn = 0, maxcnt = 0, maxcap = 0, maxpieces = 0;
         std::vector<unsigned long long> lbase(kCmpMaxWin, 0), nbase(nwin, 0);
         for (uint32_t w = 0; w < nwin; ++w) {
             lbase[w] = rowsn;
@@ -1372,12 +1380,15 @@ struct cig_engine {
             live += h[kCmpMaxWin + w];
             maxcnt = std::max<uint64_t>(maxcnt, h[w]);
             maxcap = std::max<uint64_t>(maxcap, h[kCmpMaxWin + w]);
+            // a row has max(1, ceil(deg / kCmpPiece)) pieces, deg <= cap
+            maxpieces = std::max<uint64_t>(maxpieces, h[w] + h[kCmpMaxWin + w] / kCmpPiece + 1);
         }
         const uint64_t limit = tier == 0 && budget_slots ? std::min(cap, budget_slots) : cap;
-        if (live + extra > limit || maxcnt > 0xFFFFFFFFull) return false;
+        if (live + extra > limit || maxcnt > 0xFFFFFFFFull || maxpieces > 0xFFFFFFFFull) return false;
         if (cmp_list.count < rowsn) cmp_list.alloc(rowsn);
         if (cmp_jobs.count < maxcnt) cmp_jobs.alloc(maxcnt);
         if (cmp_stage.count < maxcap) cmp_stage.alloc(maxcap);
+        if (cmp_pjob.count < maxpieces) cmp_pjob.alloc(maxpieces);
         a.list = cmp_list.p;
         HIPCHK(hipMemcpyAsync(a.lfill, lbase.data(), kCmpMaxWin * 8, hipMemcpyHostToDevice, stream));
         k_cmp_scatter<<<chunks, 256, 0, stream>>>(a);
@@ -1386,9 +1397,11 @@ struct cig_engine {
         for (uint32_t w = 0; w < nwin; ++w) {
             const uint32_t cnt = uint32_t(h[w]);
             if (!cnt) continue;
-            k_cmp_plan<<<grid_small, 256, 0, stream>>>(a, cmp_list.p + lbase[w], cnt, cursor + w, cmp_jobs.p);
-            k_cmp_in<<<grid_small, 256, 0, stream>>>(a, cmp_jobs.p, cnt, cmp_stage.p);
-            k_cmp_out<<<grid_small, 256, 0, stream>>>(a, cmp_jobs.p, cnt, cmp_stage.p, nbase[w]);
+            k_cmp_plan<<<grid_small, 256, 0, stream>>>(a, cmp_list.p + lbase[w], cnt, cursor + w, cmp_pc.p + w,
+                                                       cmp_jobs.p, cmp_pjob.p);
+            k_cmp_in<<<grid_small, 256, 0, stream>>>(a, cmp_jobs.p, cmp_pjob.p, cmp_pc.p + w, cmp_stage.p);
+            k_cmp_out<<<grid_small, 256, 0, stream>>>(a, cmp_jobs.p, cmp_pjob.p, cmp_pc.p + w, cmp_stage.p,
+                                                      nbase[w]);
             HIPCHK(hipGetLastError());
         }
         const uint64_t zero = 0;
@@ -1449,33 +1462,45 @@ struct cig_engine {
             for (uint64_t v = lo; v < hi; ++v)
                 if (in_host(hrows[v])) list[tcnt[size_t(t) * nwin + (hrows[v].off & ~kHostTier) / wslots]++] = uint32_t(v);
         });
+        // One pool of T threads walks the windows in order; per window each
+        // thread takes a contiguous slice of the window's rows: it sums their
+        // capacities (the slices' sums give each slice its base), copies its
+        // rows into the staging buffer, and after a barrier (every row of the
+        // window read) writes them to their packed places.
         std::vector<uint32_t> stage(maxcap);
-        std::vector<uint64_t> pos;
+        std::vector<uint64_t> part(size_t(T) + 1, 0);
+        SpinBarrier team_bar(T), solo_bar(1);
         uint32_t *pool = host_pool.p;
-        for (uint32_t w = 0; w < nwin; ++w) {
-            const uint64_t b = wbase[w], e = wbase[w + 1];
-            if (b == e) continue;
-            pos.resize(e - b);
-            uint64_t run = 0;
-            for (uint64_t i = b; i < e; ++i) {
-                pos[i - b] = run;
-                run += hrows[list[i]].cap;
+        run_team(T, [&](unsigned t, unsigned team) {
+            SpinBarrier &bar = team == T ? team_bar : solo_bar;
+            for (uint32_t w = 0; w < nwin; ++w) {
+                const uint64_t b = wbase[w], e = wbase[w + 1];
+                if (b == e) continue;
+                const uint64_t chunk = (e - b + team - 1) / team;
+                const uint64_t lo = std::min(e, b + t * chunk), hi = std::min(e, lo + chunk);
+                uint64_t sum = 0;
+                for (uint64_t i = lo; i < hi; ++i) sum += hrows[list[i]].cap;
+                part[t + 1] = sum;
+                bar.wait();
+                uint64_t at = 0;
+                for (unsigned j = 1; j <= t; ++j) at += part[j];
+                const uint64_t first = at;
+                for (uint64_t i = lo; i < hi; ++i) {
+                    const Row &r = hrows[list[i]];
+                    std::memcpy(stage.data() + at, pool + (r.off & ~kHostTier), size_t(r.deg) * 4);
+                    at += r.cap;
+                }
+                bar.wait();  // the window's rows are all in the staging buffer
+                at = first;
+                for (uint64_t i = lo; i < hi; ++i) {
+                    Row &r = hrows[list[i]];
+                    r.off = (nbase[w] + at) | kHostTier;
+                    std::memcpy(pool + (nbase[w] + at), stage.data() + at, size_t(r.deg) * 4);
+                    at += r.cap;
+                }
+                bar.wait();  // part and the staging buffer are free for the next window
             }
-            // all of the window's rows are read before any is written
-            parallel_for(e - b, T, [&](uint64_t lo, uint64_t hi, unsigned) {
-                for (uint64_t i = lo; i < hi; ++i) {
-                    const Row &r = hrows[list[b + i]];
-                    std::memcpy(stage.data() + pos[i], pool + (r.off & ~kHostTier), size_t(r.deg) * 4);
-                }
-            });
-            parallel_for(e - b, T, [&](uint64_t lo, uint64_t hi, unsigned) {
-                for (uint64_t i = lo; i < hi; ++i) {
-                    Row &r = hrows[list[b + i]];
-                    r.off = (nbase[w] + pos[i]) | kHostTier;
-                    std::memcpy(pool + (r.off & ~kHostTier), stage.data() + pos[i], size_t(r.deg) * 4);
-                }
-            });
-        }
+        });
         HIPCHK(hipMemcpy(rows.p, hrows.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice));
         const uint64_t zero = 0;
         HIPCHK(hipMemcpy(&ctl.p->pool_tail[1], &live, sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -1521,24 +1546,30 @@ struct cig_engine {
             if (tail + need > cap) throw HipError(CIG_ENOMEM, "tier pool cannot hold moved rows");
         }
         std::vector<CopyJob> hj;
+        std::vector<uint32_t> hp;  // copy piece -> job
         for (uint32_t v : vs) {
             Row &r = hrows[v];
             CopyJob j{};
             j.from = r.off;
             j.to = tail | (tier ? kHostTier : 0ull);
             j.len = r.deg;
+            j.pfirst = uint32_t(hp.size());
+            const uint32_t np = std::max(1u, (r.deg + kCmpPiece - 1) / kCmpPiece);
+            hp.insert(hp.end(), np, uint32_t(hj.size()));
             hj.push_back(j);
             r.off = j.to;
             tail += r.cap;
         }
         DBuf<CopyJob> dj;
         dj.alloc(hj.size());
-        DBuf<uint32_t> dcount;
+        DBuf<uint32_t> dpj, dcount;
+        dpj.alloc(hp.size());
         dcount.alloc(1);
-        const uint32_t nj = uint32_t(hj.size());
+        const uint32_t np = uint32_t(hp.size());
         HIPCHK(hipMemcpy(dj.p, hj.data(), hj.size() * sizeof(CopyJob), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dcount.p, &nj, sizeof(uint32_t), hipMemcpyHostToDevice));
-        k_move_rows<<<grid_small, 256, 0, stream>>>(dj.p, dcount.p, pools());
+        HIPCHK(hipMemcpy(dpj.p, hp.data(), hp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dcount.p, &np, sizeof(uint32_t), hipMemcpyHostToDevice));
+        k_move_rows<<<grid_small, 256, 0, stream>>>(dj.p, dpj.p, dcount.p, pools());
         HIPCHK(hipGetLastError());
         sync();
         HIPCHK(hipMemcpy(rows.p, hrows.data(), size_t(nrows) * sizeof(Row), hipMemcpyHostToDevice));
@@ -2825,8 +2856,14 @@ struct cig_engine {
         }
         DBuf<CopyJob> jobs;
         jobs.alloc(count);
-        k_move_plan<<<grid_small, 256, 0, stream>>>(rows.p, list, dcount, uint32_t(tier), ctl.p, jobs.p);
-        k_move_rows<<<grid_small, 256, 0, stream>>>(jobs.p, dcount, pools());
+        // a row has max(1, ceil(deg / kCmpPiece)) copy pieces, deg <= cap
+        DBuf<uint32_t> pjob, pcount;
+        pjob.alloc(uint64_t(count) + need / kCmpPiece + 1);
+        pcount.alloc(1);
+        fill32(pcount.p, 1, 0u);
+        k_move_plan<<<grid_small, 256, 0, stream>>>(rows.p, list, dcount, uint32_t(tier), ctl.p, jobs.p, pcount.p,
+                                                    pjob.p);
+        k_move_rows<<<grid_small, 256, 0, stream>>>(jobs.p, pjob.p, pcount.p, pools());
         HIPCHK(hipGetLastError());
         sync();  // the job list is freed on return
     }

@@ -1,6 +1,7 @@
 // Host-side thread helpers of libcig (graph synthesis and graph loading).
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <thread>
 #include <vector>
@@ -33,6 +34,51 @@ void parallel_for(uint64_t n, unsigned threads, F f) {
         throw;
     }
     for (auto &th : pool) th.join();
+}
+
+// Barrier for a fixed team of threads (spins, then yields).
+class SpinBarrier {
+  public:
+    explicit SpinBarrier(unsigned n) : n_(n) {}
+    void wait() {
+        const unsigned g = gen_.load(std::memory_order_acquire);
+        if (count_.fetch_add(1, std::memory_order_acq_rel) + 1 == n_) {
+            count_.store(0, std::memory_order_relaxed);
+            gen_.fetch_add(1, std::memory_order_release);
+            return;
+        }
+        for (unsigned spin = 0; gen_.load(std::memory_order_acquire) == g; ++spin)
+            if (spin > 1024) std::this_thread::yield();
+    }
+
+  private:
+    std::atomic<unsigned> count_{0}, gen_{0};
+    const unsigned n_;
+};
+
+// Runs f(t, team) on a team of `threads` threads (t = 0 on the caller) that
+// start together; if a thread cannot be created, the started ones return at
+// once and f runs alone as a team of one.
+template <class F>
+void run_team(unsigned threads, F f) {
+    std::atomic<int> go{0};  // 1: run, -1: give up
+    std::vector<std::thread> pool;
+    pool.reserve(threads);
+    auto member = [&](unsigned t) {
+        int g;
+        while ((g = go.load(std::memory_order_acquire)) == 0) std::this_thread::yield();
+        if (g > 0) f(t, threads);
+    };
+    bool ok = true;
+    try {
+        for (unsigned t = 1; t < threads; ++t) pool.emplace_back(member, t);
+    } catch (...) {
+        ok = false;
+    }
+    go.store(ok ? 1 : -1, std::memory_order_release);
+    if (ok) f(0u, threads);
+    for (auto &th : pool) th.join();
+    if (!ok) f(0u, 1u);
 }
 
 }  // namespace cig

@@ -1518,7 +1518,15 @@ struct PrArgs {
     // mask_max frontier vertices (and amendments) test it before gathering
     unsigned long long *amask;
     uint32_t mask_max;
+    // out-degrees of the current rows (k_pr_degrees, once per solve): the
+    // dense consume reads 4 B per vertex here instead of its 16-B row
+    uint32_t *odeg;
 };
+
+// Out-degree of every vertex for the dense consume passes of one solve.
+__global__ void k_pr_degrees(const Row *rows, uint32_t n, uint32_t *odeg) {
+    for (uint32_t v = gthread(); v < n; v += gthreads()) odeg[v] = rows[v].deg;
+}
 
 __device__ __forceinline__ bool pr_dense(const PrArgs &a, uint32_t nq) { return nq != 0 && nq >= a.pull_min; }
 
@@ -1633,7 +1641,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_consume(PrArgs a, uint64_t *prog,
                     const float t = x + y;
                     a.comp[v] = (t - x) - y;
                     a.rank[v] = t;
-                    const uint32_t deg = a.rows[v].deg;
+                    const uint32_t deg = a.odeg[v];
                     ctr.vexp += 1;
                     if (deg) d = 0.85f * rv / float(deg);
                 }
@@ -2354,7 +2362,16 @@ struct CmpJob {
     uint64_t spos;   // position in the staging buffer = offset within the window's packed range
     uint32_t len;    // live slots
     uint32_t row;
+    uint32_t pfirst;  // first of the row's copy pieces
+    uint32_t pad;
 };
+
+// Rows are copied in pieces of kCmpPiece slots, a wave per piece, so that a
+// hub row of millions of slots is spread over the whole chip. Row blocks are
+// powers of two of at least 4 slots at offsets that are sums of such blocks,
+// so every row (and its staging place) starts 16-B aligned and rounding its
+// length up to whole quads stays inside its block.
+constexpr uint32_t kCmpPiece = 2048;
 
 struct CmpArgs {
     Row *rows;
@@ -2415,53 +2432,70 @@ __global__ void k_cmp_scatter(CmpArgs a) {
 }
 
 // One window: each row's place in the packed range (workgroup-aggregated
-// reservations on the window's cursor) and its copy job.
-__global__ void k_cmp_plan(CmpArgs a, const uint32_t *list, uint32_t cnt, unsigned long long *cursor, CmpJob *jobs) {
+// reservations on the window's cursor), its copy pieces and its job.
+__global__ void k_cmp_plan(CmpArgs a, const uint32_t *list, uint32_t cnt, unsigned long long *cursor,
+                           uint32_t *pcursor, CmpJob *jobs, uint32_t *pjob) {
     __shared__ unsigned long long scratch64[kMaxWavesPerBlock + 1];
+    __shared__ uint32_t scratch32[kMaxWavesPerBlock + 1];
     for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += gthreads()) {
         const uint32_t i = base + threadIdx.x;
         Row r{};
-        uint32_t v = 0;
+        uint32_t v = 0, np = 0;
         if (i < cnt) {
             v = list[i];
             r = a.rows[v];
+            np = max(1u, (r.deg + kCmpPiece - 1) / kCmpPiece);  // an empty row still gets its offset
         }
         const uint64_t pos = block_reserve<unsigned long long>(i < cnt ? r.cap : 0ull, cursor, scratch64);
-        if (i < cnt) jobs[i] = CmpJob{r.off, pos, r.deg, v};
+        const uint32_t pf = block_reserve<uint32_t>(np, pcursor, scratch32);
+        if (i < cnt) {
+            jobs[i] = CmpJob{r.off, pos, r.deg, v, pf, 0u};
+            for (uint32_t k = 0; k < np; ++k) pjob[pf + k] = i;
+        }
     }
 }
 
-// Live slots into the staging buffer (a wave per row).
-__global__ void k_cmp_in(CmpArgs a, const CmpJob *jobs, uint32_t cnt, uint32_t *stage) {
-    for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
-        const CmpJob jb = jobs[j];
-        const uint32_t *s = a.pools.row(jb.from);
-        for (uint32_t q = lane_id(); q < jb.len; q += kWave) stage[jb.spos + q] = s[q];
+// Copies slots [k*kCmpPiece, ...) of a row, whole quads, by one wave.
+__device__ __forceinline__ void cmp_copy_piece(uint32_t *d, const uint32_t *s, uint32_t len, uint32_t k) {
+    const uint32_t lo = k * kCmpPiece, hi = min(len, lo + kCmpPiece);
+    if (hi <= lo) return;
+    const uint32_t nq = (hi - lo + 3u) >> 2;
+    const uint4 *sq = reinterpret_cast<const uint4 *>(s + lo);
+    uint4 *dq = reinterpret_cast<uint4 *>(d + lo);
+    for (uint32_t q = lane_id(); q < nq; q += kWave) dq[q] = sq[q];
+}
+
+// Live slots into the staging buffer (a wave per piece).
+__global__ void k_cmp_in(CmpArgs a, const CmpJob *jobs, const uint32_t *pjob, const uint32_t *pcount,
+                         uint32_t *stage) {
+    const uint32_t np = *pcount;
+    for (uint32_t p = gwave(); p < np; p += gwaves()) {
+        const CmpJob jb = jobs[pjob[p]];
+        cmp_copy_piece(stage + jb.spos, a.pools.row(jb.from), jb.len, p - jb.pfirst);
     }
 }
 
 // From the staging buffer to the final place, which the row now names.
-__global__ void k_cmp_out(CmpArgs a, const CmpJob *jobs, uint32_t cnt, const uint32_t *stage, uint64_t newbase) {
+__global__ void k_cmp_out(CmpArgs a, const CmpJob *jobs, const uint32_t *pjob, const uint32_t *pcount,
+                          const uint32_t *stage, uint64_t newbase) {
     const uint64_t tbit = a.tier ? kHostTier : 0ull;
-    for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
-        const CmpJob jb = jobs[j];
+    const uint32_t np = *pcount;
+    for (uint32_t p = gwave(); p < np; p += gwaves()) {
+        const CmpJob jb = jobs[pjob[p]];
         const uint64_t off = (newbase + jb.spos) | tbit;
-        uint32_t *d = a.pools.row(off);
-        for (uint32_t q = lane_id(); q < jb.len; q += kWave) d[q] = stage[jb.spos + q];
-        if (lane_id() == 0) a.rows[jb.row].off = off;
+        cmp_copy_piece(a.pools.row(off), stage + jb.spos, jb.len, p - jb.pfirst);
+        if (p == jb.pfirst && lane_id() == 0) a.rows[jb.row].off = off;
     }
 }
 
 // Moves rows between tiers (cache admission / eviction). Each job names a row
 // whose descriptor has already been rewritten; `from` is the old location.
-__global__ void k_move_rows(const CopyJob *jobs, const uint32_t *njobs, Pools pools) {
-    const uint32_t cnt = ld_acquire(njobs);
-    const uint32_t lane = lane_id();
-    for (uint32_t j = gwave(); j < cnt; j += gwaves()) {
-        const CopyJob jb = jobs[j];
-        const uint32_t *s = pools.row(jb.from);
-        uint32_t *d = pools.row(jb.to);
-        for (uint32_t q = lane; q < jb.len; q += kWave) d[q] = s[q];
+// A wave per copy piece (see kCmpPiece), whole quads.
+__global__ void k_move_rows(const CopyJob *jobs, const uint32_t *pjob, const uint32_t *pcount, Pools pools) {
+    const uint32_t np = ld_acquire(pcount);
+    for (uint32_t p = gwave(); p < np; p += gwaves()) {
+        const CopyJob jb = jobs[pjob[p]];
+        cmp_copy_piece(pools.row(jb.to), pools.row(jb.from), jb.len, p - jb.pfirst);
     }
 }
 
@@ -2536,29 +2570,33 @@ __global__ void k_adm_mark(AdmArgs a, uint32_t pass) {
 // tail, one reservation per workgroup step), the copy jobs, the rows' new
 // offsets; the vacated blocks become garbage of the other tier.
 __global__ void k_move_plan(Row *rows, const uint32_t *list, const uint32_t *count, uint32_t tier, Ctl *c,
-                            CopyJob *jobs) {
+                            CopyJob *jobs, uint32_t *pcount, uint32_t *pjob) {
     __shared__ unsigned long long s64[kMaxWavesPerBlock + 1];
+    __shared__ uint32_t s32[kMaxWavesPerBlock + 1];
     const uint32_t cnt = ld_acquire(count);
     for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += gthreads()) {
         const uint32_t i = base + threadIdx.x;
         Row r{};
-        uint32_t v = 0;
+        uint32_t v = 0, np = 0;
         if (i < cnt) {
             v = list[i];
             r = rows[v];
+            np = max(1u, (r.deg + kCmpPiece - 1) / kCmpPiece);
         }
         const unsigned long long at = block_reserve<unsigned long long>(
             (unsigned long long)r.cap, reinterpret_cast<unsigned long long *>(&c->pool_tail[tier]), s64);
         block_reserve<unsigned long long>((unsigned long long)r.cap,
                                           reinterpret_cast<unsigned long long *>(&c->garbage[1 - tier]), s64);
+        const uint32_t pf = block_reserve<uint32_t>(np, pcount, s32);
         if (i < cnt) {
             CopyJob j;
             j.from = r.off;
             j.to = at | (tier ? kHostTier : 0ull);
             j.len = r.deg;
-            j.pad = 0;
+            j.pfirst = pf;
             jobs[i] = j;
             rows[v].off = j.to;
+            for (uint32_t k = 0; k < np; ++k) pjob[pf + k] = i;
         }
     }
 }
