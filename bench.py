@@ -645,7 +645,8 @@ def serve_bench(args, c):
                                         "device_bytes", "host_pinned_bytes")},
         "compactions": {"total": end_info["compactions"], "in_place": end_info["inplace_compactions"],
                         "host_tier": end_info["host_compactions"], "ms_hbm": end_info["compaction_ms_hbm"],
-                        "ms_host": end_info["compaction_ms_host"]},
+                        "ms_host": end_info["compaction_ms_host"],
+                        "refreshes_deferred": end_info["refreshes_deferred"]},
     }
     print(json.dumps(out), flush=True)
     return 0

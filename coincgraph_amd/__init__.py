@@ -143,7 +143,7 @@ class GraphInfo(C.Structure):
         ("edges_per_lane", C.c_uint32), ("split_degree", C.c_uint32),
         ("cu_count", C.c_uint32), ("compactions", C.c_uint32), ("inplace_compactions", C.c_uint32),
         ("host_compactions", C.c_uint32), ("compaction_ms_hbm", C.c_uint32),
-        ("compaction_ms_host", C.c_uint32), ("reserved", C.c_uint32 * 2),
+        ("compaction_ms_host", C.c_uint32), ("refreshes_deferred", C.c_uint32), ("reserved", C.c_uint32 * 1),
     ]
 
     def as_dict(self):

@@ -632,7 +632,10 @@ struct cig_engine {
         }
         const uint64_t slack = cfg.pool_slack ? cfg.pool_slack : (uint64_t(1) << 22);
         hbm_slots = budget_slots ? budget_slots : hbm_need + hbm_need / 4 + slack;
-        host_slots = host_need ? host_need + host_need / 4 + std::min<uint64_t>(slack, uint64_t(1) << 20)
+        // the host tier takes the budget's evictions and the growth of the cold
+        // rows, and packing it costs seconds at scale 27: half its rows' capacity
+        // again as headroom (pinned host memory is the cheaper resource)
+        host_slots = host_need ? host_need + host_need / 2 + std::min<uint64_t>(slack, uint64_t(1) << 20)
                                : (uint64_t(1) << 16);
         hbm_pool.alloc(hbm_slots);
         host_pool.alloc(host_slots);
@@ -1254,6 +1257,7 @@ struct cig_engine {
     // allocation with room for `extra` more slots. HBM is bounded by the
     // cache budget: rows that do not fit are demoted to the host tier.
     uint64_t host_compactions = 0;
+    uint64_t refreshes_deferred = 0;  // refreshes that moved nothing rather than compact the host tier
     double compaction_ms[2] = {0.0, 0.0};
 
     void compact_pool(int tier, uint64_t extra) {
@@ -1466,7 +1470,10 @@ struct cig_engine {
         // thread takes a contiguous slice of the window's rows: it sums their
         // capacities (the slices' sums give each slice its base), copies its
         // rows into the staging buffer, and after a barrier (every row of the
-        // window read) writes them to their packed places.
+        // window read) writes them to their packed places. A window whose
+        // packed range ends at or below its own first slot (the garbage before
+        // it is at least its capacity) overlaps no unread row: its rows are
+        // copied straight to their places, with no staging pass.
         std::vector<uint32_t> stage(maxcap);
         std::vector<uint64_t> part(size_t(T) + 1, 0);
         SpinBarrier team_bar(T), solo_bar(1);
@@ -1485,6 +1492,17 @@ struct cig_engine {
                 uint64_t at = 0;
                 for (unsigned j = 1; j <= t; ++j) at += part[j];
                 const uint64_t first = at;
+                if (nbase[w] + wcap[w] <= uint64_t(w) * wslots) {
+                    for (uint64_t i = lo; i < hi; ++i) {
+                        Row &r = hrows[list[i]];
+                        const uint64_t from = r.off & ~kHostTier;
+                        r.off = (nbase[w] + at) | kHostTier;
+                        std::memcpy(pool + (nbase[w] + at), pool + from, size_t(r.deg) * 4);
+                        at += r.cap;
+                    }
+                    bar.wait();  // part is free for the next window
+                    continue;
+                }
                 for (uint64_t i = lo; i < hi; ++i) {
                     const Row &r = hrows[list[i]];
                     std::memcpy(stage.data() + at, pool + (r.off & ~kHostTier), size_t(r.deg) * 4);
@@ -2791,7 +2809,10 @@ struct cig_engine {
     // (framework.cuh:605-769, 1873-1956) on the device: age the activity
     // window, histogram the capacities by score, pick the threshold on the
     // host from 1024 sums, list the rows that change tier, move them.
-    void cache_refresh(cig_batch_stats *st) {
+    // host_compaction: whether the evictions may compact the host tier when
+    // its tail has no room (seconds at scale 27); without it such a refresh
+    // moves nothing (rows stay where they are, results do not change).
+    void cache_refresh(cig_batch_stats *st, bool host_compaction = true) {
         check_ready();
         begin_call(st);
         k_age_hotness<<<grid_small, 256, 0, stream>>>(hot.p, hist.p, score.p, nrows);
@@ -2836,9 +2857,14 @@ struct cig_engine {
             unsigned long long acc[4];
             HIPCHK(hipMemcpyAsync(acc, adm_acc.p, sizeof(acc), hipMemcpyDeviceToHost, stream));
             read_ctl();
-            if (hctl.p->small_cnt) move_rows_device(small_list.p, &ctl.p->small_cnt, hctl.p->small_cnt, 1, acc[1]);
-            if (hctl.p->big_cnt) move_rows_device(big_list.p, &ctl.p->big_cnt, hctl.p->big_cnt, 0, acc[2]);
-            moved = uint64_t(hctl.p->small_cnt) + hctl.p->big_cnt;
+            const bool fits = host_compaction || hctl.p->pool_tail[1] + acc[1] <= hctl.p->pool_cap[1];
+            if (fits) {
+                if (hctl.p->small_cnt) move_rows_device(small_list.p, &ctl.p->small_cnt, hctl.p->small_cnt, 1, acc[1]);
+                if (hctl.p->big_cnt) move_rows_device(big_list.p, &ctl.p->big_cnt, hctl.p->big_cnt, 0, acc[2]);
+                moved = uint64_t(hctl.p->small_cnt) + hctl.p->big_cnt;
+            } else {
+                ++refreshes_deferred;
+            }
         }
         end_call(st, 0);
         if (st) st->relocated_rows = moved;  // rows that changed tier
@@ -2967,6 +2993,7 @@ struct cig_engine {
         gi->host_compactions = uint32_t(host_compactions);
         gi->compaction_ms_hbm = uint32_t(compaction_ms[0]);
         gi->compaction_ms_host = uint32_t(compaction_ms[1]);
+        gi->refreshes_deferred = uint32_t(refreshes_deferred);
         gi->grid_blocks = grid_coop;
         if (!loaded) return;
         sync();
@@ -3301,7 +3328,7 @@ struct cig_server {
         }
         groups_since_refresh = 0;
         try {
-            e->cache_refresh(nullptr);
+            e->cache_refresh(nullptr, false);  // never a host-tier compaction between groups
             ++refreshes;
         } catch (...) {
             std::string err;

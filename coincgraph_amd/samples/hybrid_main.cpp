@@ -264,6 +264,10 @@ bool run() {
     cfg.cache_bytes = cache_bytes_from_flag(fd("cache"));
     cfg.n_stream = uint32_t(fi("n_stream"));
     cfg.segments = uint32_t(segments);
+    if (segments != 1 || fi("n_stream") != 3)
+        std::printf("[NOTE] --SEGMENT/--n_stream are accepted for compatibility: the engine runs each pass "
+                    "as one whole-graph launch on one compute stream (segments=%lld, n_stream=%lld)\n",
+                    segments, fi("n_stream"));
     cfg.block_size = uint32_t(fi("block_size"));
     if (fb("symmetric")) cfg.flags |= CIG_CFG_SYMMETRIC;
     if (kApp == "pr") {

@@ -213,7 +213,9 @@ typedef struct cig_graph_info {
     uint32_t host_compactions;    /* of which of the host tier */
     uint32_t compaction_ms_hbm;   /* wall time spent compacting each tier, milliseconds */
     uint32_t compaction_ms_host;
-    uint32_t reserved[2];
+    uint32_t refreshes_deferred;  /* serving-loop refreshes that moved nothing because the host tier
+                                     would have needed a compaction */
+    uint32_t reserved[1];
 } cig_graph_info;
 
 /* Result fields for cig_gather. */
@@ -330,7 +332,9 @@ typedef struct cig_serve_config {
     uint64_t hbm_reserve;        /* bytes of free HBM admission leaves untouched */
     uint32_t refresh_every;      /* cache refresh (cig_cache_refresh) after this many update groups,
                                     when no request waits; 0 = never; 1 = after every group, as the
-                                    reference after every batch (default 8; engines with a budget) */
+                                    reference after every batch (default 8; engines with a budget).
+                                    A refresh whose evictions would need a host-tier compaction
+                                    moves nothing (cig_graph_info.refreshes_deferred) */
     uint32_t max_query_wait_ms;  /* a query waiting this long is served before further update
                                     groups (default 250) */
     uint64_t reserved[3];
