@@ -212,6 +212,7 @@ def lib():
                                         C.POINTER(BatchStats)]),
         "cig_update_batches_left": (C.c_uint32, [eng]),
         "cig_cache_refresh": (C.c_int, [eng, C.POINTER(BatchStats)]),
+        "cig_compact": (C.c_int, [eng, C.c_uint32]),
         "cig_gather": (C.c_int, [eng, C.c_uint32, P, C.c_size_t]),
         "cig_rmat_csr": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                    C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
@@ -609,6 +610,11 @@ class Engine:
         st = BatchStats()
         self._call(lib().cig_cache_refresh(self._e, C.byref(st)))
         return st.as_dict()
+
+    def compact(self, tier) -> None:
+        """Packs the 'hbm' (0) or 'host' (1) tier's edge pool now."""
+        t = {"hbm": 0, "host": 1}.get(tier, tier)
+        self._call(lib().cig_compact(self._e, int(t)))
 
     def info(self) -> dict:
         gi = GraphInfo()

@@ -45,6 +45,8 @@ struct CopyJob {
     uint64_t to;
     uint32_t len;
     uint32_t pfirst;  // first of the job's copy pieces (k_move_rows)
+    uint32_t cap;     // block capacity (slots)
+    uint32_t pad;
 };
 
 // Device control block: counters of the current batch and of the fixed-point

@@ -245,6 +245,13 @@ struct cig_engine {
     DBuf<Tile> ptiles;
     DBuf<PullRow> prow;
     DBuf<float> tpart;
+    DBuf<unsigned long long> fl_off;  // host-tier free blocks by class (FreeList), when the tier holds rows
+    DBuf<int> fl_cnt;
+    static constexpr uint32_t kFlCap = 1u << 20;  // blocks listed per class
+    FreeList free_list() const { return FreeList{fl_off.p, fl_cnt.p, fl_off.p ? kFlCap : 0u}; }
+    void clear_free_list() {
+        if (fl_cnt.p) fill32(reinterpret_cast<uint32_t *>(fl_cnt.p), kFlClasses, 0u);
+    }
     DBuf<unsigned long long> amask;  // active-source mask of the pull passes
     DBuf<uint32_t> odeg;             // out-degrees for the dense consume of one solve
     DBuf<unsigned long long> adm_hist, adm_acc;  // cache admission: capacity per score, sums
@@ -640,6 +647,16 @@ struct cig_engine {
         hbm_pool.alloc(hbm_slots);
         host_pool.alloc(host_slots);
         host_pool_dev = host_pool.device_ptr();
+        if (host_need) {
+            if (!fl_off.p) {
+                fl_off.alloc(uint64_t(kFlClasses) * kFlCap);
+                fl_cnt.alloc(kFlClasses);
+            }
+        } else {
+            fl_off.release();
+            fl_cnt.release();
+        }
+        clear_free_list();
         // rows keep row-id order within each tier; the copies run in parallel
         uint64_t htail = 0, ctail = 0;
         for (uint32_t v = 0; v < nrows; ++v) {
@@ -1138,6 +1155,7 @@ struct cig_engine {
         m.ctl = ctl.p;
         m.rows = rows.p;
         m.pools = pools();
+        m.fl = free_list();
         m.n = n;
         m.rev = rev ? 1u : 0u;
         m.hot = (cfg.flags & CIG_CFG_NO_HOTNESS) || !budget_slots ? nullptr : hot.p;
@@ -1185,6 +1203,7 @@ struct cig_engine {
         }
         k_add_plan<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
         k_add_copy<<<grid_small, kBlock, 0, stream>>>(m);
+        if (m.fl.off) k_fl_release<<<grid_small, 256, 0, stream>>>(m);
         HIPCHK(hipGetLastError());
     }
 
@@ -1247,6 +1266,7 @@ struct cig_engine {
         mm.pieces = pieces.p;
         k_add_plan<<<grid_small, 256, 0, stream>>>(mm, add_list.p, &ctl.p->add_cnt);
         k_add_copy<<<grid_small, kBlock, 0, stream>>>(mm);
+        if (mm.fl.off) k_fl_release<<<grid_small, 256, 0, stream>>>(mm);
         HIPCHK(hipGetLastError());
         read_ctl();
         if (hctl.p->overflow_cnt) throw HipError(CIG_ENOMEM, "edge pool exhausted after compaction");
@@ -1325,6 +1345,7 @@ struct cig_engine {
             std::swap(host_pool.count, np.count);
             host_pool_dev = host_pool.device_ptr();
             host_slots = cap;
+            clear_free_list();
         }
         uint64_t vals[2] = {tail, cap};
         uint64_t zero = 0;
@@ -1523,6 +1544,7 @@ struct cig_engine {
         const uint64_t zero = 0;
         HIPCHK(hipMemcpy(&ctl.p->pool_tail[1], &live, sizeof(uint64_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(&ctl.p->garbage[1], &zero, sizeof(uint64_t), hipMemcpyHostToDevice));
+        clear_free_list();  // the listed blocks lay in the garbage just packed away
         ++inplace_compactions;
         return true;
     }
@@ -2888,8 +2910,9 @@ struct cig_engine {
         pcount.alloc(1);
         fill32(pcount.p, 1, 0u);
         k_move_plan<<<grid_small, 256, 0, stream>>>(rows.p, list, dcount, uint32_t(tier), ctl.p, jobs.p, pcount.p,
-                                                    pjob.p);
+                                                    pjob.p, free_list());
         k_move_rows<<<grid_small, 256, 0, stream>>>(jobs.p, pjob.p, pcount.p, pools());
+        if (tier == 0 && fl_off.p) k_move_release<<<grid_small, 256, 0, stream>>>(jobs.p, dcount, free_list());
         HIPCHK(hipGetLastError());
         sync();  // the job list is freed on return
     }
@@ -3487,6 +3510,18 @@ uint32_t cig_update_batches_left(const cig_engine *e) {
 
 cig_status cig_cache_refresh(cig_engine *e, cig_batch_stats *st) {
     return guarded(e, [&] { e->cache_refresh(st); });
+}
+
+cig_status cig_compact(cig_engine *e, uint32_t tier) {
+    return guarded(e, [&] {
+        if (tier > 1) throw HipError(CIG_EINVAL, "tier is 0 (HBM) or 1 (host)");
+        if (!e->loaded) throw HipError(CIG_ESTATE, "no graph loaded");
+        e->flush_finish();
+        e->read_ctl();
+        if (e->hctl.p->pool_tail[tier] == 0) return;  // nothing placed in this tier
+        e->compact_pool(int(tier), 0);
+        e->sync();
+    });
 }
 
 cig_status cig_gather(cig_engine *e, uint32_t field, void *out, size_t bytes) {

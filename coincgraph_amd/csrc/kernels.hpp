@@ -130,10 +130,45 @@ __global__ void k_map_records(cig_edge *r, uint64_t cnt, const uint32_t *perm, u
     }
 }
 
+// Freed blocks of the pinned host tier, one stack per size class (class =
+// log2 of the block's slots). A relocated host-tier row's old block is pushed
+// once its slots have been copied (k_fl_release, after k_add_copy; tier
+// moves: k_move_release after k_move_rows), and a later relocation or
+// eviction of the same class pops it instead of growing the tail. Pushes and
+// pops never share a launch. A full stack drops the block, which stays
+// garbage until the tier is packed; packing the tier empties every stack.
+struct FreeList {
+    unsigned long long *off;  // [kFlClasses * cap] block offsets (tier bit clear); null: no list
+    int *cnt;                 // [kFlClasses]
+    uint32_t cap;
+};
+constexpr uint32_t kFlClasses = 32;
+
+__device__ __forceinline__ uint32_t fl_class(uint64_t cap) { return 63u - uint32_t(__clzll(cap)); }
+
+__device__ __forceinline__ bool fl_pop(const FreeList &f, uint32_t cls, uint64_t &off) {
+    if (!f.off) return false;
+    const int old = atomicSub(&f.cnt[cls], 1);
+    if (old > 0) {
+        off = f.off[uint64_t(cls) * f.cap + uint32_t(old - 1)];
+        return true;
+    }
+    atomicAdd(&f.cnt[cls], 1);  // empty: undo (no push runs in the same launch)
+    return false;
+}
+
+__device__ __forceinline__ void fl_push(const FreeList &f, uint32_t cls, uint64_t off) {
+    if (!f.off) return;
+    const int at = atomicAdd(&f.cnt[cls], 1);
+    if (at < int(f.cap)) f.off[uint64_t(cls) * f.cap + uint32_t(at)] = off;
+    else atomicSub(&f.cnt[cls], 1);  // full: the block stays garbage
+}
+
 struct MutArgs {
     Ctl *ctl;
     Row *rows;
     Pools pools;
+    FreeList fl;       // host-tier free blocks (see FreeList)
     uint32_t n;
     uint32_t rev;      // 1: rows [n, 2n) hold the incoming rows and get every update mirrored
     const cig_edge *dels;
@@ -750,8 +785,9 @@ struct AddPiece {
     const uint32_t *src;
     uint32_t *dst;
     uint32_t len;
-    uint32_t pad;
+    uint32_t flags;  // kFlRelease | class: src is a host-tier block to list once copied
 };
+constexpr uint32_t kFlRelease = 0x80000000u;
 constexpr uint32_t kPieceLen = 1024;
 
 __device__ __forceinline__ uint32_t pieces_of(uint64_t len) { return uint32_t((len + kPieceLen - 1) / kPieceLen); }
@@ -779,12 +815,15 @@ __global__ void k_add_plan(MutArgs a, const uint32_t *list, const uint32_t *list
             grow = newdeg > r.cap;
             if (grow) newcap = next_pow2_dev(newdeg);
         }
+        // a growing host-tier row takes a freed block of its new class when one is listed
+        uint64_t popped = 0;
+        const bool reuse = grow && tier == 1 && newcap <= 0xFFFFFFFFull && fl_pop(a.fl, fl_class(newcap), popped);
         const uint64_t o0 = block_reserve<unsigned long long>(grow && tier == 0 ? newcap : 0ull,
                                                               (unsigned long long *)&a.ctl->pool_tail[0], scratch64);
-        const uint64_t o1 = block_reserve<unsigned long long>(grow && tier == 1 ? newcap : 0ull,
+        const uint64_t o1 = block_reserve<unsigned long long>(grow && tier == 1 && !reuse ? newcap : 0ull,
                                                               (unsigned long long *)&a.ctl->pool_tail[1], scratch64);
-        const uint64_t o = tier ? o1 : o0;
-        const bool over = grow && (o + newcap > a.ctl->pool_cap[tier] || newcap > 0xFFFFFFFFull);
+        const uint64_t o = tier ? (reuse ? popped : o1) : o0;
+        const bool over = grow && !reuse && (o + newcap > a.ctl->pool_cap[tier] || newcap > 0xFFFFFFFFull);
         block_append(over, u, a.overflow, &a.ctl->overflow_cnt, scratch);
         const bool go = on && !over && na != 0;
         const uint64_t noff = grow ? (o | (tier ? kHostTier : 0ull)) : r.off;
@@ -794,9 +833,12 @@ __global__ void k_add_plan(MutArgs a, const uint32_t *list, const uint32_t *list
             const uint32_t *old_row = a.pools.row(r.off);
             uint32_t *row = a.pools.row(noff);
             uint32_t k = pb;
+            // the first piece of a relocated host-tier row carries the old
+            // block's class: k_fl_release lists the block once it is copied
             if (grow)
                 for (uint32_t p = 0; p < r.deg; p += kPieceLen)
-                    a.pieces[k++] = AddPiece{old_row + p, row + p, min(kPieceLen, r.deg - p), 0u};
+                    a.pieces[k++] = AddPiece{old_row + p, row + p, min(kPieceLen, r.deg - p),
+                                             p == 0 && tier == 1 ? kFlRelease | fl_class(r.cap) : 0u};
             const uint32_t *al = a.grouped + a.gofs[u] + nd;
             for (uint32_t p = 0; p < na; p += kPieceLen)
                 a.pieces[k++] = AddPiece{al + p, row + r.deg + p, min(kPieceLen, na - p), 0u};
@@ -857,6 +899,16 @@ __global__ __launch_bounds__(kBlock) void k_add_copy(MutArgs a) {
                 if (ok[e]) *dp[e] = val[e];
         }
         wave_lds_sync();
+    }
+}
+
+// Lists the old blocks of the host-tier rows this insert phase relocated
+// (their slots were copied by k_add_copy, which ran before).
+__global__ void k_fl_release(MutArgs a) {
+    const uint32_t cnt = ld_acquire(&a.ctl->copy_cnt);
+    for (uint32_t i = gthread(); i < cnt; i += gthreads()) {
+        const AddPiece pc = a.pieces[i];
+        if (pc.flags & kFlRelease) fl_push(a.fl, pc.flags & 63u, uint64_t(pc.src - a.pools.host));
     }
 }
 
@@ -2488,6 +2540,16 @@ __global__ void k_cmp_out(CmpArgs a, const CmpJob *jobs, const uint32_t *pjob, c
     }
 }
 
+// Lists the host-tier blocks that rows loaded into HBM vacated (k_move_rows,
+// which copied them, ran before).
+__global__ void k_move_release(const CopyJob *jobs, const uint32_t *njobs, FreeList fl) {
+    const uint32_t cnt = ld_acquire(njobs);
+    for (uint32_t j = gthread(); j < cnt; j += gthreads()) {
+        const CopyJob jb = jobs[j];
+        if ((jb.from & kHostTier) && jb.cap) fl_push(fl, fl_class(jb.cap), jb.from & ~kHostTier);
+    }
+}
+
 // Moves rows between tiers (cache admission / eviction). Each job names a row
 // whose descriptor has already been rewritten; `from` is the old location.
 // A wave per copy piece (see kCmpPiece), whole quads.
@@ -2570,7 +2632,7 @@ __global__ void k_adm_mark(AdmArgs a, uint32_t pass) {
 // tail, one reservation per workgroup step), the copy jobs, the rows' new
 // offsets; the vacated blocks become garbage of the other tier.
 __global__ void k_move_plan(Row *rows, const uint32_t *list, const uint32_t *count, uint32_t tier, Ctl *c,
-                            CopyJob *jobs, uint32_t *pcount, uint32_t *pjob) {
+                            CopyJob *jobs, uint32_t *pcount, uint32_t *pjob, FreeList fl) {
     __shared__ unsigned long long s64[kMaxWavesPerBlock + 1];
     __shared__ uint32_t s32[kMaxWavesPerBlock + 1];
     const uint32_t cnt = ld_acquire(count);
@@ -2578,13 +2640,19 @@ __global__ void k_move_plan(Row *rows, const uint32_t *list, const uint32_t *cou
         const uint32_t i = base + threadIdx.x;
         Row r{};
         uint32_t v = 0, np = 0;
+        uint64_t popped = 0;
+        bool reuse = false;
         if (i < cnt) {
             v = list[i];
             r = rows[v];
             np = max(1u, (r.deg + kCmpPiece - 1) / kCmpPiece);
+            // an evicted row takes a freed host-tier block of its class when one is listed
+            reuse = tier == 1 && fl_pop(fl, fl_class(r.cap), popped);
         }
-        const unsigned long long at = block_reserve<unsigned long long>(
-            (unsigned long long)r.cap, reinterpret_cast<unsigned long long *>(&c->pool_tail[tier]), s64);
+        const unsigned long long bumped = block_reserve<unsigned long long>(
+            reuse ? 0ull : (unsigned long long)r.cap, reinterpret_cast<unsigned long long *>(&c->pool_tail[tier]),
+            s64);
+        const unsigned long long at = reuse ? popped : bumped;
         block_reserve<unsigned long long>((unsigned long long)r.cap,
                                           reinterpret_cast<unsigned long long *>(&c->garbage[1 - tier]), s64);
         const uint32_t pf = block_reserve<uint32_t>(np, pcount, s32);
@@ -2594,6 +2662,8 @@ __global__ void k_move_plan(Row *rows, const uint32_t *list, const uint32_t *cou
             j.to = at | (tier ? kHostTier : 0ull);
             j.len = r.deg;
             j.pfirst = pf;
+            j.cap = r.cap;
+            j.pad = 0;
             jobs[i] = j;
             rows[v].off = j.to;
             for (uint32_t k = 0; k < np; ++k) pjob[pf + k] = i;

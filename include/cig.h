@@ -145,8 +145,9 @@ typedef struct cig_config {
                                      (each event record idles the stream a few us); total_ms is
                                      always timed; 0 or 1 = every call */
     uint32_t repair_delta;    /* bucket width of CIG_CFG_ORDERED_REPAIR; 0 = 128 (the reference's) */
-    uint32_t pool_slack;      /* edge slots each pool gets beyond 1.25x the loaded rows' capacity
-                                 (room for row growth before a compaction); 0 = 4 Mi (HBM),
+    uint32_t pool_slack;      /* edge slots each pool gets beyond its proportional headroom (HBM
+                                 without a budget: 1.25x the loaded rows' capacity; host tier:
+                                 1.5x), room for row growth before a compaction; 0 = 4 Mi (HBM),
                                  1 Mi (host tier) */
 } cig_config;
 
@@ -415,6 +416,15 @@ uint32_t cig_update_batches_left(const cig_engine *e);
  * activity window, admits the hottest rows that fit the budget, writes
  * evicted rows back to the host tier and compacts the HBM pool. */
 cig_status cig_cache_refresh(cig_engine *e, cig_batch_stats *stats);
+
+/* Packs one tier's edge pool now (tier 0: HBM, 1: pinned host), dropping the
+ * garbage that relocated and moved rows left, in place when the live rows fit
+ * (else into a larger pool), and empties the tier's free-block lists. No
+ * counterpart in the reference, which copies the whole budget at every
+ * refresh (driver.cuh:236-247); here packing otherwise happens only when a
+ * tier's tail overflows inside a batch, so a caller with idle time (between
+ * serving bursts) can take the cost there. Results do not change. */
+cig_status cig_compact(cig_engine *e, uint32_t tier);
 
 /* ---- results ------------------------------------------------------------ */
 /* GatherValue / GatherParent / GatherBuffer (framework.cuh GraphDatum

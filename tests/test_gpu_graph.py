@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import coincgraph_amd as cg
-from tests.helpers import check_topology, check_traversal, oracle_apply, oracle_graph
+from tests.helpers import check_incoming, check_topology, check_traversal, oracle_apply, oracle_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -71,12 +71,13 @@ def test_graph_batches_recapture_after_compaction():
 
 @pytest.mark.parametrize("tier", ["hbm", "host"])
 def test_in_place_compaction_keeps_results(tier):
-    """A pool with almost no slack overflows from relocation garbage alone
-    (as many deletes as inserts): the tier is packed in place on the device
-    (no second pool, no host copy of the rows) and every result stays exact."""
+    """Relocation garbage is packed away in place (HBM: device kernels; host
+    tier: host threads), by cig_compact between batches and by a tail that
+    overflows inside a batch (a pool with almost no slack, as many deletes
+    as inserts); every result stays exact."""
     print(f"[gpu-case] in-place compaction {tier}", flush=True)
     g = cg.rmat_graph(11, 16, seed=115)
-    s = cg.rmat_stream(g, 11, 12, 1500, 1500, seed=116)
+    s = cg.rmat_stream(g, 11, 16, 1500, 1500, seed=116)
     og = oracle_graph(g)
     kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=3, pool_slack=64)
     if tier == "host":
@@ -90,6 +91,46 @@ def test_in_place_compaction_keeps_results(tier):
             e.apply_batch(dels, adds)
             oracle_apply(og, dels, adds)
             check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 3, where=f"{tier} batch {b}")
+            if b in (5, 11):
+                before = e.info()
+                e.compact(tier)
+                after = e.info()
+                assert after["inplace_compactions"] == before["inplace_compactions"] + 1, (before, after)
+                used = "host_pool_used" if tier == "host" else "hbm_pool_used"
+                assert after[used] <= before[used], (before, after)
+                check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 3, where=f"{tier} compacted after {b}")
+                check_topology(e, og, f"{tier} compacted after {b}")
         info = e.info()
-        assert info["inplace_compactions"] > 0, info
+        if tier == "host":
+            assert info["host_compactions"] >= 2, info
         check_topology(e, og, f"{tier} final")
+
+
+def test_host_tier_free_blocks_reused():
+    """Relocated host-tier rows and rows loaded into HBM free their host
+    blocks; later relocations and evictions of the same size class take them
+    (free-block lists) instead of growing the tail. Results and topology stay
+    exact through batches, refreshes that move rows both ways, and packing."""
+    print("[gpu-case] host free blocks", flush=True)
+    g = cg.rmat_graph(12, 16, seed=117)
+    s = cg.rmat_stream(g, 12, 18, 800, 1200, seed=118)
+    og = oracle_graph(g)
+    with cg.Engine("sssp", weight_mode=cg.WEIGHT_HASH64, weight_seed=5, cache_bytes=g.m * 4 // 2,
+                   pool_slack=64) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        moved = 0
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            e.apply_batch(dels, adds)
+            oracle_apply(og, dels, adds)
+            check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 5, where=f"free-list batch {b}")
+            if b % 3 == 2:
+                moved += e.cache_refresh()["relocated_rows"]
+                check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 5, where=f"free-list refresh {b}")
+            if b == 10:
+                e.compact("host")
+        assert moved > 0, "no refresh moved a row"
+        check_topology(e, og, "free-list final")
+        check_incoming(e, og, "free-list final")
