@@ -478,10 +478,13 @@ def slowest_groups(res, k=6):
         if key in seen:
             continue
         seen.add(key)
+        st = r["stats"]
         out.append({"kind": kind, "start_s": round((r["start_ms"] - t0) / 1e3, 3),
                     "service_ms": round(r["done_ms"] - r["start_ms"], 1),
                     "waited_ms": round(r["start_ms"] - r["submit_ms"], 1), "group_size": r["group_size"],
-                    "device_ms": round(r["stats"]["total_ms"], 1)})
+                    "device_ms": round(st["total_ms"], 1), "rounds": st["rounds"],
+                    "records": st["deletes_applied"] + st["missing_deletes"] + st["adds_applied"],
+                    "relocated_rows": st["relocated_rows"], "compactions_so_far": st["pool_compactions"]})
     out.sort(key=lambda x: -x["service_ms"])
     return out[:k]
 

@@ -753,15 +753,25 @@ struct cig_engine {
     }
 
     // Per-record arrays of a call: the grouped destinations and the ranks.
+    // Per-call buffers grow to at least half again what they held, so calls of
+    // growing size (a serving loop's groups) reallocate O(log) times: every
+    // reallocation frees the old buffer, and the free waits for the device.
+    template <class T>
+    static bool grow(DBuf<T> &b, uint64_t need) {
+        if (b.count >= need) return false;
+        b.alloc(std::max<uint64_t>(need, b.count + b.count / 2));
+        return true;
+    }
+
     void ensure_grouped(uint64_t records) {
-        if (grouped.count < records) grouped.alloc(records);
-        if (rslot.count < records) rslot.alloc(records);
+        grow(grouped, records);
+        grow(rslot, records);
     }
 
     void ensure_records(uint64_t nd, uint64_t na) {
         const uint64_t need = std::max(nd, na);
-        if (rec_dels.count < need) rec_dels.alloc(need);
-        if (rec_adds.count < need) rec_adds.alloc(need);
+        grow(rec_dels, need);
+        grow(rec_adds, need);
         ensure_grouped(nd + na);
     }
 
@@ -2346,22 +2356,22 @@ struct cig_engine {
     // tiles are at most (live slots)/kDelTile + (rows with deletes).
     void ensure_delete_space(uint64_t nd) {
         const uint64_t tab = 20 * nd + 64;
-        if (hkeys.count < tab) {
-            hkeys.alloc(tab);
-            hcnts.alloc(tab);
-            fill32(hkeys.p, tab, 0);
-            fill32(reinterpret_cast<uint32_t *>(hcnts.p), 2 * tab, 0);
-        }
-        const uint64_t tiles = (hbm_slots + host_slots) / kDelTile + nd + 64;
-        if (dtiles.count < tiles) dtiles.alloc(tiles);
         if (tab > 0xFFFFFFFFull) throw HipError(CIG_ELIMIT, "batch too large for the delete tables");
+        if (hkeys.count < tab) {
+            // table offsets are 32-bit: the growth stays below 2^32 slots
+            const uint64_t n2 = std::min<uint64_t>(std::max(tab, hkeys.count + hkeys.count / 2), 0xFFFFFFFFull);
+            hkeys.alloc(n2);
+            hcnts.alloc(n2);
+            fill32(hkeys.p, n2, 0);
+            fill32(reinterpret_cast<uint32_t *>(hcnts.p), 2 * n2, 0);
+        }
+        grow(dtiles, (hbm_slots + host_slots) / kDelTile + nd + 64);
     }
 
     // Insert pieces: a relocated row gives ceil(deg/kPieceLen) copy pieces and
     // every touched source ceil(adds/kPieceLen) append pieces.
     void ensure_pieces(uint64_t nd, uint64_t na) {
-        const uint64_t need = (hbm_slots + host_slots) / kPieceLen + 2 * (nd + na) + na / kPieceLen + 64;
-        if (pieces.count < need) pieces.alloc(need);
+        grow(pieces, (hbm_slots + host_slots) / kPieceLen + 2 * (nd + na) + na / kPieceLen + 64);
     }
 
     void apply_device(const cig_edge *d, uint32_t nd, const cig_edge *a, uint32_t na, uint32_t flags,
@@ -3147,12 +3157,13 @@ struct cig_server {
         return fr;
     }
 
-    // Bytes a group of `nrec` records adds to the staging buffers.
+    // Bytes a group of nd deletes and na adds adds to the staging buffers
+    // (a buffer that grows takes at least half again its size: cig_engine::grow).
     uint64_t staging_need(uint64_t nd, uint64_t na) const {
-        uint64_t need = 0;
-        if (nd > sd.count) need += nd * sizeof(cig_edge);
-        if (na > sa.count) need += na * sizeof(cig_edge);
-        return need;
+        auto bytes = [](uint64_t want, uint64_t have) -> uint64_t {
+            return want > have ? std::max(want, have + have / 2) * sizeof(cig_edge) : 0;
+        };
+        return bytes(nd, sd.count) + bytes(na, sa.count);
     }
 
     void finish(std::vector<std::shared_ptr<ServeRequest>> &batch, cig_status st, const cig_batch_stats &stats,
@@ -3184,8 +3195,8 @@ struct cig_server {
         cig_status st = CIG_OK;
         try {
             if (nd > 0x7FFFFFFFull || na > 0x7FFFFFFFull) throw HipError(CIG_ELIMIT, "group too large");
-            if (sd.count < nd) sd.alloc(nd);
-            if (sa.count < na) sa.alloc(na);
+            cig_engine::grow(sd, nd);
+            cig_engine::grow(sa, na);
             uint64_t od = 0, oa = 0;
             for (auto &r : batch) {
                 if (!r->dels.empty())
