@@ -240,6 +240,13 @@ def main(argv=None):
                     help="CC: union-find repair (CIG_CFG_CC_UNION_REPAIR, the reference's union repair)")
     ap.add_argument("--graph", action="store_true",
                     help="replay each batch's fixed-length parts from captured HIP graphs (CIG_CFG_GRAPH_BATCHES)")
+    ap.add_argument("--adapt-split", type=int, default=0,
+                    help="frontier rounds keep rows of up to this many slots whole when waves take few "
+                         "items (cig_config.adapt_split; 0 = the library default, 1 = off)")
+    ap.add_argument("--round-blocks", type=int, default=0,
+                    help="workgroups per CU of the round kernels (cig_config.round_blocks_per_cu; 0 = default)")
+    ap.add_argument("--split", type=int, default=0, help="tile split degree (cig_config.split_degree; 0 = default)")
+    ap.add_argument("--tile-len", type=int, default=0, help="slots per tile (cig_config.tile_len; 0 = default)")
     ap.add_argument("--pr-push", action="store_true",
                     help="PageRank: scatter every round with atomics (keeps no incoming rows)")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving: arrival window (s)")
@@ -296,6 +303,14 @@ def main(argv=None):
         kw["flags"] |= cg.CFG_CC_UNION_REPAIR
     if args.graph:
         kw["flags"] |= cg.CFG_GRAPH_BATCHES
+    if args.adapt_split:
+        kw["adapt_split"] = args.adapt_split
+    if args.round_blocks:
+        kw["round_blocks_per_cu"] = args.round_blocks
+    if args.split:
+        kw["split_degree"] = args.split
+    if args.tile_len:
+        kw["tile_len"] = args.tile_len
     if pr:
         kw.update(pr_epsilon=1e-7, pr_max_rounds=1000)
         if args.pr_push:
@@ -442,6 +457,8 @@ def main(argv=None):
         "repair_kernel_ms_per_batch": round(sum(s["scan_kernel_ms"] for s in staged) / len(staged), 4),
         "affected_per_batch": round(float(np.mean([s["affected"] for s in stats])), 1),
         "rounds_per_batch": round(float(np.mean([s["rounds"] for s in stats])), 1),
+        **({"invalidation_rounds_per_batch": round(float(np.mean([s["inval_rounds"] for s in stats])), 1)}
+           if not pr else {}),
         **({"pull_rounds_per_batch": round(float(np.mean([s["pull_rounds"] for s in stats])), 1)} if pr else {}),
         "batched_form": grouped,
         "multi_source_form": multi,

@@ -76,6 +76,7 @@ class Config(C.Structure):
         ("tile_len", C.c_uint32), ("round_blocks_per_cu", C.c_uint32),
         ("pr_pull_divisor", C.c_uint32), ("pull_blocks_per_cu", C.c_uint32),
         ("kernel_timing_every", C.c_uint32), ("repair_delta", C.c_uint32), ("pool_slack", C.c_uint32),
+        ("adapt_split", C.c_uint32),
     ]
 
 
@@ -93,7 +94,7 @@ class BatchStats(C.Structure):
         ("pr_residual_l1", C.c_double), ("stage_edges", C.c_uint64 * 4),
         ("stage_vertices", C.c_uint64 * 4), ("stage_host_edges", C.c_uint64 * 4),
         ("scan_kernel_ms", C.c_double), ("round_kernel_ms", C.c_double),
-        ("round_launches", C.c_uint64), ("pull_rounds", C.c_uint64),
+        ("round_launches", C.c_uint64), ("pull_rounds", C.c_uint64), ("inval_rounds", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -449,7 +450,7 @@ class Engine:
                  device=0, pr_epsilon=1e-6, pr_max_rounds=100, flags=0, edges_per_lane=0,
                  split_degree=0, max_rounds=0, n_stream=3, segments=1, tile_len=0,
                  round_blocks_per_cu=0, pr_pull_divisor=0, pull_blocks_per_cu=0, kernel_timing_every=0,
-                 repair_delta=0, pool_slack=0):
+                 repair_delta=0, pool_slack=0, adapt_split=0):
         L = lib()
         cfg = Config()
         _check(L.cig_config_default(C.byref(cfg)))
@@ -473,6 +474,7 @@ class Engine:
         cfg.kernel_timing_every = kernel_timing_every
         cfg.repair_delta = repair_delta
         cfg.pool_slack = pool_slack
+        cfg.adapt_split = adapt_split
         self.config = cfg
         self._e = C.c_void_p()
         st = L.cig_create(C.byref(cfg), C.byref(self._e))

@@ -98,9 +98,10 @@ __device__ __forceinline__ uint32_t weight(uint32_t u, uint32_t v, uint64_t seed
 // (zero weights, CC): the pair is replaced only on a strict value decrease,
 // so a parent always got its value earlier than its child (acyclic tree).
 template <bool kStrict>
-__device__ __forceinline__ bool relax(uint64_t *state, uint32_t dst, uint32_t cand,
-                                      uint32_t src) {
-    uint64_t cur = state[dst];  // may be stale-high inside a fixed point: safe
+__device__ __forceinline__ bool relax_from(uint64_t *state, uint32_t dst, uint32_t cand,
+                                           uint32_t src, uint64_t cur) {
+    // cur: a value of state[dst] read earlier (may be stale-high inside a
+    // fixed point: safe, the atomic decides)
     if (uint32_t(cur >> 32) <= cand) return false;
     const uint64_t nv = (uint64_t(cand) << 32) | src;
     if constexpr (!kStrict) {
@@ -117,9 +118,21 @@ __device__ __forceinline__ bool relax(uint64_t *state, uint32_t dst, uint32_t ca
     }
 }
 
+template <bool kStrict>
+__device__ __forceinline__ bool relax(uint64_t *state, uint32_t dst, uint32_t cand,
+                                      uint32_t src) {
+    return relax_from<kStrict>(state, dst, cand, src, state[dst]);
+}
+
 // Frontier dedup: true for the first claimant of `tag` on v.
 __device__ __forceinline__ bool claim(uint32_t *stamp, uint32_t v, uint32_t tag) {
     if (stamp[v] == tag) return false;
+    return atomicExch(&stamp[v], tag) != tag;
+}
+// claim() without the test load, for a thread that has just improved v: the
+// first claimant of a round usually finds the tag absent, so the exchange
+// alone saves a dependent memory round trip.
+__device__ __forceinline__ bool claim_now(uint32_t *stamp, uint32_t v, uint32_t tag) {
     return atomicExch(&stamp[v], tag) != tag;
 }
 
@@ -129,6 +142,11 @@ __device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v) {
 __device__ __forceinline__ bool bit_claim(uint32_t *bits, uint32_t v) {
     const uint32_t m = 1u << (v & 31);
     if (bits[v >> 5] & m) return false;
+    return !(atomicOr(&bits[v >> 5], m) & m);
+}
+// bit_claim() without the test load, where the bit is usually still clear.
+__device__ __forceinline__ bool bit_claim_now(uint32_t *bits, uint32_t v) {
+    const uint32_t m = 1u << (v & 31);
     return !(atomicOr(&bits[v >> 5], m) & m);
 }
 
@@ -153,10 +171,12 @@ struct RelaxOp {
     // expand; the rest were deferred to the next round (delta 0: unordered).
     uint32_t delta = 0, bucket = kInf;
     __device__ bool source(uint32_t v, uint32_t, Row &row, uint32_t &pay) const {
+        // both loads issued before either is used (one memory round trip)
+        const Row r = rows[v];
         const uint32_t val = uint32_t(state[v] >> 32);
         if (val == kInf) return false;
         if (delta && val / delta > bucket) return false;
-        row = rows[v];
+        row = r;
         pay = val;
         if (hot) hot[v] += 1;
         return row.deg != 0;
@@ -169,8 +189,8 @@ struct RelaxOp {
     __device__ bool edge(uint32_t u, uint32_t val, uint32_t d, uint64_t cur) const {
         const uint64_t c = uint64_t(val) + weight<W>(u, d, seed);
         if (c >= uint32_t(cur >> 32)) return false;  // also rejects c >= kInf
-        if (!relax<kStrict>(state, d, uint32_t(c), u)) return false;
-        return claim(stamp, d, tag);
+        if (!relax_from<kStrict>(state, d, uint32_t(c), u, cur)) return false;
+        return claim_now(stamp, d, tag);
     }
 };
 
@@ -193,7 +213,9 @@ struct InvalidateOp {
     __device__ uint64_t peek(uint32_t y) const { return state[y]; }
     __device__ bool edge(uint32_t x, uint32_t, uint32_t y, uint64_t cur) const {
         if (uint32_t(cur) != x) return false;
-        return bit_claim(aff, y);
+        // y's parent is x: y is a tree child reached once (twice only over
+        // parallel edges), so its bit is almost always clear
+        return bit_claim_now(aff, y);
     }
 };
 
@@ -276,6 +298,11 @@ struct ExpandCfg {
     uint32_t tile_len;  // slots per tile (a multiple of 4)
     uint32_t stage;     // Stage whose counters this pass feeds
     StatShard *stats;
+    // non-zero: a wave that takes fewer than 64 items raises the split in
+    // proportion, up to this many slots (expand_range), so rows that its walk
+    // can hold expand in this round instead of as tiles in the next one
+    uint32_t adapt = 0;
+    uint32_t pad = 0;
 };
 
 struct Sink {
@@ -675,8 +702,14 @@ __device__ void expand_range(const Op &op, const ExpandCfg &cfg, const uint32_t 
     const uint32_t nwaves = gridDim.x * wpb;
     uint32_t per = kWave;
     while (per > 1 && uint64_t(per / 2) * nwaves >= count) per >>= 1;
+    // A wave walks at most per * split slots of short rows; with fewer items
+    // per wave the same bound admits longer rows. A row cut into tiles waits
+    // for the next round, so in a chain of frontier rounds every row kept
+    // whole here saves its children a round.
+    ExpandCfg c = cfg;
+    if (cfg.adapt) c.split = max(cfg.split, min(cfg.adapt, cfg.split * (kWave / per)));
     for (uint64_t base = uint64_t(gwave) * per; base < count; base += uint64_t(nwaves) * per)
-        expand_chunk<QPL>(op, cfg, items, uint32_t(base), uint32_t(min(uint64_t(count), base + per)), L,
+        expand_chunk<QPL>(op, c, items, uint32_t(base), uint32_t(min(uint64_t(count), base + per)), L,
                           out, ctr);
 }
 

@@ -197,6 +197,7 @@ struct CopyLane {
 
 constexpr uint32_t kTileLenDefault = 512;
 constexpr uint32_t kSplitDefault = 64;
+constexpr uint32_t kAdaptSplitDefault = 1024;
 constexpr uint32_t kQplDefault = 2;  // quads (4 slots) per lane and step
 // PR rounds whose frontier holds at least n / kPullDivisorDefault vertices
 // run in pull form; below n / kPullMaskDivisor the pull tests the
@@ -217,6 +218,11 @@ struct cig_engine {
     int cus = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
+    // A second compute stream: a batch's invalidation waves run on it while
+    // the engine stream groups the batch's records (see traversal_batch).
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool overlapped = false;  // this call's invalidation ran beside the grouping
     bool scan_timed = false;  // ev[8]/ev[9] bracket this call's repair scan kernel
     // CIG_CFG_KERNEL_TIMING: an event pair per propagation round launch
     static constexpr uint32_t kRoundEvents = 512;
@@ -303,6 +309,9 @@ struct cig_engine {
 
     // launch shape
     uint32_t split = kSplitDefault, tile_len = kTileLenDefault, qpl = kQplDefault;
+    // frontier rounds keep rows of up to this many slots whole when waves
+    // take few items (ExpandCfg::adapt); 0 = always cut at `split`
+    uint32_t adapt_split = kAdaptSplitDefault;
     uint32_t rblock = kBlock;  // relax round workgroup size (256 or 512)
     uint32_t grid_small = 0;   // one-shot kernels
     uint32_t grid_coop = 0;    // persistent cooperative kernels
@@ -312,6 +321,12 @@ struct cig_engine {
 
     // ------------------------------------------------------------ plumbing
     Pools pools() const { return Pools{hbm_pool.p, host_pool_dev}; }
+    // Workgroups of 256 threads for `items` work items, `per` per thread, at
+    // most grid_small and at least one.
+    uint32_t grid_for(uint64_t items, uint32_t per) const {
+        const uint64_t g = (items + 256ull * per - 1) / (256ull * per);
+        return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(grid_small, g)));
+    }
     ExpandCfg ecfg(uint32_t stage) const { return ExpandCfg{pools(), split, tile_len, stage, stats.p}; }
     uint32_t weight_kind() const {
         switch (cfg.algo) {
@@ -328,6 +343,9 @@ struct cig_engine {
         HIPCHK(hipGetDeviceProperties(&prop, cfg.device));
         cus = prop.multiProcessorCount;
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
         scratch_dev = scratch_address(cfg.device);
         lane.init(stream);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
@@ -350,6 +368,7 @@ struct cig_engine {
         HIPCHK(hipMemcpyAsync(ctl.p, hctl.p, sizeof(Ctl), hipMemcpyHostToDevice, stream));
         if (cfg.split_degree) split = std::max<uint32_t>(64, cfg.split_degree);
         if (cfg.tile_len) tile_len = (std::max<uint32_t>(64, cfg.tile_len) + 3u) & ~3u;  // whole quads
+        if (cfg.adapt_split) adapt_split = cfg.adapt_split <= split ? 0u : cfg.adapt_split;
         // edges per lane come in whole quads: 4, 8 or 16
         if (cfg.edges_per_lane) qpl = cfg.edges_per_lane >= 16 ? 4 : cfg.edges_per_lane >= 8 ? 2 : 1;
         coop = (cfg.flags & CIG_CFG_COOPERATIVE) != 0;
@@ -799,6 +818,7 @@ struct cig_engine {
         a.t0 = t0.p;
         a.t1 = t1.p;
         a.cfg = ecfg(kStagePropagate);
+        a.cfg.adapt = adapt_split;
         a.seed = cfg.weight_seed;
         a.max_rounds = max_rounds;
         a.delta = ordered() ? (cfg.repair_delta ? cfg.repair_delta : 128u) : 0u;
@@ -820,7 +840,8 @@ struct cig_engine {
     // the stream and at most kLookahead no-op rounds follow the fixed point.
     // Returns the number of rounds that had work.
     template <class Launch>
-    uint64_t run_chain(Launch launch, uint64_t bound, const char *what) {
+    uint64_t run_chain(Launch launch, uint64_t bound, const char *what, hipStream_t on = nullptr) {
+        if (!on) on = stream;
         constexpr uint64_t kLookahead = 3;
         const uint64_t epoch = ++chain_epoch & 0xFFFFFFFFull;
         uint64_t issued = 0, seen = 0;
@@ -835,7 +856,7 @@ struct cig_engine {
                 w = __atomic_load_n(prog.p, __ATOMIC_ACQUIRE);
                 if ((w >> 32) == epoch && ((w >> 1) & 0x7FFFFFFFull) > seen) break;
                 if ((spin & 1023) == 1023) {
-                    const hipError_t q = hipStreamQuery(stream);
+                    const hipError_t q = hipStreamQuery(on);
                     if (q != hipSuccess && q != hipErrorNotReady)
                         throw HipError(CIG_EDEVICE, std::string(what) + ": " + hipGetErrorString(q));
                     if (q == hipSuccess) {  // stream idle: the word must be final now
@@ -856,6 +877,7 @@ struct cig_engine {
     }
 
     uint64_t round_launches = 0;  // timed round launches of the current call
+    uint64_t inval_waves = 0;     // invalidation waves with work in the current call
     uint64_t calls = 0;           // calls begun (selects the timed ones)
     bool timing_now = false;      // this call's round kernels are timed
     bool stages_now = true;       // this call records its stage markers
@@ -1089,8 +1111,10 @@ struct cig_engine {
         if (st) std::memset(st, 0, sizeof(*st));
         flush_finish();
         scan_timed = false;
+        overlapped = false;
         round_timed = 0;
         round_launches = 0;
+        inval_waves = 0;
         const uint32_t every = std::max<uint32_t>(1, cfg.kernel_timing_every);
         stages_now = calls % every == 0;
         timing_now = !rev_begin.empty() && stages_now;
@@ -1125,8 +1149,9 @@ struct cig_engine {
         st->total_ms = elapsed(0, 7);
         if (marks >= 4 && stages_now) {  // ev: 0 start, 1 mutated, 2 invalidated, 3 repaired, 5 seeded, 4 propagated
             st->mutate_ms = elapsed(0, 1);
-            st->invalidate_ms = elapsed(1, 2);
-            st->repair_ms = elapsed(2, 3);
+            // overlapped: the invalidation ran beside the mutation, from the start
+            st->invalidate_ms = overlapped ? elapsed(0, 2) : elapsed(1, 2);
+            st->repair_ms = overlapped ? elapsed(1, 3) : elapsed(2, 3);
             st->seed_ms = elapsed(3, 5);
             st->propagate_ms = elapsed(5, 4);
         }
@@ -1137,6 +1162,7 @@ struct cig_engine {
             st->round_kernel_ms += ms;
         }
         st->round_launches = round_launches;
+        st->inval_rounds = inval_waves;
         for (int i = 0; i < 4; ++i) {
             st->stage_edges[i] = c.st_edges[i];
             st->stage_vertices[i] = c.st_vexp[i];
@@ -1195,10 +1221,16 @@ struct cig_engine {
         return m;
     }
 
-    void mutate_group(const MutArgs &m) {
-        k_count<<<grid_small, 256, 0, stream>>>(m);
-        k_plan<<<grid_small, 256, 0, stream>>>(m);
-        k_scatter<<<grid_small, 256, 0, stream>>>(m);
+    void mutate_group(const MutArgs &m, hipStream_t on = nullptr) {
+        if (!on) on = stream;
+        // Grids sized by the call's records (all are grid-stride loops): a
+        // grid of idle workgroups would hold the CUs that the invalidation
+        // waves, running beside the grouping, need. Graph-captured calls read
+        // their record counts on the device and keep the full grid.
+        const uint64_t recs = (uint64_t(m.nd) + m.na) << m.rev;
+        k_count<<<m.ref ? grid_small : grid_for(recs, 4), 256, 0, on>>>(m);
+        k_plan<<<m.ref ? grid_small : grid_for(recs, 1), 256, 0, on>>>(m);
+        k_scatter<<<m.ref ? grid_small : grid_for(recs, 1), 256, 0, on>>>(m);
         HIPCHK(hipGetLastError());
     }
 
@@ -1652,6 +1684,7 @@ struct cig_engine {
         x.t0 = t0.p;
         x.t1 = t1.p;
         x.cfg = ecfg(kStageInvalidate);
+        x.cfg.adapt = adapt_split;
         x.dels = d;
         x.nd = nd;
         x.adds = a;
@@ -1720,8 +1753,9 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
     }
 
-    void invalidate_and_reset(const InvalArgs &ia) {
-        k_inval_seed<<<grid_small, 256, 0, stream>>>(ia);
+    void invalidate_and_reset(const InvalArgs &ia, hipStream_t on = nullptr) {
+        if (!on) on = stream;
+        k_inval_seed<<<ia.ref ? grid_small : grid_for(ia.nd, 1), 256, 0, on>>>(ia);
         HIPCHK(hipGetLastError());
         InvalArgs a = ia;
         if (coop) {
@@ -1730,13 +1764,13 @@ struct cig_engine {
             if (!grid) grid = coop_grid_for(kernel);
             launch_coop(kernel, a, grid);
         } else {
-            run_chain(
+            inval_waves += run_chain(
                 [&](uint64_t tag, uint32_t seq) {
-                    k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(a, prog_dev, tag, seq);
+                    k_inval_round<1><<<grid_round, kBlock, 0, on>>>(a, prog_dev, tag, seq);
                 },
-                round_bound(), "invalidation");
+                round_bound(), "invalidation", on);
         }
-        k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
+        k_reset_affected<<<grid_small, 256, 0, on>>>(ia);
         HIPCHK(hipGetLastError());
     }
 
@@ -1837,7 +1871,7 @@ struct cig_engine {
         });
         mark(1);
         if (nd) {
-            run_chain([&](uint64_t tag, uint32_t seq) {
+            inval_waves += run_chain([&](uint64_t tag, uint32_t seq) {
                           k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(ia, prog_dev, tag, seq);
                       },
                       round_bound(), "invalidation");
@@ -1879,12 +1913,35 @@ struct cig_engine {
             return;
         }
         MutArgs m = mut_args(d_dels, nd, d_adds, na);
-        mutate_group(m);
-        mutate_apply(m);
-        mark(1);
         InvalArgs ia = inval_args(d_dels, nd, d_adds, na);
-        if (nd) invalidate_and_reset(ia);
-        mark(2);
+        if (nd && !coop) {
+            // The invalidation waves read the rows as they were before the
+            // batch: a child y of x (parent(y) = x) is found over any edge
+            // x->y, and one whose edges x->y are all deleted is a seed
+            // anyway, so the affected set is the one the mutated rows give.
+            // The records are grouped (k_count / k_plan / k_scatter, which
+            // only read the rows) on the side stream while the waves run on
+            // the engine stream; the deletes, which rewrite rows, follow
+            // both. The grouping is enqueued first, so that its join event
+            // exists before the host drives the waves' chain and the deletes
+            // can follow the chain at once.
+            overlapped = true;
+            HIPCHK(hipEventRecord(fork_ev, stream));
+            HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+            mutate_group(m, side);
+            HIPCHK(hipEventRecord(join_ev, side));
+            invalidate_and_reset(ia);
+            mark(2);
+            HIPCHK(hipStreamWaitEvent(stream, join_ev, 0));
+            mutate_apply(m);
+            mark(1);
+        } else {
+            mutate_group(m);
+            mutate_apply(m);
+            mark(1);
+            if (nd) invalidate_and_reset(ia);
+            mark(2);
+        }
         if (nd) repair<W, S>(ia);
         mark(3);
         if (na) insert_seed<W, S>(ia, nd != 0);
@@ -3054,6 +3111,12 @@ struct cig_engine {
     }
 
     ~cig_engine() {
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            if (fork_ev) (void)hipEventDestroy(fork_ev);
+            if (join_ev) (void)hipEventDestroy(join_ev);
+            (void)hipStreamDestroy(side);
+        }
         if (stream) {
             (void)hipStreamSynchronize(stream);
             for (auto &e : ev)

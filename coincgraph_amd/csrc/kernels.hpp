@@ -495,25 +495,36 @@ __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
         uint32_t key[kDelWaveMax];
 #pragma unroll
         for (uint32_t j = 0; j < kDelWaveMax; ++j) key[j] = __shfl(mine, int(g * G + j), kWave);
+        // the most requests any of the wave's rows has: the match below
+        // compares against that many keys only (most medium rows have one or
+        // two deletes, so most waves skip most of the eight compares a slot)
+        uint32_t wmax = nd;
+        wmax = max(wmax, uint32_t(__shfl_xor(int(wmax), 16, kWave)));
+        wmax = max(wmax, uint32_t(__shfl_xor(int(wmax), 32, kWave)));
         uint32_t open = (1u << nd) - 1u;  // uniform within the group
         uint32_t holes = 0;               // uniform within the group
         for (uint32_t q0 = 0; __ballot(open != 0u && q0 < nq) != 0; q0 += G * Q) {
             uint32_t d[Q][4], cand = 0;  // bit 4k+s: slot (k, s) equals an open request
             if (open != 0u && q0 < nq) {
+                uint32_t livemask = 0;
 #pragma unroll
                 for (uint32_t k = 0; k < Q; ++k) {
                     const uint32_t q = q0 + k * G + gl;
                     uint32_t live = load_quad(p, q < nq ? q : nq - 1, deg, d[k]);
                     if (q >= nq) live = 0;
-#pragma unroll
-                    for (uint32_t s = 0; s < 4; ++s) {
-                        // claimed and unused requests hold kNone, which no live slot holds
-                        bool hit = false;
-#pragma unroll
-                        for (uint32_t j = 0; j < kDelWaveMax; ++j) hit |= key[j] == d[k][s];
-                        if (s < live && hit) cand |= 1u << (4 * k + s);
-                    }
+                    livemask |= ((1u << live) - 1u) << (4 * k);
                 }
+                // claimed and unused requests hold kNone, which no live slot holds
+#pragma unroll
+                for (uint32_t j = 0; j < kDelWaveMax; ++j) {
+                    if (j >= wmax) break;  // wave-uniform
+#pragma unroll
+                    for (uint32_t k = 0; k < Q; ++k)
+#pragma unroll
+                        for (uint32_t s = 0; s < 4; ++s)
+                            if (key[j] == d[k][s]) cand |= 1u << (4 * k + s);
+                }
+                cand &= livemask;
             }
             // each group claims its candidates one at a time, in lane order
             for (;;) {

@@ -149,6 +149,10 @@ typedef struct cig_config {
                                  without a budget: 1.25x the loaded rows' capacity; host tier:
                                  1.5x), room for row growth before a compaction; 0 = 4 Mi (HBM),
                                  1 Mi (host tier) */
+    uint32_t adapt_split;     /* frontier rounds (relax, invalidation) whose waves take fewer than 64
+                                 items each raise the split in proportion, keeping rows of up to this
+                                 many slots whole in the round instead of cutting them into tiles
+                                 expanded a round later; 0 = default (1024); <= split_degree: off */
 } cig_config;
 
 /* Per-call statistics. Times are device (HIP event) milliseconds of the
@@ -189,6 +193,7 @@ typedef struct cig_batch_stats {
     uint64_t round_launches;  /* round kernels whose durations round_kernel_ms sums (incl. no-op
                                  lookahead); 0 in a call that kernel_timing_every skips */
     uint64_t pull_rounds;     /* PR rounds run in pull form */
+    uint64_t inval_rounds;    /* invalidation waves that found work (single-source engines) */
 } cig_batch_stats;
 
 typedef struct cig_graph_info {
