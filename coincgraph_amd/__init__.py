@@ -46,6 +46,7 @@ CFG_CC_UNION_REPAIR = 1 << 7
 CFG_GRAPH_BATCHES = 1 << 8
 CFG_PR_KEEP_IDS = 1 << 9
 BATCH_STAGED = 1
+BATCHES_EACH = 1 << 3  # CIG_BATCHES_EACH
 INF = 0xFFFFFFFF
 ABI_VERSION = 1
 NOALLOC_SCRATCH_BYTES = 256 << 20  # CIG_NOALLOC_SCRATCH_BYTES
@@ -571,11 +572,18 @@ class Engine:
                                               C.byref(st)))
         return st.as_dict()
 
-    def apply_next_batches(self, count: int) -> dict:
+    def apply_next_batches(self, count: int, each: bool = False, staged: bool = False):
         """The next `count` resident batches as one group (same results as
-        applying them one by one, one mutation and one fixed point)."""
+        applying them one by one, one mutation and one fixed point): one
+        stats dict. each=True: applied one at a time in one library call
+        (every intermediate state produced): a list of per-batch dicts."""
+        flags = BATCH_STAGED if staged else 0
+        if each:
+            arr = (BatchStats * count)()
+            self._call(lib().cig_apply_next_batches(self._e, count, flags | BATCHES_EACH, arr))
+            return [x.as_dict() for x in arr]
         st = BatchStats()
-        self._call(lib().cig_apply_next_batches(self._e, count, 0, C.byref(st)))
+        self._call(lib().cig_apply_next_batches(self._e, count, flags, C.byref(st)))
         return st.as_dict()
 
     def autotune(self, source: int = 0) -> dict:

@@ -206,7 +206,7 @@ constexpr uint32_t kQplDefault = 2;  // quads (4 slots) per lane and step
 // of n/4096 hub-heavy vertices.)
 constexpr uint32_t kPullDivisorDefault = 4096;
 constexpr uint32_t kPullMaskDivisor = 4;
-constexpr uint32_t kPullBlocksPerCu = 16;
+constexpr uint32_t kPullBlocksPerCu = 64;
 
 }  // namespace
 
@@ -374,7 +374,7 @@ struct cig_engine {
         coop = (cfg.flags & CIG_CFG_COOPERATIVE) != 0;
         if (coop && !prop.cooperativeLaunch) throw HipError(CIG_EDEVICE, "device lacks cooperative launch");
         grid_small = uint32_t(cus) * 8;
-        grid_pull = uint32_t(cus) * (cfg.pull_blocks_per_cu ? std::min<uint32_t>(cfg.pull_blocks_per_cu, 32) : kPullBlocksPerCu);
+        grid_pull = uint32_t(cus) * (cfg.pull_blocks_per_cu ? std::min<uint32_t>(cfg.pull_blocks_per_cu, 128) : kPullBlocksPerCu);
         grid_round = uint32_t(cus) * (cfg.round_blocks_per_cu ? std::min<uint32_t>(cfg.round_blocks_per_cu, 16) : 1);
     }
 
@@ -2607,6 +2607,17 @@ struct cig_engine {
     }
 
     // The next `k` resident batches as one group.
+    // The next k batches one at a time (each its own call's work); st, if
+    // not null, receives one entry per batch.
+    void apply_next_each(uint32_t k, uint32_t flags, cig_batch_stats *st) {
+        check_ready();
+        if (flags & (CIG_BATCH_DELETE_PHASE | CIG_BATCH_ADD_PHASE))
+            throw HipError(CIG_EINVAL, "whole batches only");
+        const uint32_t left = uint32_t(batch_sizes.size() / 2) - next_batch;
+        if (!k || k > left) throw HipError(CIG_ELIMIT, "not that many update batches left");
+        for (uint32_t b = 0; b < k; ++b) apply_next(flags, st ? st + b : nullptr);
+    }
+
     void apply_next_group(uint32_t k, uint32_t flags, cig_batch_stats *st) {
         check_ready();
         if (deletes_done) throw HipError(CIG_ESTATE, "finish the add phase of this batch first");
@@ -3569,7 +3580,10 @@ cig_status cig_query_noalloc(cig_engine *e, uint32_t algo, const uint32_t *sourc
 }
 
 cig_status cig_apply_next_batches(cig_engine *e, uint32_t count, uint32_t flags, cig_batch_stats *st) {
-    return guarded(e, [&] { e->apply_next_group(count, flags, st); });
+    return guarded(e, [&] {
+        if (flags & CIG_BATCHES_EACH) e->apply_next_each(count, flags & ~uint32_t(CIG_BATCHES_EACH), st);
+        else e->apply_next_group(count, flags, st);
+    });
 }
 
 cig_status cig_apply_next_batch(cig_engine *e, uint32_t flags, cig_batch_stats *st) {

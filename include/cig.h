@@ -139,7 +139,7 @@ typedef struct cig_config {
     uint32_t pr_pull_divisor; /* PR rounds whose frontier holds at least n / pr_pull_divisor
                                  vertices gather along the incoming rows instead of scattering
                                  (no atomics); 0 = default (4096); >= n: every round */
-    uint32_t pull_blocks_per_cu; /* workgroups per CU of the PR pull kernels; 0 = default (16) */
+    uint32_t pull_blocks_per_cu; /* workgroups per CU of the PR pull kernels (at most 128); 0 = default (64) */
     uint32_t kernel_timing_every; /* time one call in this many: its stage markers (the *_ms stage
                                      fields) and, with CIG_CFG_KERNEL_TIMING, its round kernels
                                      (each event record idles the stream a few us); total_ms is
@@ -295,8 +295,13 @@ enum {
     CIG_BATCH_DELETE_PHASE = 1u << 1, /* cig_apply_next_batch: apply only the
                                          current batch's deletes (del_edge); the
                                          batch stays current */
-    CIG_BATCH_ADD_PHASE = 1u << 2     /* cig_apply_next_batch: apply only the
+    CIG_BATCH_ADD_PHASE = 1u << 2,    /* cig_apply_next_batch: apply only the
                                          current batch's adds (add_edge) and advance */
+    CIG_BATCHES_EACH = 1u << 3        /* cig_apply_next_batches: apply the batches one
+                                         at a time, each with its own mutation and fixed
+                                         point (every intermediate state is produced);
+                                         stats, if not null, points to `count` entries,
+                                         one per batch */
 };
 /* One update batch: all deletes, then all adds (README.md:17-37). For PR this
  * is Engine::UpdatePageRank (framework.cuh:1660-1704). */
@@ -413,6 +418,8 @@ cig_status cig_query_noalloc(cig_engine *e, uint32_t algo, const uint32_t *sourc
  * multiset after the call equal applying the batches one by one; statistics
  * cover the whole group. Not allowed between the two phases of a batch. */
 cig_status cig_apply_next_batches(cig_engine *e, uint32_t count, uint32_t flags, cig_batch_stats *stats);
+/* (With CIG_BATCHES_EACH the call is the host loop over the next `count`
+ * batches, as the reference's samples run it: no grouping, no state skipped.) */
 uint32_t cig_update_batches_left(const cig_engine *e);
 
 /* ---- edge cache (hot rows in HBM, cold rows zero-copy from pinned host) --

@@ -99,3 +99,33 @@ def test_grouped_pagerank_matches_fp64():
         check_topology(e, og, "pr group")
         l1n, _ = pr_errors(e.values(), og.pagerank()[0])
         assert l1n <= 1e-6
+
+
+@pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_HASH64), ("cc", 0)])
+def test_each_form_matches_one_call_per_batch(algo, wmode):
+    """cig_apply_next_batches with CIG_BATCHES_EACH is the host loop over the
+    batches: per-batch statistics equal one call per batch, and the state
+    after the call is the oracle's."""
+    print(f"[gpu-case] each-form {algo}", flush=True)
+    g = cg.rmat_graph(12, 16, seed=45, symmetric=algo == "cc")
+    s = _conflict_stream(g, 6, 500, 46)
+    og = oracle_graph(g)
+    keys = ("deletes_applied", "adds_applied", "missing_deletes", "invalid_updates")
+    with cg.Engine(algo, weight_mode=wmode, weight_seed=5) as a, \
+            cg.Engine(algo, weight_mode=wmode, weight_seed=5) as b:
+        for e in (a, b):
+            e.load_graph(g)
+            e.init_graph(0)
+            e.start()
+            e.load_updates(s)
+        one = [b.apply_next_batch() for _ in range(6)]
+        each = a.apply_next_batches(6, each=True)
+        assert len(each) == 6
+        for j in range(6):
+            oracle_apply(og, *s.batch(j))
+            assert [each[j][k] for k in keys] == [one[j][k] for k in keys], j
+            assert each[j]["total_ms"] > 0
+        assert np.array_equal(a.values(), b.values())
+        check_traversal(a, og, algo, 0, wmode, 5, where="each form")
+        check_topology(a, og, "each form")
+        assert a.batches_left() == 0
