@@ -222,6 +222,14 @@ struct cig_engine {
     // the engine stream groups the batch's records (see traversal_batch).
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // After an overlapped call the side stream clears the call's per-row
+    // counts and resets the mutation counters (finish_split): side_done_ev
+    // marks the end of that work, which anything else on the engine stream
+    // waits for (flush_finish); side_ready says the next overlapped call may
+    // group its records without waiting for the engine stream.
+    hipEvent_t side_done_ev = nullptr;
+    bool side_pending = false, side_ready = false;
+    bool fresh_records = false;  // this call enqueued nothing before its batch (resident records)
     bool overlapped = false;  // this call's invalidation ran beside the grouping
     bool scan_timed = false;  // ev[8]/ev[9] bracket this call's repair scan kernel
     // CIG_CFG_KERNEL_TIMING: an event pair per propagation round launch
@@ -346,6 +354,7 @@ struct cig_engine {
         HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&side_done_ev, hipEventDisableTiming));
         scratch_dev = scratch_address(cfg.device);
         lane.init(stream);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
@@ -800,8 +809,8 @@ struct cig_engine {
         sync();
     }
 
-    void reset_ctl(bool keep_rounds) {
-        k_reset_ctl<<<1, 64, 0, stream>>>(ctl.p, keep_rounds ? 1u : 0u);
+    void reset_ctl(bool keep_rounds, bool keep_mut = false) {
+        k_reset_ctl<<<1, 64, 0, stream>>>(ctl.p, (keep_rounds ? kResetKeepRounds : 0u) | (keep_mut ? kResetKeepMut : 0u));
         HIPCHK(hipGetLastError());
     }
 
@@ -1107,9 +1116,13 @@ struct cig_engine {
     }
 
     // ------------------------------------------------------------ stats
-    void begin_call(cig_batch_stats *st) {
+    // quick: an overlapped call whose side stream is ready (side_ready): the
+    // start event comes first, nothing waits for the side stream and the
+    // mutation counters are left to it.
+    void begin_call(cig_batch_stats *st, bool quick = false) {
         if (st) std::memset(st, 0, sizeof(*st));
-        flush_finish();
+        if (quick) HIPCHK(hipEventRecord(ev[0], stream));
+        else flush_finish();
         scan_timed = false;
         overlapped = false;
         round_timed = 0;
@@ -1119,8 +1132,8 @@ struct cig_engine {
         stages_now = calls % every == 0;
         timing_now = !rev_begin.empty() && stages_now;
         ++calls;
-        reset_ctl(true);
-        HIPCHK(hipEventRecord(ev[0], stream));
+        reset_ctl(true, quick);
+        if (!quick) HIPCHK(hipEventRecord(ev[0], stream));
     }
 
     float elapsed(int a, int b) {
@@ -1263,6 +1276,23 @@ struct cig_engine {
     MutArgs pending_m{};
     bool pending_aff = false;
 
+    // End of an overlapped call: the per-row counts and the mutation counters
+    // are cleared on the side stream (which groups the next call's records),
+    // the affected bits on the engine stream (which runs the next waves).
+    // Called after the call's final synchronisation, so the engine stream is
+    // idle. The clear also adds the activity counts (hot), which the next
+    // call's rounds add to as well: with activity on, the next call is not
+    // quick and waits for it (flush_finish).
+    void finish_split(const MutArgs &m) {
+        k_clear_touched<<<grid_small, 256, 0, side>>>(m, nullptr, nullptr);
+        k_reset_mut<<<1, 64, 0, side>>>(ctl.p);
+        HIPCHK(hipEventRecord(side_done_ev, side));
+        k_clear_aff<<<grid_small, 256, 0, stream>>>(ctl.p, aff.p, alist.p);
+        HIPCHK(hipGetLastError());
+        side_pending = true;
+        side_ready = m.hot == nullptr;
+    }
+
     void defer_finish(const MutArgs &m, bool clear_affected) {
         pending_m = m;
         pending_aff = clear_affected;
@@ -1270,6 +1300,11 @@ struct cig_engine {
     }
 
     void flush_finish() {
+        if (side_pending) {
+            HIPCHK(hipStreamWaitEvent(stream, side_done_ev, 0));
+            side_pending = false;
+        }
+        side_ready = false;
         if (!finish_pending) return;
         finish_pending = false;
         if (msk) msi_clear();
@@ -1903,7 +1938,13 @@ struct cig_engine {
     void traversal_batch(const cig_edge *d_dels, uint32_t nd, const cig_edge *d_adds, uint32_t na,
                          bool staged, cig_batch_stats *st) {
         if (graph_mode(staged)) return traversal_batch_graph<W, S>(d_dels, nd, d_adds, na, st);
-        begin_call(st);
+        const bool overlap = nd && !coop && !(staged && na);
+        // quick: the side stream's counters are reset and this call enqueued
+        // nothing before its batch, so the grouping needs no fork
+        const bool quick = overlap && side_ready && fresh_records;
+        fresh_records = false;
+        side_ready = false;  // set again only once this call has finished (finish_split)
+        begin_call(st, quick);
         if (staged && nd && na) {
             // stage 1: deletes to their fixed point, then stage 2: adds
             delete_stage<W, S>(d_dels, nd);
@@ -1914,7 +1955,7 @@ struct cig_engine {
         }
         MutArgs m = mut_args(d_dels, nd, d_adds, na);
         InvalArgs ia = inval_args(d_dels, nd, d_adds, na);
-        if (nd && !coop) {
+        if (overlap) {
             // The invalidation waves read the rows as they were before the
             // batch: a child y of x (parent(y) = x) is found over any edge
             // x->y, and one whose edges x->y are all deleted is a seed
@@ -1926,8 +1967,10 @@ struct cig_engine {
             // exists before the host drives the waves' chain and the deletes
             // can follow the chain at once.
             overlapped = true;
-            HIPCHK(hipEventRecord(fork_ev, stream));
-            HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+            if (!quick) {
+                HIPCHK(hipEventRecord(fork_ev, stream));
+                HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+            }
             mutate_group(m, side);
             HIPCHK(hipEventRecord(join_ev, side));
             invalidate_and_reset(ia);
@@ -1958,7 +2001,8 @@ struct cig_engine {
             relax();
             end_call(st, 4);  // the control block still holds the whole call's totals
         }
-        defer_finish(m, nd != 0);
+        if (overlap) finish_split(m);
+        else defer_finish(m, nd != 0);
     }
 
     template <int W, bool S>
@@ -2484,6 +2528,13 @@ struct cig_engine {
             apply_device(nullptr, 0, s_adds.p + add_off, na, 0, st);
         } else {
             if (deletes_done) throw HipError(CIG_ESTATE, "finish the add phase of this batch first");
+            // resident records: nothing is enqueued ahead of the batch (the
+            // flag is consumed by traversal_batch and dropped on any exit)
+            struct Fresh {
+                bool &f;
+                ~Fresh() { f = false; }
+            } fresh{fresh_records};
+            fresh_records = true;
             apply_device(s_dels.p + del_off, nd, s_adds.p + add_off, na, flags, st);
         }
         deletes_done = false;
@@ -3126,6 +3177,7 @@ struct cig_engine {
             (void)hipStreamSynchronize(side);
             if (fork_ev) (void)hipEventDestroy(fork_ev);
             if (join_ev) (void)hipEventDestroy(join_ev);
+            if (side_done_ev) (void)hipEventDestroy(side_done_ev);
             (void)hipStreamDestroy(side);
         }
         if (stream) {

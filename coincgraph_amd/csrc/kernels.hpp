@@ -33,19 +33,25 @@ __global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t value) {
 }
 
 // Per-call reset of the control block counters (not the round state).
-__global__ void k_reset_ctl(Ctl *c, uint32_t keep_rounds) {
+// flags: kResetKeepRounds keeps the round state; kResetKeepMut keeps the
+// mutation counters (the second stream resets those itself, k_reset_mut).
+constexpr uint32_t kResetKeepRounds = 1u, kResetKeepMut = 2u;
+__global__ void k_reset_ctl(Ctl *c, uint32_t flags) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const bool keep_rounds = flags & kResetKeepRounds;
     c->acount = 0;
     c->icount[0] = c->icount[1] = c->icount[2] = 0;
     c->ifront[0] = c->ifront[1] = c->ifront[2] = 0;
     c->iround = 0;
     c->scount = 0;
-    c->touched = 0;
-    c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
-    c->mid_cnt = c->mid_host_cnt = 0;
-    c->dtile_cnt = c->ht_cursor = 0;
+    if (!(flags & kResetKeepMut)) {
+        c->touched = 0;
+        c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
+        c->mid_cnt = c->mid_host_cnt = 0;
+        c->dtile_cnt = c->ht_cursor = 0;
+        c->group_cursor = 0;
+    }
     c->pr_invalid = 0;
-    c->group_cursor = 0;
     c->pull_rounds = 0;
     c->edges_scanned = c->host_edges = c->vexp = 0;
     c->deletes_applied = c->missing = c->invalid = c->adds_applied = c->relocated = 0;
@@ -1064,6 +1070,12 @@ __global__ void k_clear_touched(MutArgs a, uint32_t *aff, const uint32_t *alist)
         const uint32_t cnt = ld_acquire(&a.ctl->acount);
         for (uint32_t i = gthread(); i < cnt; i += gthreads()) aff[alist[i] >> 5] = 0u;
     }
+}
+
+// The affected-bit part of k_clear_touched alone.
+__global__ void k_clear_aff(const Ctl *c, uint32_t *aff, const uint32_t *alist) {
+    const uint32_t cnt = ld_acquire(&c->acount);
+    for (uint32_t i = gthread(); i < cnt; i += gthreads()) aff[alist[i] >> 5] = 0u;
 }
 
 // ---------------------------------------------------- deletion invalidation
