@@ -785,10 +785,22 @@ struct cig_engine {
     // growing size (a serving loop's groups) reallocate O(log) times: every
     // reallocation frees the old buffer, and the free waits for the device.
     template <class T>
-    static bool grow(DBuf<T> &b, uint64_t need) {
+    bool grow(DBuf<T> &b, uint64_t need) {
         if (b.count >= need) return false;
+        retire(b);
         b.alloc(std::max<uint64_t>(need, b.count + b.count / 2));
         return true;
+    }
+
+    // A buffer replaced by a larger one is freed with the engine, not at once:
+    // hipFree waits for the whole device, which would hold this call behind
+    // work on the engine's other streams (a serving loop's result copies).
+    // Growth is geometric, so what is kept stays below the buffers' own size.
+    std::vector<void *> retired;
+    template <class T>
+    void retire(DBuf<T> &b) {
+        if (b.p) retired.push_back(b.p);
+        b.p = nullptr;  // count kept: the next size grows from it
     }
 
     void ensure_grouped(uint64_t records) {
@@ -1313,22 +1325,25 @@ struct cig_engine {
 
     // Rows that found no room: reclaim garbage (compaction), demote to the
     // host tier if HBM is still short, then append their adds again.
+    DBuf<RowCount> ovf_rows;  // descriptors and counts of the rows whose adds found no room
     bool recover_overflow(const MutArgs &m) {
         read_ctl();
         const uint32_t nover = hctl.p->overflow_cnt;
         if (!nover) return false;
         std::vector<uint32_t> list(nover);
         HIPCHK(hipMemcpyAsync(list.data(), overflow.p, nover * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        std::vector<uint64_t> cnts(nover);
-        for (uint32_t i = 0; i < nover; ++i)
-            HIPCHK(hipMemcpyAsync(&cnts[i], cnt64.p + list[i], sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+        // only the overflowing rows' descriptors and counts cross to the host
+        // (a copy of every descriptor is 4 GB at scale 27)
+        grow(ovf_rows, nover);
+        k_gather_rows<<<grid_for(nover, 1), 256, 0, stream>>>(rows.p, cnt64.p, overflow.p, nover, ovf_rows.p);
+        HIPCHK(hipGetLastError());
+        std::vector<RowCount> rc(nover);
+        HIPCHK(hipMemcpyAsync(rc.data(), ovf_rows.p, nover * sizeof(RowCount), hipMemcpyDeviceToHost, stream));
         sync();
-        std::vector<Row> hrows(nrows);
-        HIPCHK(hipMemcpy(hrows.data(), rows.p, size_t(nrows) * sizeof(Row), hipMemcpyDeviceToHost));
         uint64_t need[2] = {0, 0};
         for (uint32_t i = 0; i < nover; ++i) {
-            const Row &r = hrows[list[i]];
-            need[(r.off & kHostTier) ? 1 : 0] += next_pow2_cap(uint64_t(r.deg) + uint32_t(cnts[i] >> 32));
+            const Row &r = rc[i].row;
+            need[(r.off & kHostTier) ? 1 : 0] += next_pow2_cap(uint64_t(r.deg) + uint32_t(rc[i].cnt >> 32));
         }
         for (int tier = 0; tier < 2; ++tier)
             if (need[tier]) compact_pool(tier, need[tier]);
@@ -1487,10 +1502,10 @@ struct cig_engine {
         }
         const uint64_t limit = tier == 0 && budget_slots ? std::min(cap, budget_slots) : cap;
         if (live + extra > limit || maxcnt > 0xFFFFFFFFull || maxpieces > 0xFFFFFFFFull) return false;
-        if (cmp_list.count < rowsn) cmp_list.alloc(rowsn);
-        if (cmp_jobs.count < maxcnt) cmp_jobs.alloc(maxcnt);
-        if (cmp_stage.count < maxcap) cmp_stage.alloc(maxcap);
-        if (cmp_pjob.count < maxpieces) cmp_pjob.alloc(maxpieces);
+        grow(cmp_list, rowsn);
+        grow(cmp_jobs, maxcnt);
+        grow(cmp_stage, maxcap);
+        grow(cmp_pjob, maxpieces);
         a.list = cmp_list.p;
         HIPCHK(hipMemcpyAsync(a.lfill, lbase.data(), kCmpMaxWin * 8, hipMemcpyHostToDevice, stream));
         k_cmp_scatter<<<chunks, 256, 0, stream>>>(a);
@@ -1938,7 +1953,7 @@ struct cig_engine {
     void traversal_batch(const cig_edge *d_dels, uint32_t nd, const cig_edge *d_adds, uint32_t na,
                          bool staged, cig_batch_stats *st) {
         if (graph_mode(staged)) return traversal_batch_graph<W, S>(d_dels, nd, d_adds, na, st);
-        const bool overlap = nd && !coop && !(staged && na);
+        const bool overlap = nd && !coop && !(staged && na) && cfg.n_stream != 1;
         // quick: the side stream's counters are reset and this call enqueued
         // nothing before its batch, so the grouping needs no fork
         const bool quick = overlap && side_ready && fresh_records;
@@ -2461,6 +2476,8 @@ struct cig_engine {
         if (hkeys.count < tab) {
             // table offsets are 32-bit: the growth stays below 2^32 slots
             const uint64_t n2 = std::min<uint64_t>(std::max(tab, hkeys.count + hkeys.count / 2), 0xFFFFFFFFull);
+            retire(hkeys);
+            retire(hcnts);
             hkeys.alloc(n2);
             hcnts.alloc(n2);
             fill32(hkeys.p, n2, 0);
@@ -3173,6 +3190,9 @@ struct cig_engine {
     }
 
     ~cig_engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
+        for (void *p : retired) (void)hipFree(p);
         if (side) {
             (void)hipStreamSynchronize(side);
             if (fork_ev) (void)hipEventDestroy(fork_ev);
@@ -3321,8 +3341,8 @@ struct cig_server {
         cig_status st = CIG_OK;
         try {
             if (nd > 0x7FFFFFFFull || na > 0x7FFFFFFFull) throw HipError(CIG_ELIMIT, "group too large");
-            cig_engine::grow(sd, nd);
-            cig_engine::grow(sa, na);
+            e->grow(sd, nd);
+            e->grow(sa, na);
             uint64_t od = 0, oa = 0;
             for (auto &r : batch) {
                 if (!r->dels.empty())

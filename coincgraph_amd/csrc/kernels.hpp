@@ -1072,6 +1072,20 @@ __global__ void k_clear_touched(MutArgs a, uint32_t *aff, const uint32_t *alist)
     }
 }
 
+// The descriptor and the delete/add counts of each row listed in `list`
+// (the rows whose adds found no room), for the host's compaction sizing.
+struct RowCount {
+    Row row;
+    uint64_t cnt;
+};
+__global__ void k_gather_rows(const Row *rows, const uint64_t *cnt64, const uint32_t *list, uint32_t cnt,
+                              RowCount *out) {
+    for (uint32_t i = gthread(); i < cnt; i += gthreads()) {
+        const uint32_t u = list[i];
+        out[i] = RowCount{rows[u], cnt64[u]};
+    }
+}
+
 // The affected-bit part of k_clear_touched alone.
 __global__ void k_clear_aff(const Ctl *c, uint32_t *aff, const uint32_t *alist) {
     const uint32_t cnt = ld_acquire(&c->acount);

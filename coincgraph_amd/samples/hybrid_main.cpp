@@ -58,7 +58,8 @@ void define_flags() {
     def("updatefile", "", "Update records: 'a|d src dst w' per line");
     def("update_size", "", "Batch sizes: 'adds dels' per line");
     def("cache", "4", "GPU edge cache budget in GiB (reference table for 0..4)");
-    def("n_stream", "3", "the number of stream (accepted; one device stream is used)");
+    def("n_stream", "3", "the number of streams (1: one compute stream; more: batches group their records on a "
+                         "second stream beside the invalidation waves)");
     def("SEGMENT", "1", "the number of segment (accepted, 1..512)");
     def("hybrid", "0", "accepted for compatibility (no effect in the reference either)");
     def("weight_num", "0", "accepted: graph-file weights are not used by the engine");
@@ -264,10 +265,10 @@ bool run() {
     cfg.cache_bytes = cache_bytes_from_flag(fd("cache"));
     cfg.n_stream = uint32_t(fi("n_stream"));
     cfg.segments = uint32_t(segments);
-    if (segments != 1 || fi("n_stream") != 3)
-        std::printf("[NOTE] --SEGMENT/--n_stream are accepted for compatibility: the engine runs each pass "
-                    "as one whole-graph launch on one compute stream (segments=%lld, n_stream=%lld)\n",
-                    segments, fi("n_stream"));
+    if (segments != 1)
+        std::printf("[NOTE] --SEGMENT is accepted for compatibility: the engine runs each pass as one "
+                    "whole-graph launch (segments=%lld)\n",
+                    segments);
     cfg.block_size = uint32_t(fi("block_size"));
     if (fb("symmetric")) cfg.flags |= CIG_CFG_SYMMETRIC;
     if (kApp == "pr") {

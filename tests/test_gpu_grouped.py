@@ -129,3 +129,66 @@ def test_each_form_matches_one_call_per_batch(algo, wmode):
         check_traversal(a, og, algo, 0, wmode, 5, where="each form")
         check_topology(a, og, "each form")
         assert a.batches_left() == 0
+
+
+@pytest.mark.parametrize("budget,n_stream", [(0.0, 3), (0.3, 3), (0.0, 1)])
+def test_two_stream_batches_interleaved_with_other_calls(budget, n_stream):
+    """A traversal batch groups its records on a second stream beside the
+    invalidation waves and clears its counts there at the end; the next
+    resident batch starts without waiting for the engine stream. Resident
+    batches interleaved with host-pointer batches, staged batches, grouped
+    batches, gathers, cache refreshes and a compaction must stay equal to the
+    oracle after every step (n_stream = 1: one compute stream, the same
+    results)."""
+    print(f"[gpu-case] two-stream interleave budget={budget} n_stream={n_stream}", flush=True)
+    g = cg.rmat_graph(12, 16, seed=47)
+    s = _conflict_stream(g, 12, 400, 48)
+    extra = _conflict_stream(g, 3, 300, 49)
+    og = oracle_graph(g)
+    cache = int(g.m * 4 * 2 * budget)
+    with cg.Engine("sssp", weight_mode=cg.WEIGHT_HASH64, weight_seed=7, cache_bytes=cache,
+                   n_stream=n_stream) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        e.load_updates(s)
+        j = 0
+
+        def resident(k=1, each=False):
+            nonlocal j
+            if each:
+                e.apply_next_batches(k, each=True)
+            elif k > 1:
+                e.apply_next_batches(k)
+            else:
+                e.apply_next_batch()
+            for b in range(j, j + k):
+                oracle_apply(og, *s.batch(b))
+            j += k
+
+        resident()
+        resident()  # consecutive resident batches (no fork)
+        check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 7, where="two resident")
+        d, a = extra.batch(0)
+        e.apply_batch(d, a)  # host records: copied on the engine stream first
+        oracle_apply(og, d, a)
+        resident()
+        check_topology(e, og, "after host batch")
+        e.cache_refresh()
+        resident(2, each=True)
+        d, a = extra.batch(1)
+        e.apply_batch(d, a, staged=True)
+        oracle_apply(og, d, a)
+        resident()
+        resident(3)  # grouped
+        check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 7, where="after grouped")
+        e.compact(0)
+        resident()
+        d, a = extra.batch(2)
+        e.apply_batch(d, a)
+        oracle_apply(og, d, a)
+        resident(2, each=True)
+        check_traversal(e, og, "sssp", 0, cg.WEIGHT_HASH64, 7, where="end")
+        check_topology(e, og, "end")
+        check_incoming(e, og, "end")
+        assert e.batches_left() == 0

@@ -34,3 +34,25 @@ def test_autotune_keeps_results(algo):
             e.apply_batch(dels, adds)
             oracle_apply(og, dels, adds)
             check_traversal(e, og, algo, 0, cg.WEIGHT_HASH64, 2, where=f"tuned batch {b}")
+
+
+@pytest.mark.parametrize("algo,adapt", [("sssp", 1), ("sssp", 4096), ("bfs", 16384), ("cc", 1), ("cc", 4096)])
+def test_adaptive_split_keeps_results(algo, adapt):
+    """cig_config.adapt_split only moves rows between in-round expansion and
+    next-round tiles: every setting (1 = off) gives the oracle's values after
+    the initial fixed point and after every batch, with long rows present
+    (scale 14 hubs hold thousands of slots)."""
+    print(f"[gpu-case] adapt_split {algo} {adapt}", flush=True)
+    g = cg.rmat_graph(14, 16, seed=73, symmetric=algo == "cc")
+    s = cg.rmat_stream(g, 14, 4, 1500, 1500, seed=74, symmetric=algo == "cc")
+    og = oracle_graph(g)
+    with cg.Engine(algo, weight_mode=cg.WEIGHT_HASH64, weight_seed=4, adapt_split=adapt) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        check_traversal(e, og, algo, 0, cg.WEIGHT_HASH64, 4, where=f"adapt {adapt} initial")
+        e.load_updates(s)
+        for b in range(s.nbatches):
+            e.apply_next_batch()
+            oracle_apply(og, *s.batch(b))
+            check_traversal(e, og, algo, 0, cg.WEIGHT_HASH64, 4, where=f"adapt {adapt} batch {b}")
