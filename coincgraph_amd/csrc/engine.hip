@@ -1988,7 +1988,13 @@ struct cig_engine {
             }
             mutate_group(m, side);
             HIPCHK(hipEventRecord(join_ev, side));
-            invalidate_and_reset(ia);
+            try {
+                invalidate_and_reset(ia);
+            } catch (...) {
+                // the grouping may still run: nothing later may touch its counters
+                (void)hipStreamSynchronize(side);
+                throw;
+            }
             mark(2);
             HIPCHK(hipStreamWaitEvent(stream, join_ev, 0));
             mutate_apply(m);
