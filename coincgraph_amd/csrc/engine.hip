@@ -302,6 +302,9 @@ struct cig_engine {
     HBuf<uint64_t> prog;  // round progress word written by the GPU (coherent, mapped)
     uint64_t *prog_dev = nullptr;
     uint64_t chain_epoch = 0;
+    HBuf<CtlPub> pub;     // the control block as a call's last kernel publishes it (coherent, mapped)
+    CtlPub *pub_dev = nullptr;
+    uint64_t pub_seq = 0;
 
     DBuf<uint32_t> hbm_pool;
     HBuf<uint32_t> host_pool;
@@ -371,6 +374,9 @@ struct cig_engine {
         prog.alloc(1, hipHostMallocCoherent | hipHostMallocMapped);
         *prog.p = 0;
         prog_dev = prog.device_ptr();
+        pub.alloc(1, hipHostMallocCoherent | hipHostMallocMapped);
+        std::memset(pub.p, 0, sizeof(CtlPub));
+        pub_dev = pub.device_ptr();
         stats.alloc(kStatShards);
         HIPCHK(hipMemsetAsync(stats.p, 0, stats.bytes(), stream));
         std::memset(hctl.p, 0, sizeof(Ctl));
@@ -1148,9 +1154,21 @@ struct cig_engine {
         if (!quick) HIPCHK(hipEventRecord(ev[0], stream));
     }
 
+    // Time between two recorded events. A call that ends by polling its
+    // published control block has not synchronised the stream: an event the
+    // runtime has not seen complete yet is waited for first.
+    static void event_ms(hipEvent_t a, hipEvent_t b, float *ms) {
+        hipError_t r = hipEventElapsedTime(ms, a, b);
+        if (r == hipErrorNotReady) {
+            HIPCHK(hipEventSynchronize(b));
+            r = hipEventElapsedTime(ms, a, b);
+        }
+        HIPCHK(r);
+    }
+
     float elapsed(int a, int b) {
         float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+        event_ms(ev[a], ev[b], &ms);
         return ms;
     }
 
@@ -1159,10 +1177,33 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
     }
 
+    // The control block at the end of a call: published by the stream's last
+    // kernel into mapped host memory and read by polling its sequence word
+    // (a copy plus a stream synchronisation costs tens of microseconds of
+    // wake-up between consecutive calls). When it returns, all of the call's
+    // work on the engine stream is done.
+    void read_ctl_published() {
+        const uint64_t seq = ++pub_seq;
+        k_publish_ctl<<<1, 64, 0, stream>>>(ctl.p, pub_dev, seq);
+        HIPCHK(hipGetLastError());
+        for (uint32_t spin = 0;; ++spin) {
+            if (__atomic_load_n(&pub.p->seq, __ATOMIC_ACQUIRE) == seq) break;
+            if ((spin & 1023) == 1023) {
+                const hipError_t q = hipStreamQuery(stream);
+                if (q != hipSuccess && q != hipErrorNotReady)
+                    throw HipError(CIG_EDEVICE, std::string("end of call: ") + hipGetErrorString(q));
+                if (q == hipSuccess && __atomic_load_n(&pub.p->seq, __ATOMIC_ACQUIRE) != seq)
+                    throw HipError(CIG_EINTERNAL, "end of call: control block not published");
+            }
+            __builtin_ia32_pause();
+        }
+        std::memcpy(hctl.p, &pub.p->c, sizeof(Ctl));
+    }
+
     void end_call(cig_batch_stats *st, int marks) {
         HIPCHK(hipEventRecord(ev[7], stream));
         reduce_stats();
-        read_ctl();
+        read_ctl_published();
         if (hctl.p->pr_invalid) throw HipError(CIG_EINTERNAL, "PR non-finite rank/residual");
         if (!is_pr() && !cfg.max_rounds) {
             const uint32_t r = hctl.p->round;
@@ -1183,7 +1224,7 @@ struct cig_engine {
         if (scan_timed) st->scan_kernel_ms = elapsed(8, 9);
         for (uint32_t i = 0; i < round_timed; ++i) {
             float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, rev_begin[i], rev_end[i]));
+            event_ms(rev_begin[i], rev_end[i], &ms);
             st->round_kernel_ms += ms;
         }
         st->round_launches = round_launches;

@@ -2420,6 +2420,25 @@ __global__ void k_reduce_stats(StatShard *stats, Ctl *c) {
     }
 }
 
+// End of a call: the control block goes to mapped pinned host memory, then
+// its sequence number (system scope, after a system fence), so the host
+// reads it by polling one word instead of a copy and a stream
+// synchronisation. One wave; every lane stores its share of the words.
+struct CtlPub {
+    Ctl c;
+    uint64_t seq;
+};
+__global__ void k_publish_ctl(const Ctl *c, CtlPub *pub, uint64_t seq) {
+    static_assert(sizeof(Ctl) % 8 == 0, "whole words");
+    constexpr uint32_t kWords = sizeof(Ctl) / 8;
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(c);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(&pub->c);
+    for (uint32_t i = threadIdx.x; i < kWords; i += blockDim.x) dst[i] = src[i];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Pool compaction: copies every row of `tier` to its new offset.
 __global__ void k_relocate_rows(const Row *old_rows, Row *new_rows, uint32_t n, uint32_t tier,
                                 const uint32_t *src_pool, uint32_t *dst_pool) {
