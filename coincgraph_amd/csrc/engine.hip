@@ -1888,6 +1888,7 @@ struct cig_engine {
         ~Captured() { reset(); }
     };
     Captured graph_mut, graph_repair;
+    Captured graph_group, graph_apply;  // the overlapped form: side-stream grouping, engine-stream rest
     DBuf<BatchRef> bref;
     uint64_t graph_captures = 0, graph_replays = 0;
 
@@ -1903,32 +1904,45 @@ struct cig_engine {
     }
 
     template <class F>
-    void graph_run(Captured &c, const std::vector<uint8_t> &sig, F body) {
+    void graph_run(Captured &c, const std::vector<uint8_t> &sig, F body, hipStream_t on = nullptr) {
+        if (!on) on = stream;
         if (!c.x || c.sig != sig) {
             c.reset();
-            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            HIPCHK(hipStreamBeginCapture(on, hipStreamCaptureModeThreadLocal));
             try {
                 body();
             } catch (...) {
                 hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(stream, &g);
+                (void)hipStreamEndCapture(on, &g);
                 if (g) (void)hipGraphDestroy(g);
                 throw;
             }
-            HIPCHK(hipStreamEndCapture(stream, &c.g));
+            HIPCHK(hipStreamEndCapture(on, &c.g));
             HIPCHK(hipGraphInstantiate(&c.x, c.g, nullptr, nullptr, 0));
             c.sig = sig;
             ++graph_captures;
         }
-        HIPCHK(hipGraphLaunch(c.x, stream));
+        HIPCHK(hipGraphLaunch(c.x, on));
         ++graph_replays;
     }
 
+    // The engine-stream graph of an overlapped batch, launched once both the
+    // invalidation chain and the side stream's grouping are done: the reset
+    // of the affected vertices, the deletes and adds, the pull repair and
+    // the insertion seeds.
     template <int W, bool S, int QPL>
-    void graph_repair_body(const InvalArgs &ia) {
+    void graph_apply_body(const MutArgs &mc, const InvalArgs &ic) {
+        k_reset_affected<<<grid_small, 256, 0, stream>>>(ic);
+        HIPCHK(hipGetLastError());
+        mutate_apply(mc);
+        graph_repair_body<W, S, QPL>(ic, false);
+    }
+
+    template <int W, bool S, int QPL>
+    void graph_repair_body(const InvalArgs &ia, bool reset = true) {
         InvalArgs a = ia;
         a.cfg = ecfg(kStageRepair);
-        k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
+        if (reset) k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
         k_repair_pull<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
         k_repair_pull_tiles<W, S, QPL><<<grid_small, kBlock, 0, stream>>>(a);
         k_insert_seed<W, S><<<grid_small, 256, 0, stream>>>(ia, 1u);  // no vertex is affected without deletes
@@ -1938,9 +1952,21 @@ struct cig_engine {
     template <int W, bool S>
     void traversal_batch_graph(const cig_edge *d_dels, uint32_t nd, const cig_edge *d_adds, uint32_t na,
                                cig_batch_stats *st) {
-        begin_call(st);
+        // with deletes and a second stream, the grouping replays on the side
+        // stream beside the invalidation chain (as traversal_batch runs it)
+        const bool overlap = nd && cfg.n_stream != 1;
+        const bool quick = overlap && side_ready && fresh_records;
+        fresh_records = false;
+        side_ready = false;
+        begin_call(st, quick);
         if (!bref.p) bref.alloc(1);
-        k_set_batch<<<1, 64, 0, stream>>>(bref.p, d_dels, nd, d_adds, na);
+        if (overlap && !quick) {
+            HIPCHK(hipEventRecord(fork_ev, stream));
+            HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+        }
+        // the side stream's graph reads the batch reference first; the engine
+        // stream's graph reads it after the join
+        k_set_batch<<<1, 64, 0, overlap ? side : stream>>>(bref.p, d_dels, nd, d_adds, na);
         HIPCHK(hipGetLastError());
         MutArgs m = mut_args(d_dels, nd, d_adds, na);
         InvalArgs ia = inval_args(d_dels, nd, d_adds, na);
@@ -1954,6 +1980,48 @@ struct cig_engine {
         sig_add(sig, ic);
         sig_add(sig, grid_small);
         sig_add(sig, qpl);
+        if (overlap) {
+            overlapped = true;
+            graph_run(graph_group, sig, [&] { mutate_group(mc, side); }, side);
+            HIPCHK(hipEventRecord(join_ev, side));
+            try {
+                k_inval_seed<<<grid_for(nd, 1), 256, 0, stream>>>(ia);
+                HIPCHK(hipGetLastError());
+                inval_waves += run_chain([&](uint64_t tag, uint32_t seq) {
+                              k_inval_round<1><<<grid_round, kBlock, 0, stream>>>(ia, prog_dev, tag, seq);
+                          },
+                          round_bound(), "invalidation");
+            } catch (...) {
+                (void)hipStreamSynchronize(side);
+                throw;
+            }
+            mark(2);
+            HIPCHK(hipStreamWaitEvent(stream, join_ev, 0));
+            graph_run(graph_apply, sig, [&] {
+                switch (qpl) {
+                case 1: graph_apply_body<W, S, 1>(mc, ic); break;
+                case 4: graph_apply_body<W, S, 4>(mc, ic); break;
+                default: graph_apply_body<W, S, 2>(mc, ic); break;
+                }
+            });
+            // deletes, adds and the repair replay as one graph: their stage
+            // times are reported together as the mutation's
+            mark(1);
+            mark(3);
+            mark(5);
+            relax();
+            mark(4);
+            end_call(st, 4);
+            if (na && hctl.p->overflow_cnt) {
+                recover_overflow(m);
+                ia.ref = bref.p;
+                insert_seed<W, S>(ia, false);
+                relax();
+                end_call(st, 4);
+            }
+            finish_split(m);
+            return;
+        }
         graph_run(graph_mut, sig, [&] {
             mutate_group(mc);
             mutate_apply(mc);

@@ -20,14 +20,15 @@ def _need_gpu():
 
 @pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_HASH64), ("bfs", 0), ("cc", 0)])
 @pytest.mark.parametrize("resident", [True, False])
-def test_graph_batches_match_oracle(algo, wmode, resident):
-    print(f"[gpu-case] graph batches {algo} resident={resident}", flush=True)
+@pytest.mark.parametrize("n_stream", [3, 1])  # 3: grouping graph on the second stream
+def test_graph_batches_match_oracle(algo, wmode, resident, n_stream):
+    print(f"[gpu-case] graph batches {algo} resident={resident} n_stream={n_stream}", flush=True)
     sym = algo == "cc"
     g = cg.rmat_graph(12, 16, seed=111, symmetric=sym)
     s = cg.rmat_stream(g, 12, 8, 400, 400, seed=112, symmetric=sym)
     og = oracle_graph(g)
     flags = cg.CFG_GRAPH_BATCHES | (cg.CFG_SYMMETRIC if sym else 0)
-    with cg.Engine(algo, weight_mode=wmode, weight_seed=31, flags=flags) as e, \
+    with cg.Engine(algo, weight_mode=wmode, weight_seed=31, flags=flags, n_stream=n_stream) as e, \
             cg.Engine(algo, weight_mode=wmode, weight_seed=31, flags=flags & ~cg.CFG_GRAPH_BATCHES) as eager:
         for x in (e, eager):
             x.load_graph(g)
