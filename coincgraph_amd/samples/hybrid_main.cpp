@@ -96,7 +96,7 @@ void define_flags() {
         def("pr_max_rounds", "100", "Maximum PR rounds per initialization/batch (1..100)");
         def("error", "0.000001", "Absolute signed PR residual threshold");
     }
-    // this build
+    // flags of this engine (not in the reference)
     def("weight_mode", "ref128", "SSSP weights: ref128 = (u+v)%128+1, hash64 = uniform [1,64]");
     def("weight_seed", "0", "seed of the hash64 weights");
     def("fused", "false", "apply each batch as one fused repair (no delete-stage check)", true);

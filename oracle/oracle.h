@@ -58,7 +58,7 @@ void og_export(const og_graph *g, uint64_t *offsets, uint32_t *dst);
 /* Reference mutation semantics. Records with an endpoint >= n are counted in
  * *invalid and ignored (the reference counts source >= n the same way,
  * source_local_chunk_store.h:258-264; it would append an out-of-range
- * destination, which this build rejects instead). Deletes remove, per source,
+ * destination, which the engine rejects instead). Deletes remove, per source,
  * the first min(requested, present) occurrences of each destination in row
  * order, keeping survivor order (BuildDeletionPlan + CompactSourceInPlace,
  * :670-780); unmatched deletes are counted in *missing. Adds are appended per
