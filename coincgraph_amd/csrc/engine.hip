@@ -1302,12 +1302,10 @@ struct cig_engine {
 
     void mutate_apply(const MutArgs &m) {
         if (m.nd) {
-            k_del_lane<<<grid_small, 256, 0, stream>>>(m);
+            k_del_lane_host<<<grid_small, kBlock, 0, stream>>>(m, grid_small / 2);
             k_del_rows<4, false><<<grid_small, kBlock, 0, stream>>>(m);
-            k_del_rows<2, true><<<grid_small, kBlock, 0, stream>>>(m);
             k_del_scan<<<grid_small, 256, 0, stream>>>(m);
-            k_del_fix_rows<<<grid_small, 256, 0, stream>>>(m);
-            k_del_fix<<<grid_small, 256, 0, stream>>>(m);
+            k_del_fix_both<<<grid_small, 256, 0, stream>>>(m, grid_small / 2);
         }
         k_add_plan<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
         k_add_copy<<<grid_small, kBlock, 0, stream>>>(m);

@@ -434,9 +434,9 @@ __device__ __forceinline__ void del_hole(const MutArgs &a, uint32_t row, uint32_
 }
 
 // Tiny rows: one lane per delete record.
-__global__ void k_del_lane(MutArgs a) {
+__device__ void del_lane_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
     const uint32_t total = rec_nd(a) << a.rev;
-    for (uint32_t i = gthread(); i < total; i += gthreads()) {
+    for (uint32_t i = bid * blockDim.x + threadIdx.x; i < total; i += nb * blockDim.x) {
         const MutRecord r = del_record(a, i);
         if (!r.valid || a.dclass[r.row] != kDelClassTiny) continue;
         const Row row = a.rows[r.row];
@@ -465,6 +465,8 @@ __global__ void k_del_lane(MutArgs a) {
     }
 }
 
+__global__ void k_del_lane(MutArgs a) { del_lane_body(a, blockIdx.x, gridDim.x); }
+
 // Medium rows (64 < deg <= 4096 slots, at most kDelWaveMax deletes): one
 // 16-lane group per row (four rows per wave) reads the row once, 256 slots
 // a step, against all of the row's requested destinations, which the
@@ -477,14 +479,15 @@ __global__ void k_del_lane(MutArgs a) {
 // what a step reads past the last claim costs host-link bandwidth, which
 // bounds them.
 template <uint32_t Q, bool HOST>
-__global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
+__device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
     constexpr uint32_t G = 16, GPW = kWave / G;
     static_assert(kDelWaveMax <= G, "a group holds its row's requests");
     static_assert(4 * Q <= 32, "candidate bits fit one word");
     const uint32_t cnt = ld_acquire(HOST ? &a.ctl->mid_host_cnt : &a.ctl->mid_cnt);
     const uint32_t lane = lane_id(), g = lane / G, gl = lane % G;
     const uint64_t gmask = ((1ull << G) - 1ull) << (g * G);
-    for (uint32_t base = gwave() * GPW; base < cnt; base += gwaves() * GPW) {
+    const uint32_t wpb = blockDim.x / kWave;
+    for (uint32_t base = (bid * wpb + wave_in_block()) * GPW; base < cnt; base += nb * wpb * GPW) {
         const uint32_t i = base + g;
         uint32_t u = 0, nd = 0, nq = 0, deg = 0;
         uint32_t *grp = nullptr, *p = nullptr;
@@ -570,11 +573,24 @@ __global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) {
     }
 }
 
+template <uint32_t Q, bool HOST>
+__global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) { del_rows_body<Q, HOST>(a, blockIdx.x, gridDim.x); }
+
+// The tiny-row claims and the host-tier medium rows in one launch (disjoint
+// rows): the first `split` workgroups take the records of tiny rows, the
+// others the host-tier rows, whose reads wait on the host link.
+__global__ __launch_bounds__(kBlock) void k_del_lane_host(MutArgs a, uint32_t split) {
+    if (blockIdx.x < split) del_lane_body(a, blockIdx.x, split);
+    else del_rows_body<2, true>(a, blockIdx.x - split, gridDim.x - split);
+}
+
 // Compaction of rows with at most kDelThreadFix deletes, one thread per row.
-__global__ void k_del_fix_rows(MutArgs a) {
+// bid / nb: this workgroup's index among the nb workgroups that share the
+// work (the whole grid, or its part in k_del_fix_both).
+__device__ void del_fix_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
     const uint32_t cnt = ld_acquire(&a.ctl->small_cnt);
     uint64_t applied = 0, missing = 0;
-    for (uint32_t t = gthread(); t < cnt; t += gthreads()) {
+    for (uint32_t t = bid * blockDim.x + threadIdx.x; t < cnt; t += nb * blockDim.x) {
         const uint32_t u = a.small_list[t];
         const uint32_t nd = uint32_t(a.cnt64[u]);
         const uint32_t k = a.hcnt[u];
@@ -620,6 +636,8 @@ __global__ void k_del_fix_rows(MutArgs a) {
     stat_add(&sh->deletes_applied, wave_sum64(applied));
     stat_add(&sh->missing, wave_sum64(missing));
 }
+
+__global__ void k_del_fix_rows(MutArgs a) { del_fix_rows_body(a, blockIdx.x, gridDim.x); }
 
 // Exclusive scan of a 0/1 flag over a 256-thread workgroup; *total gets the
 // sum. Contains barriers: every thread of the workgroup must call it.
@@ -729,12 +747,12 @@ __global__ __launch_bounds__(256) void k_del_scan(MutArgs a) {
 // workgroup per row: with k holes the row shrinks to deg-k and the live
 // slots of the last k positions move into the holes below deg-k (O(k)
 // stores). The row's table and hole counter are cleared for the next call.
-__global__ __launch_bounds__(256) void k_del_fix(MutArgs a) {
+__device__ void del_fix_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
     __shared__ uint32_t wsum[256 / kWave];
     const uint32_t cnt = ld_acquire(&a.ctl->big_cnt);
     const uint32_t tid = threadIdx.x;
     uint64_t applied = 0, missing = 0;
-    for (uint32_t j = blockIdx.x; j < cnt; j += gridDim.x) {
+    for (uint32_t j = bid; j < cnt; j += nb) {
         const uint32_t u = a.big_list[j];
         const Row r = a.rows[u];
         const uint32_t nd = uint32_t(a.cnt64[u]);
@@ -787,6 +805,16 @@ __global__ __launch_bounds__(256) void k_del_fix(MutArgs a) {
         stat_add(&sh->deletes_applied, applied);
         stat_add(&sh->missing, missing);
     }
+}
+
+__global__ __launch_bounds__(256) void k_del_fix(MutArgs a) { del_fix_body(a, blockIdx.x, gridDim.x); }
+
+// Both hole fixes in one launch (they own disjoint rows): the first `split`
+// workgroups take the rows fixed by one thread each, the others one row per
+// workgroup, so the short per-thread pass and the per-workgroup pass overlap.
+__global__ __launch_bounds__(256) void k_del_fix_both(MutArgs a, uint32_t split) {
+    if (blockIdx.x < split) del_fix_rows_body(a, blockIdx.x, split);
+    else del_fix_body(a, blockIdx.x - split, gridDim.x - split);
 }
 
 // Insert phase (PatchOrInvalidateCachedAdjacency's grow path,
