@@ -210,7 +210,9 @@ struct InvalidateOp {
         pay = 0;
         return true;
     }
-    __device__ uint64_t peek(uint32_t y) const { return state[y]; }
+    // a slot claimed by a delete of the same batch (kNone) has no state: the
+    // waves may run while the deletes claim (its parent field never equals x)
+    __device__ uint64_t peek(uint32_t y) const { return y != kNone ? state[y] : ~0ull; }
     __device__ bool edge(uint32_t x, uint32_t, uint32_t y, uint64_t cur) const {
         if (uint32_t(cur) != x) return false;
         // y's parent is x: y is a tree child reached once (twice only over

@@ -1300,11 +1300,22 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
     }
 
-    void mutate_apply(const MutArgs &m) {
+    // The delete claims: each delete record's slot is found and set to
+    // kNone (a tombstone) and listed as a hole; no live slot moves.
+    void mutate_claims(const MutArgs &m, hipStream_t on = nullptr) {
+        if (!on) on = stream;
+        if (!m.nd) return;
+        k_del_lane_host<<<grid_small, kBlock, 0, on>>>(m, grid_small / 2);
+        k_del_rows<4, false><<<grid_small, kBlock, 0, on>>>(m);
+        k_del_scan<<<grid_small, 256, 0, on>>>(m);
+        HIPCHK(hipGetLastError());
+    }
+
+    // claimed: mutate_claims ran already (the second stream of an overlapped
+    // batch); then only the hole fixes and the adds are left.
+    void mutate_apply(const MutArgs &m, bool claimed = false) {
         if (m.nd) {
-            k_del_lane_host<<<grid_small, kBlock, 0, stream>>>(m, grid_small / 2);
-            k_del_rows<4, false><<<grid_small, kBlock, 0, stream>>>(m);
-            k_del_scan<<<grid_small, 256, 0, stream>>>(m);
+            if (!claimed) mutate_claims(m);
             k_del_fix_both<<<grid_small, 256, 0, stream>>>(m, grid_small / 2);
         }
         k_add_plan<<<grid_small, 256, 0, stream>>>(m, add_list.p, &ctl.p->add_cnt);
@@ -1932,7 +1943,7 @@ struct cig_engine {
     void graph_apply_body(const MutArgs &mc, const InvalArgs &ic) {
         k_reset_affected<<<grid_small, 256, 0, stream>>>(ic);
         HIPCHK(hipGetLastError());
-        mutate_apply(mc);
+        mutate_apply(mc, true);
         graph_repair_body<W, S, QPL>(ic, false);
     }
 
@@ -1980,7 +1991,10 @@ struct cig_engine {
         sig_add(sig, qpl);
         if (overlap) {
             overlapped = true;
-            graph_run(graph_group, sig, [&] { mutate_group(mc, side); }, side);
+            graph_run(graph_group, sig, [&] {
+                mutate_group(mc, side);
+                mutate_claims(mc, side);
+            }, side);
             HIPCHK(hipEventRecord(join_ev, side));
             try {
                 k_inval_seed<<<grid_for(nd, 1), 256, 0, stream>>>(ia);
@@ -2083,17 +2097,21 @@ struct cig_engine {
             // x->y, and one whose edges x->y are all deleted is a seed
             // anyway, so the affected set is the one the mutated rows give.
             // The records are grouped (k_count / k_plan / k_scatter, which
-            // only read the rows) on the side stream while the waves run on
-            // the engine stream; the deletes, which rewrite rows, follow
-            // both. The grouping is enqueued first, so that its join event
-            // exists before the host drives the waves' chain and the deletes
-            // can follow the chain at once.
+            // only read the rows) and the deletes claim their slots (a slot
+            // becomes kNone; nothing moves) on the side stream while the
+            // waves run on the engine stream: a wave reads each slot either
+            // before its claim (the deleted edge x->y, whose y is a seed when
+            // it is a tree edge) or after it (kNone, which the waves skip).
+            // The hole fixes, which move slots, and the adds follow both. The
+            // side work is enqueued first, so that its join event exists
+            // before the host drives the waves' chain.
             overlapped = true;
             if (!quick) {
                 HIPCHK(hipEventRecord(fork_ev, stream));
                 HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
             }
             mutate_group(m, side);
+            mutate_claims(m, side);
             HIPCHK(hipEventRecord(join_ev, side));
             try {
                 invalidate_and_reset(ia);
@@ -2104,7 +2122,7 @@ struct cig_engine {
             }
             mark(2);
             HIPCHK(hipStreamWaitEvent(stream, join_ev, 0));
-            mutate_apply(m);
+            mutate_apply(m, true);
             mark(1);
         } else {
             mutate_group(m);
