@@ -243,6 +243,9 @@ def main(argv=None):
     ap.add_argument("--adapt-split", type=int, default=0,
                     help="frontier rounds keep rows of up to this many slots whole when waves take few "
                          "items (cig_config.adapt_split; 0 = the library default, 1 = off)")
+    ap.add_argument("--n-stream", type=int, default=0,
+                    help="compute streams (cig_config.n_stream: 1 one stream, 2 grouping beside the "
+                         "invalidation waves, 3 grouping and delete claims beside them; 0 = default 3)")
     ap.add_argument("--round-blocks", type=int, default=0,
                     help="workgroups per CU of the round kernels (cig_config.round_blocks_per_cu; 0 = default)")
     ap.add_argument("--split", type=int, default=0, help="tile split degree (cig_config.split_degree; 0 = default)")
@@ -314,6 +317,8 @@ def main(argv=None):
         kw["adapt_split"] = args.adapt_split
     if args.round_blocks:
         kw["round_blocks_per_cu"] = args.round_blocks
+    if args.n_stream:
+        kw["n_stream"] = args.n_stream
     if args.split:
         kw["split_degree"] = args.split
     if args.tile_len:

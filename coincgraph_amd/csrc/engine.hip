@@ -1287,6 +1287,11 @@ struct cig_engine {
         return m;
     }
 
+    // Overlapped batches: the deletes' claims run on the second stream beside
+    // the invalidation waves too (n_stream >= 3, the default), or only the
+    // grouping does (n_stream = 2).
+    bool claims_beside() const { return cfg.n_stream == 0 || cfg.n_stream >= 3; }
+
     void mutate_group(const MutArgs &m, hipStream_t on = nullptr) {
         if (!on) on = stream;
         // Grids sized by the call's records (all are grid-stride loops): a
@@ -1943,7 +1948,7 @@ struct cig_engine {
     void graph_apply_body(const MutArgs &mc, const InvalArgs &ic) {
         k_reset_affected<<<grid_small, 256, 0, stream>>>(ic);
         HIPCHK(hipGetLastError());
-        mutate_apply(mc, true);
+        mutate_apply(mc, claims_beside());
         graph_repair_body<W, S, QPL>(ic, false);
     }
 
@@ -1993,7 +1998,7 @@ struct cig_engine {
             overlapped = true;
             graph_run(graph_group, sig, [&] {
                 mutate_group(mc, side);
-                mutate_claims(mc, side);
+                if (claims_beside()) mutate_claims(mc, side);
             }, side);
             HIPCHK(hipEventRecord(join_ev, side));
             try {
@@ -2111,7 +2116,8 @@ struct cig_engine {
                 HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
             }
             mutate_group(m, side);
-            mutate_claims(m, side);
+            const bool side_claims = claims_beside();
+            if (side_claims) mutate_claims(m, side);
             HIPCHK(hipEventRecord(join_ev, side));
             try {
                 invalidate_and_reset(ia);
@@ -2122,7 +2128,7 @@ struct cig_engine {
             }
             mark(2);
             HIPCHK(hipStreamWaitEvent(stream, join_ev, 0));
-            mutate_apply(m, true);
+            mutate_apply(m, side_claims);
             mark(1);
         } else {
             mutate_group(m);

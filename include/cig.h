@@ -125,10 +125,11 @@ typedef struct cig_config {
     uint64_t weight_seed;     /* seed of CIG_WEIGHT_HASH64 */
     uint64_t cache_bytes;     /* HBM edge-cache budget; 0 = whole graph resident */
     int32_t device;           /* HIP device ordinal */
-    uint32_t n_stream;        /* compute streams: 1 = one stream; otherwise (default 3) a traversal
-                                 batch groups its records on a second stream beside the
-                                 invalidation waves (the reference spreads its segments' passes
-                                 over n_stream streams instead) */
+    uint32_t n_stream;        /* compute streams: 1 = one stream; 2 = a traversal batch groups its
+                                 records on a second stream beside the invalidation waves; 3 or
+                                 more (default 3) = the deletes also claim their slots there (the
+                                 reference spreads its segments' passes over n_stream streams
+                                 instead) */
     uint32_t segments;        /* accepted for flag compatibility (1..512) */
     uint32_t block_size;      /* threads per workgroup; 0 = tuned default */
     uint32_t edges_per_lane;  /* edges each lane keeps in flight: 4, 8 or 16 (whole 16-B quads); 0 = tuned default */
