@@ -1858,7 +1858,7 @@ struct cig_engine {
         HIPCHK(hipGetLastError());
     }
 
-    void invalidate_and_reset(const InvalArgs &ia, hipStream_t on = nullptr) {
+    void invalidate_and_reset(const InvalArgs &ia, hipStream_t on = nullptr, bool reset = true) {
         if (!on) on = stream;
         k_inval_seed<<<ia.ref ? grid_small : grid_for(ia.nd, 1), 256, 0, on>>>(ia);
         HIPCHK(hipGetLastError());
@@ -1875,7 +1875,7 @@ struct cig_engine {
                 },
                 round_bound(), "invalidation", on);
         }
-        k_reset_affected<<<grid_small, 256, 0, on>>>(ia);
+        if (reset) k_reset_affected<<<grid_small, 256, 0, on>>>(ia);
         HIPCHK(hipGetLastError());
     }
 
@@ -2120,14 +2120,17 @@ struct cig_engine {
             if (side_claims) mutate_claims(m, side);
             HIPCHK(hipEventRecord(join_ev, side));
             try {
-                invalidate_and_reset(ia);
+                invalidate_and_reset(ia, nullptr, false);
             } catch (...) {
                 // the grouping may still run: nothing later may touch its counters
                 (void)hipStreamSynchronize(side);
                 throw;
             }
-            mark(2);
+            // the join first: the host's wait call then overlaps the reset
             HIPCHK(hipStreamWaitEvent(stream, join_ev, 0));
+            k_reset_affected<<<grid_small, 256, 0, stream>>>(ia);
+            HIPCHK(hipGetLastError());
+            mark(2);
             mutate_apply(m, side_claims);
             mark(1);
         } else {

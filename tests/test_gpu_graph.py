@@ -20,7 +20,7 @@ def _need_gpu():
 
 @pytest.mark.parametrize("algo,wmode", [("sssp", cg.WEIGHT_HASH64), ("bfs", 0), ("cc", 0)])
 @pytest.mark.parametrize("resident", [True, False])
-@pytest.mark.parametrize("n_stream", [3, 1])  # 3: grouping graph on the second stream
+@pytest.mark.parametrize("n_stream", [3, 2, 1])  # 2, 3: grouping graph on the second stream
 def test_graph_batches_match_oracle(algo, wmode, resident, n_stream):
     print(f"[gpu-case] graph batches {algo} resident={resident} n_stream={n_stream}", flush=True)
     sym = algo == "cc"

@@ -131,7 +131,7 @@ def test_each_form_matches_one_call_per_batch(algo, wmode):
         assert a.batches_left() == 0
 
 
-@pytest.mark.parametrize("budget,n_stream", [(0.0, 3), (0.3, 3), (0.0, 1)])
+@pytest.mark.parametrize("budget,n_stream", [(0.0, 3), (0.3, 3), (0.0, 2), (0.0, 1)])
 def test_two_stream_batches_interleaved_with_other_calls(budget, n_stream):
     """A traversal batch groups its records on a second stream beside the
     invalidation waves and clears its counts there at the end; the next
