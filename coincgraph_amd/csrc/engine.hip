@@ -2479,12 +2479,43 @@ struct cig_engine {
             return;
         }
         MutArgs m = mut_args(d, nd, a, na);
-        mutate_group(m);
-        mutate_apply(m);
-        mark(1);
         const MsiArgs x = msi_args(d, nd, a, na);
-        if (nd) msi_deletes<W>(x);
-        else mark(2);
+        if (nd && cfg.n_stream != 1) {
+            // as traversal_batch: grouping and claims on the second stream
+            // beside the invalidation waves (MsiInvalOp skips claimed slots)
+            overlapped = true;
+            HIPCHK(hipEventRecord(fork_ev, stream));
+            HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+            mutate_group(m, side);
+            const bool side_claims = claims_beside();
+            if (side_claims) mutate_claims(m, side);
+            HIPCHK(hipEventRecord(join_ev, side));
+            try {
+                switch (qpl) {
+                case 1: msi_invalidate<1>(x); break;
+                case 4: msi_invalidate<4>(x); break;
+                default: msi_invalidate<2>(x); break;
+                }
+            } catch (...) {
+                (void)hipStreamSynchronize(side);
+                throw;
+            }
+            mark(2);
+            HIPCHK(hipStreamWaitEvent(stream, join_ev, 0));
+            mutate_apply(m, side_claims);
+            mark(1);
+            switch (qpl) {
+            case 1: msi_repair<W, 1>(x); break;
+            case 4: msi_repair<W, 4>(x); break;
+            default: msi_repair<W, 2>(x); break;
+            }
+        } else {
+            mutate_group(m);
+            mutate_apply(m);
+            mark(1);
+            if (nd) msi_deletes<W>(x);
+            else mark(2);
+        }
         mark(3);
         if (na) msi_insert_seed<W>(x, nd != 0);
         mark(5);

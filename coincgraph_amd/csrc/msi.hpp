@@ -122,6 +122,9 @@ struct MsiInvalOp {
     }
     __device__ uint64_t peek(uint32_t) const { return 0; }
     __device__ bool edge(uint32_t x, uint32_t, uint32_t y, uint64_t) const {
+        // a slot claimed by a delete of the same batch (the claims may run
+        // beside the waves) holds kNone and no vertex
+        if (y == kNone) return false;
         const unsigned long long *sy = st + uint64_t(y) * k;
         unsigned long long m = 0;
         for (unsigned long long b = snap[x]; b;) {
