@@ -125,8 +125,9 @@ typedef struct cig_config {
     uint64_t weight_seed;     /* seed of CIG_WEIGHT_HASH64 */
     uint64_t cache_bytes;     /* HBM edge-cache budget; 0 = whole graph resident */
     int32_t device;           /* HIP device ordinal */
-    uint32_t n_stream;        /* compute streams: 1 = one stream; 2 = a traversal batch groups its
-                                 records on a second stream beside the invalidation waves; 3 or
+    uint32_t n_stream;        /* compute streams: 1 = one stream; 2 = a traversal or multi-source
+                                 batch groups its records on a second stream beside the
+                                 invalidation waves; 3 or
                                  more (default 3) = the deletes also claim their slots there (the
                                  reference spreads its segments' passes over n_stream streams
                                  instead) */
