@@ -61,11 +61,14 @@ def test_multisource_batches_match_oracle(algo, wmode, k):
             _check_all(e, og, algo, sources, wmode, 11, f"batch {b}")
 
 
-@pytest.mark.parametrize("form", ["staged", "separate", "grouped", "host-tier", "planted"])
+@pytest.mark.parametrize("form", ["staged", "separate", "grouped", "host-tier", "planted",
+                                  "one-stream", "claims-on-engine-stream"])
 def test_multisource_forms(form):
     """Staged batches, separate delete/insert calls, the batched (grouped) form,
-    rows in the pinned host tier, and a stream with duplicate, missing and
-    out-of-range records all keep every source exact."""
+    rows in the pinned host tier, a stream with duplicate, missing and
+    out-of-range records, and batches whose grouping and claims run on the
+    engine stream (n_stream 1) or whose claims do (n_stream 2) all keep every
+    source exact."""
     print(f"[gpu-case] multi-source form {form}", flush=True)
     g = cg.rmat_graph(11, 16, seed=83)
     s = cg.rmat_stream(g, 11, 6, 300, 300, seed=84)
@@ -74,6 +77,10 @@ def test_multisource_forms(form):
     kw = dict(weight_mode=cg.WEIGHT_HASH64, weight_seed=12)
     if form == "host-tier":
         kw["cache_bytes"] = g.m * 4 * 2 // 3
+    if form == "one-stream":
+        kw["n_stream"] = 1
+    if form == "claims-on-engine-stream":
+        kw["n_stream"] = 2
     with cg.Engine("sssp", **kw) as e:
         e.load_graph(g)
         if form == "host-tier":
