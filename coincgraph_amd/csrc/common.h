@@ -69,6 +69,7 @@ struct Ctl {
     uint32_t touched;
     uint32_t small_cnt, big_cnt, add_cnt, copy_cnt, overflow_cnt;
     uint32_t mid_cnt, mid_host_cnt;  // medium rows with deletes: HBM tier, host tier
+    uint32_t mid16_cnt, mid_pad;     // HBM medium rows with 9-16 deletes
     uint32_t dtile_cnt, ht_cursor;  // long-row delete tiles and table slots
     uint32_t pr_invalid;
     uint32_t group_cursor;  // records grouped so far (< 2^32 per call)

@@ -272,7 +272,7 @@ struct cig_engine {
     uint64_t ptile_cap = 0;
     DBuf<uint64_t> cnt64, gofs;
     DBuf<uint32_t> rslot;  // per record of a call: rank within its row's deletes / adds
-    DBuf<uint32_t> touched, grouped, small_list, mid_list, big_list, add_list, overflow;
+    DBuf<uint32_t> touched, grouped, small_list, mid_list, mid16_list, big_list, add_list, overflow;
     DBuf<AddPiece> pieces;              // insert-phase copy pieces
     DBuf<StatShard> stats;              // sharded statistics (see k_reduce_stats)
     DBuf<uint32_t> htoff, hkeys, hcnt;  // long-row delete tables
@@ -733,6 +733,7 @@ struct cig_engine {
         touched.alloc(nrows);
         small_list.alloc(nrows);
         mid_list.alloc(nrows);
+        mid16_list.alloc(nrows);
         big_list.alloc(nrows);
         add_list.alloc(nrows);
         overflow.alloc(nrows);
@@ -1272,6 +1273,7 @@ struct cig_engine {
         m.grouped = grouped.p;
         m.small_list = small_list.p;
         m.mid_list = mid_list.p;
+        m.mid16_list = mid16_list.p;
         m.big_list = big_list.p;
         m.add_list = add_list.p;
         m.overflow = overflow.p;
@@ -1292,6 +1294,12 @@ struct cig_engine {
     // grouping does (n_stream = 2).
     bool claims_beside() const { return cfg.n_stream == 0 || cfg.n_stream >= 3; }
 
+    // A large batch (by its delete records, mirrors included) sends the HBM
+    // medium rows with 9-16 deletes to the 16-request pass; graph-captured
+    // calls, whose counts the host does not know, keep the plain form. The
+    // grouping and the claims of one call take the same form.
+    static bool mid16(const MutArgs &m) { return !m.ref && (uint64_t(m.nd) << m.rev) >= kMid16MinRecords; }
+
     void mutate_group(const MutArgs &m, hipStream_t on = nullptr) {
         if (!on) on = stream;
         // Grids sized by the call's records (all are grid-stride loops): a
@@ -1300,7 +1308,8 @@ struct cig_engine {
         // their record counts on the device and keep the full grid.
         const uint64_t recs = (uint64_t(m.nd) + m.na) << m.rev;
         k_count<<<m.ref ? grid_small : grid_for(recs, 4), 256, 0, on>>>(m);
-        k_plan<<<m.ref ? grid_small : grid_for(recs, 1), 256, 0, on>>>(m);
+        if (mid16(m)) k_plan<true><<<m.ref ? grid_small : grid_for(recs, 1), 256, 0, on>>>(m);
+        else k_plan<false><<<m.ref ? grid_small : grid_for(recs, 1), 256, 0, on>>>(m);
         k_scatter<<<m.ref ? grid_small : grid_for(recs, 1), 256, 0, on>>>(m);
         HIPCHK(hipGetLastError());
     }
@@ -1310,7 +1319,13 @@ struct cig_engine {
     void mutate_claims(const MutArgs &m, hipStream_t on = nullptr) {
         if (!on) on = stream;
         if (!m.nd) return;
-        k_del_lane_host<<<grid_small, kBlock, 0, on>>>(m, grid_small / 2);
+        // tiny rows, host-tier medium rows (and in a large batch the HBM
+        // medium rows with 9-16 deletes) in one launch, whose register needs
+        // are alike; the HBM medium rows with up to 8 deletes keep their own
+        if (mid16(m))
+            k_del_lane_host<true><<<grid_small + grid_small / 2, kBlock, 0, on>>>(m, grid_small / 2, grid_small);
+        else
+            k_del_lane_host<false><<<grid_small, kBlock, 0, on>>>(m, grid_small / 2, grid_small);
         k_del_rows<4, false><<<grid_small, kBlock, 0, on>>>(m);
         k_del_scan<<<grid_small, 256, 0, on>>>(m);
         HIPCHK(hipGetLastError());

@@ -47,7 +47,7 @@ __global__ void k_reset_ctl(Ctl *c, uint32_t flags) {
     if (!(flags & kResetKeepMut)) {
         c->touched = 0;
         c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
-        c->mid_cnt = c->mid_host_cnt = 0;
+        c->mid_cnt = c->mid_host_cnt = c->mid16_cnt = 0;
         c->dtile_cnt = c->ht_cursor = 0;
         c->group_cursor = 0;
     }
@@ -71,7 +71,7 @@ __global__ void k_reset_mut(Ctl *c) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     c->touched = 0;
     c->small_cnt = c->big_cnt = c->add_cnt = c->copy_cnt = c->overflow_cnt = 0;
-    c->mid_cnt = c->mid_host_cnt = 0;
+    c->mid_cnt = c->mid_host_cnt = c->mid16_cnt = 0;
     c->dtile_cnt = c->ht_cursor = 0;
     c->group_cursor = 0;
 }
@@ -189,6 +189,7 @@ struct MutArgs {
     uint32_t *small_list;
     uint32_t *mid_list;   // medium rows with deletes (k_del_rows): HBM-tier rows from the
                           // front, host-tier rows from the back (n << rev entries)
+    uint32_t *mid16_list; // HBM-tier medium rows with 9-16 deletes (n << rev entries)
     uint32_t *big_list;
     uint32_t *add_list;
     uint32_t *overflow;
@@ -220,8 +221,9 @@ __device__ __forceinline__ const cig_edge *rec_adds(const A &a) { return a.ref ?
 // source_local_chunk_store.h:670-723, restated for the device):
 //  - tiny rows (<= 64 slots): one lane per delete record reads the whole row
 //    and claims a matching slot with a CAS (slot := kNone);
-//  - medium rows (<= 4096 slots, <= kDelWaveMax deletes): one wave per row
-//    against all of the row's deletes at once, 1024 slots a step;
+//  - medium rows (<= 4096 slots, <= kDelWaveMax deletes; HBM rows with up to
+//    kDelWaveMax16 in a second kernel): a 16-lane group per row against all
+//    of the row's deletes at once, 256 slots a step;
 //  - other rows: tiles of kDelTile slots over the whole grid against the
 //    row's delete multiset in an open-addressing table of 2^bits >= 2*nd
 //    slots.
@@ -243,16 +245,28 @@ struct DelTile {
 };
 static_assert(sizeof(DelTile) == 32, "DelTile layout");
 constexpr uint32_t kDelTinyDeg = 64, kDelMediumDeg = 4096, kDelThreadFix = 16;
-enum : uint8_t { kDelClassNone = 0, kDelClassTiny = 1, kDelClassMedium = 2, kDelClassLong = 3 };
+enum : uint8_t { kDelClassNone = 0, kDelClassTiny = 1, kDelClassMedium = 2, kDelClassLong = 3, kDelClassMedium16 = 4 };
 
 // A medium row with many deletes is cheaper as one tile against its table
 // than as one wave per delete (each of which may read the whole row).
 constexpr uint32_t kDelWaveMax = 8;
+// In large batches, HBM-tier medium rows with more deletes, up to this many,
+// take a second medium-row pass whose groups hold 16 requests (one tile
+// against the row's table costs far more for a row of a few thousand
+// slots); the 8-request kernel keeps its registers. Batches of fewer delete
+// records (mirrors included) than kMid16MinRecords have few such rows and
+// keep the plain form, which saves them the extra classification and
+// workgroups.
+constexpr uint32_t kDelWaveMax16 = 16;
+constexpr uint64_t kMid16MinRecords = 1u << 20;
 
-__device__ __forceinline__ uint8_t del_class(uint32_t nd, uint32_t deg) {
+template <bool M16>
+__device__ __forceinline__ uint8_t del_class(uint32_t nd, uint32_t deg, bool host) {
     if (nd == 0) return kDelClassNone;
     if (deg <= kDelTinyDeg) return kDelClassTiny;
-    return deg <= kDelMediumDeg && nd <= kDelWaveMax ? kDelClassMedium : kDelClassLong;
+    if (deg > kDelMediumDeg) return kDelClassLong;
+    if (nd <= kDelWaveMax) return kDelClassMedium;
+    return M16 && nd <= kDelWaveMax16 && !host ? kDelClassMedium16 : kDelClassLong;
 }
 
 __device__ __forceinline__ uint32_t del_table_bits(uint32_t nd) {
@@ -343,14 +357,24 @@ __global__ void k_count(MutArgs a) {
 }
 
 // Reserves each touched row's group and classifies its work; long rows
-// with deletes also reserve their table and scan tiles. All seven cursors are
-// reserved together, one round trip per workgroup step.
+// with deletes also reserve their table and scan tiles. All cursors (eight,
+// nine with M16: the 16-request medium rows of a large batch) are reserved
+// together, one round trip per workgroup step.
+template <bool M16>
 __global__ void k_plan(MutArgs a) {
-    __shared__ uint32_t scratch[8 * (kMaxWavesPerBlock + 1)];
+    constexpr int NC = M16 ? 9 : 8;
+    __shared__ uint32_t scratch[NC * (kMaxWavesPerBlock + 1)];
     const uint32_t touched = ld_acquire(&a.ctl->touched);
-    uint32_t *const ctr[8] = {&a.ctl->group_cursor, &a.ctl->ht_cursor, &a.ctl->dtile_cnt,
-                              &a.ctl->small_cnt,    &a.ctl->big_cnt,    &a.ctl->add_cnt,
-                              &a.ctl->mid_cnt,      &a.ctl->mid_host_cnt};
+    uint32_t *ctr[NC];
+    ctr[0] = &a.ctl->group_cursor;
+    ctr[1] = &a.ctl->ht_cursor;
+    ctr[2] = &a.ctl->dtile_cnt;
+    ctr[3] = &a.ctl->small_cnt;
+    ctr[4] = &a.ctl->big_cnt;
+    ctr[5] = &a.ctl->add_cnt;
+    ctr[6] = &a.ctl->mid_cnt;
+    ctr[7] = &a.ctl->mid_host_cnt;
+    if constexpr (M16) ctr[8] = &a.ctl->mid16_cnt;
     for (uint32_t base = blockIdx.x * blockDim.x; base < touched; base += gthreads()) {
         const uint32_t t = base + threadIdx.x;
         uint32_t u = 0, nd = 0, na = 0, deg = 0;
@@ -364,18 +388,23 @@ __global__ void k_plan(MutArgs a) {
             const Row row = a.rows[u];
             deg = row.deg;
             host = (row.off & kHostTier) != 0;
-            cls = del_class(nd, deg);
+            cls = del_class<M16>(nd, deg, host);
             a.dclass[u] = cls;
         }
         const bool table = cls == kDelClassLong;
         const bool thread_fix = nd != 0 && nd <= kDelThreadFix && !table;
-        const uint32_t x[8] = {nd + na, table ? (1u << del_table_bits(nd)) : 0u,
-                               table ? (deg + kDelTile - 1) / kDelTile : 0u, thread_fix ? 1u : 0u,
-                               (nd != 0 && !thread_fix) ? 1u : 0u, na != 0 ? 1u : 0u,
-                               cls == kDelClassMedium && !host ? 1u : 0u,
-                               cls == kDelClassMedium && host ? 1u : 0u};
-        uint32_t at[8];
-        block_reserve_n<8>(x, ctr, at, scratch);
+        uint32_t x[NC];
+        x[0] = nd + na;
+        x[1] = table ? (1u << del_table_bits(nd)) : 0u;
+        x[2] = table ? (deg + kDelTile - 1) / kDelTile : 0u;
+        x[3] = thread_fix ? 1u : 0u;
+        x[4] = (nd != 0 && !thread_fix) ? 1u : 0u;
+        x[5] = na != 0 ? 1u : 0u;
+        x[6] = cls == kDelClassMedium && !host ? 1u : 0u;
+        x[7] = cls == kDelClassMedium && host ? 1u : 0u;
+        if constexpr (M16) x[8] = cls == kDelClassMedium16 ? 1u : 0u;
+        uint32_t at[NC];
+        block_reserve_n<NC>(x, ctr, at, scratch);
         if (t < touched) {
             a.gofs[u] = at[0];
             a.htoff[u] = table ? at[1] : kNone;
@@ -399,6 +428,8 @@ __global__ void k_plan(MutArgs a) {
             if (x[5]) a.add_list[at[5]] = u;
             if (x[6]) a.mid_list[at[6]] = u;
             if (x[7]) a.mid_list[(a.n << a.rev) - 1u - at[7]] = u;
+            if constexpr (M16)
+                if (x[8]) a.mid16_list[at[8]] = u;
         }
     }
 }
@@ -467,7 +498,8 @@ __device__ void del_lane_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
 
 __global__ void k_del_lane(MutArgs a) { del_lane_body(a, blockIdx.x, gridDim.x); }
 
-// Medium rows (64 < deg <= 4096 slots, at most kDelWaveMax deletes): one
+// Medium rows (64 < deg <= 4096 slots, at most K deletes: kDelWaveMax, or
+// kDelWaveMax16 for the HBM rows of the second kernel): one
 // 16-lane group per row (four rows per wave) reads the row once, 256 slots
 // a step, against all of the row's requested destinations, which the
 // group's lanes hold in registers (a destination requested twice is two
@@ -478,12 +510,14 @@ __global__ void k_del_lane(MutArgs a) { del_lane_body(a, blockIdx.x, gridDim.x);
 // per lane a step: 4 for HBM rows; host-tier rows take 2 (HOST), since
 // what a step reads past the last claim costs host-link bandwidth, which
 // bounds them.
-template <uint32_t Q, bool HOST>
+template <uint32_t Q, bool HOST, uint32_t K = kDelWaveMax>
 __device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
     constexpr uint32_t G = 16, GPW = kWave / G;
-    static_assert(kDelWaveMax <= G, "a group holds its row's requests");
+    static_assert(K <= G, "a group holds its row's requests");
+    static_assert(!(HOST && K > kDelWaveMax), "the 16-request rows are HBM rows");
+    constexpr bool M16 = K > kDelWaveMax;
     static_assert(4 * Q <= 32, "candidate bits fit one word");
-    const uint32_t cnt = ld_acquire(HOST ? &a.ctl->mid_host_cnt : &a.ctl->mid_cnt);
+    const uint32_t cnt = ld_acquire(M16 ? &a.ctl->mid16_cnt : HOST ? &a.ctl->mid_host_cnt : &a.ctl->mid_cnt);
     const uint32_t lane = lane_id(), g = lane / G, gl = lane % G;
     const uint64_t gmask = ((1ull << G) - 1ull) << (g * G);
     const uint32_t wpb = blockDim.x / kWave;
@@ -492,8 +526,8 @@ __device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
         uint32_t u = 0, nd = 0, nq = 0, deg = 0;
         uint32_t *grp = nullptr, *p = nullptr;
         if (i < cnt) {
-            u = a.mid_list[HOST ? (a.n << a.rev) - 1u - i : i];
-            nd = min(uint32_t(a.cnt64[u]), kDelWaveMax);
+            u = M16 ? a.mid16_list[i] : a.mid_list[HOST ? (a.n << a.rev) - 1u - i : i];
+            nd = min(uint32_t(a.cnt64[u]), K);
             const Row row = a.rows[u];
             deg = row.deg;
             nq = (deg + 3u) >> 2;
@@ -501,9 +535,9 @@ __device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
             p = a.pools.row(row.off);
         }
         const uint32_t mine = gl < nd ? grp[gl] : kNone;
-        uint32_t key[kDelWaveMax];
+        uint32_t key[K];
 #pragma unroll
-        for (uint32_t j = 0; j < kDelWaveMax; ++j) key[j] = __shfl(mine, int(g * G + j), kWave);
+        for (uint32_t j = 0; j < K; ++j) key[j] = __shfl(mine, int(g * G + j), kWave);
         // the most requests any of the wave's rows has: the match below
         // compares against that many keys only (most medium rows have one or
         // two deletes, so most waves skip most of the eight compares a slot)
@@ -525,7 +559,7 @@ __device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
                 }
                 // claimed and unused requests hold kNone, which no live slot holds
 #pragma unroll
-                for (uint32_t j = 0; j < kDelWaveMax; ++j) {
+                for (uint32_t j = 0; j < K; ++j) {
                     if (j >= wmax) break;  // wave-uniform
 #pragma unroll
                     for (uint32_t k = 0; k < Q; ++k)
@@ -548,7 +582,7 @@ __device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
                     const uint32_t x = d[b >> 2][b & 3u];
                     uint32_t m = 0;
 #pragma unroll
-                    for (uint32_t j = 0; j < kDelWaveMax; ++j) m |= (key[j] == x ? 1u : 0u) << j;
+                    for (uint32_t j = 0; j < K; ++j) m |= (key[j] == x ? 1u : 0u) << j;
                     m &= open;
                     if (m) {
                         claimed = uint32_t(__ffs(m) - 1);
@@ -561,7 +595,7 @@ __device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
                 if (claimed != kNone) {
                     open &= ~(1u << claimed);
 #pragma unroll
-                    for (uint32_t j = 0; j < kDelWaveMax; ++j)
+                    for (uint32_t j = 0; j < K; ++j)
                         if (j == claimed) key[j] = kNone;
                     if (gl == 0) grp[holes] = slot;  // the requests were read into registers above
                     ++holes;
@@ -573,15 +607,19 @@ __device__ void del_rows_body(const MutArgs &a, uint32_t bid, uint32_t nb) {
     }
 }
 
-template <uint32_t Q, bool HOST>
-__global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) { del_rows_body<Q, HOST>(a, blockIdx.x, gridDim.x); }
+template <uint32_t Q, bool HOST, uint32_t K = kDelWaveMax>
+__global__ __launch_bounds__(kBlock) void k_del_rows(MutArgs a) { del_rows_body<Q, HOST, K>(a, blockIdx.x, gridDim.x); }
 
 // The tiny-row claims and the host-tier medium rows in one launch (disjoint
-// rows): the first `split` workgroups take the records of tiny rows, the
-// others the host-tier rows, whose reads wait on the host link.
-__global__ __launch_bounds__(kBlock) void k_del_lane_host(MutArgs a, uint32_t split) {
+// rows): workgroups [0, split) take the records of tiny rows, [split,
+// split2) the host-tier rows, whose reads wait on the host link. With M16
+// (a large batch) the workgroups from split2 on take the HBM medium rows
+// with 9-16 deletes (a launch of their own would add a kernel boundary).
+template <bool M16>
+__global__ __launch_bounds__(kBlock) void k_del_lane_host(MutArgs a, uint32_t split, uint32_t split2) {
     if (blockIdx.x < split) del_lane_body(a, blockIdx.x, split);
-    else del_rows_body<2, true>(a, blockIdx.x - split, gridDim.x - split);
+    else if (!M16 || blockIdx.x < split2) del_rows_body<2, true>(a, blockIdx.x - split, split2 - split);
+    else if constexpr (M16) del_rows_body<4, false, kDelWaveMax16>(a, blockIdx.x - split2, gridDim.x - split2);
 }
 
 // Compaction of rows with at most kDelThreadFix deletes, one thread per row.

@@ -202,6 +202,37 @@ def test_multigraph_semantics(algo):
             check_incoming(e, og, f"multigraph batch {b}")
 
 
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
+def test_large_batch_medium_rows(algo):
+    """A batch of more than 2^20 delete records (mirrors included) sends the
+    medium rows (65-4096 slots) with 9-16 deletes to the 16-request claim
+    pass; a smaller batch sends them to the long-row scan. Both give the
+    oracle's results and topology."""
+    _announce(f"large batch {algo}")
+    g = cg.rmat_graph(18, 16, seed=91)
+    s = cg.rmat_stream(g, 18, 2, 700000, 300000, seed=92)
+    og = oracle_graph(g)
+    # the case exists: medium rows with 9-16 deletes in the first batch
+    deg = np.concatenate([np.diff(g.offsets).astype(np.int64), np.bincount(g.dst, minlength=g.n)])
+    dels0, _ = s.batch(0)
+    cnt = np.bincount(np.concatenate([dels0["u"].astype(np.int64), g.n + dels0["v"].astype(np.int64)]),
+                      minlength=2 * g.n)
+    assert np.count_nonzero((deg > 64) & (deg <= 4096) & (cnt > 8) & (cnt <= 16)) > 100
+    with cg.Engine(algo) as e:
+        e.load_graph(g)
+        e.init_graph(0)
+        e.start()
+        for b in range(s.nbatches):
+            dels, adds = s.batch(b)
+            if b == 1:
+                dels = dels[:100000]  # below the threshold: the plain form
+            st = e.apply_batch(dels, adds)
+            miss, inval = oracle_apply(og, dels, adds)
+            assert (st["missing_deletes"], st["invalid_updates"]) == (miss, inval), b
+            check_traversal(e, og, algo, 0, where=f"large batch {b}")
+            check_topology(e, og, f"large batch {b}")
+
+
 def test_empty_and_noop_batches():
     _announce("empty batches")
     g = cg.rmat_graph(10, 8, seed=3)
