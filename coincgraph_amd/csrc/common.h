@@ -1,5 +1,6 @@
 // Shared host/device declarations of libcig.
 #pragma once
+#include <cstddef>
 #include <cstdint>
 
 namespace cig {
@@ -50,29 +51,41 @@ struct CopyJob {
 };
 
 // Device control block: counters of the current batch and of the fixed-point
-// rounds. Lives in device memory; zeroed per batch by a fill kernel.
+// rounds. Lives in device memory; zeroed per batch by a fill kernel. The
+// counters that many workgroups of one kernel advance with atomics each
+// start a 128-B line of their own (kCtlLine): atomics to one line are
+// serialised where the line lives, and a workgroup step of k_plan advances
+// eight or nine cursors at once.
+constexpr size_t kCtlLine = 128;
 struct Ctl {
     // fixed-point rounds (relax / PR); slot = absolute round % 3
-    uint32_t qcount[3];
-    uint32_t tcount[3];
+    alignas(kCtlLine) uint32_t qcount[3];
+    alignas(kCtlLine) uint32_t tcount[3];
     uint32_t round;        // next absolute round to run
     uint32_t converged;
-    uint32_t done;         // workgroups finished in the current round kernel
+    alignas(kCtlLine) uint32_t done;  // workgroups finished in the current round kernel
     uint32_t iround;       // invalidation: first round of the running chain
     uint32_t istart[3];    // invalidation: where round slot k's frontier starts in the affected list
     uint32_t ifront[3];    // invalidation: frontier length of round slot k
     // invalidation
-    uint32_t acount;       // affected list length
-    uint32_t icount[3];    // invalidation tile counts
+    alignas(kCtlLine) uint32_t acount;  // affected list length
+    alignas(kCtlLine) uint32_t icount[3];  // invalidation tile counts
     uint32_t scount;       // one-shot scan tile count
     // mutation
-    uint32_t touched;
-    uint32_t small_cnt, big_cnt, add_cnt, copy_cnt, overflow_cnt;
-    uint32_t mid_cnt, mid_host_cnt;  // medium rows with deletes: HBM tier, host tier
-    uint32_t mid16_cnt, mid_pad;     // HBM medium rows with 9-16 deletes
-    uint32_t dtile_cnt, ht_cursor;  // long-row delete tiles and table slots
+    alignas(kCtlLine) uint32_t touched;
+    alignas(kCtlLine) uint32_t small_cnt;
+    alignas(kCtlLine) uint32_t big_cnt;
+    alignas(kCtlLine) uint32_t add_cnt;
+    alignas(kCtlLine) uint32_t copy_cnt;
+    alignas(kCtlLine) uint32_t overflow_cnt;
+    alignas(kCtlLine) uint32_t mid_cnt;       // medium rows with deletes: HBM tier
+    alignas(kCtlLine) uint32_t mid_host_cnt;  // and host tier
+    alignas(kCtlLine) uint32_t mid16_cnt;     // HBM medium rows with 9-16 deletes
+    uint32_t mid_pad;
+    alignas(kCtlLine) uint32_t dtile_cnt;     // long-row delete tiles
+    alignas(kCtlLine) uint32_t ht_cursor;     // and their table slots
     uint32_t pr_invalid;
-    uint32_t group_cursor;  // records grouped so far (< 2^32 per call)
+    alignas(kCtlLine) uint32_t group_cursor;  // records grouped so far (< 2^32 per call)
     uint32_t pl_tiles;      // PR pull plan: tiles of long incoming rows
     uint32_t pl_multi;      // PR pull plan: rows of more than one tile
     uint32_t pull_rounds;   // PR rounds of the current call run in pull form
@@ -80,7 +93,7 @@ struct Ctl {
     uint32_t amend_push;
     uint32_t bmin[3];       // delta-ordered rounds: smallest bucket of round slot k's frontier
     uint32_t ufsel[2];      // union-find repair: the sampled component left out of hooking, its votes
-    uint64_t pool_tail[2]; // bump tails of the HBM (0) and host (1) pools
+    alignas(kCtlLine) uint64_t pool_tail[2];  // bump tails of the HBM (0) and host (1) pools
     uint64_t pool_cap[2];
     uint64_t garbage[2];   // slots abandoned by relocations
     // statistics of the current call
